@@ -1,0 +1,203 @@
+"""Benchmark: RTM images/hour and G point-updates/s on one MI355X.
+
+Workload (default, "c1"): plain single-simulation RTM, 1701x351 model,
+h = 10 m, CPML 50 points on every side (padded 1801x451), dt = 0.8 ms,
+nt = 4000, Ricker 20 Hz at the surface centre (850, 2), 1701 receivers at
+iz = 2, FP32 stepping, the source wavefield stored every step (TS = 1) in
+FP32 on HBM (9.55 GB).  The velocity is a seeded 12-layer folded and
+faulted model with a 450 m water layer; the seismogram is forward-modelled
+on it (FP32, receiver extraction on the GPU) and passed through ZFP 0.5.5
+fixed-accuracy at tol 1e-4 on the host; the migration velocity is the true
+model smoothed (sigma 10 points) times (1 + 0.02 g), g a seeded Gaussian
+random field.
+
+One step = one full image: forward pass (nt steps, snapshot store) then
+backward pass (nt steps, receiver replacement, imaging), all inputs resident
+in HBM.  images/hour = 3600 / seconds per step;
+G updates/s = 2 nt X Z / seconds per step.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "RTM images/hour + Gpt-updates/s, batched vs plain, 1 GPU, image relL2<=1e-4"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c1_params():
+    import rtmuq2d_amd as R
+    nx, nz, T = 1701, 351, 50
+    return R.make_params(
+        nx=nx, nz=nz, thickness_left=T, thickness_right=T, thickness_up=T, thickness_down=T,
+        spacing_grid=10.0, sampling=0.0008, number_of_time_steps=4000, sampling_wavelet=4000,
+        first_location_shot=850, depth_shot=2, spacing_shot=1, number_of_shots=1,
+        first_location_receiver=0, depth_receiver=2, spacing_receiver=2, number_of_receivers=nx,
+        number_of_simulations=1, zfp_tolerance=1e-4, time_slices=1)
+
+
+def c1_inputs(p, seed=2024):
+    from rtmuq2d_amd import synth
+    v_true = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, seed)
+    v_mig = synth.migration_velocity(v_true, seed + 1)
+    w = synth.ricker(20.0, p.sampling, p.sampling_wavelet)
+    return v_true, v_mig, w
+
+
+def cpu_baseline(p, v_mig, w, seis):
+    """The reference's own rtm_routine (oracle/_ref, -O3 -march=native
+    OpenMP) on a bounded sample of the workload: the same 1801x451 padded
+    grid over the first 1000 time steps, on this host's cores."""
+    ref = os.path.join(REPO, "oracle", "_ref", "c1s_native", "ref_main")
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    nt_s = 1000
+    X, Z = p.X, p.Z
+    import rtmuq2d_amd as R
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as run:
+        for d in ("INPUTS", "OUTPUTS", "TEMPORARY_FILES"):
+            os.makedirs(os.path.join(run, d))
+        v_mig.astype(np.float32).tofile(os.path.join(run, "INPUTS", "vp_00001.bin"))
+        w[:nt_s].astype(np.float32).tofile(os.path.join(run, "INPUTS", "wavelet.bin"))
+        z = R.zfp_compress(np.ascontiguousarray(seis[:, :nt_s]), 1e-4)
+        with open(os.path.join(run, "INPUTS", "seismogram_00001.zfp"), "wb") as f:
+            f.write(z)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_SCHEDULE="static",
+                   OMP_PROC_BIND="close")
+        r = subprocess.run([ref, "1", "1"], cwd=run, env=env, capture_output=True, text=True,
+                           timeout=900)
+        if r.returncode != 0:
+            log("cpu baseline failed:", r.stdout[-500:], r.stderr[-500:])
+            return None
+        secs = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("REF_SECONDS")][0]
+    upd = 2.0 * nt_s * X * Z
+    sec_per_image = secs * (p.number_of_time_steps / nt_s)
+    return {
+        "value": round(3600.0 / sec_per_image, 3), "unit": "images/hour", "cores": cores,
+        "kind": "reference",
+        "gupdates_per_s": round(upd / secs / 1e9, 4),
+        "sample": (f"reference rtm_routine (gcc -O3 -march=native, OpenMP, disk snapshots every "
+                   f"20 steps) on the {X}x{Z} padded c1 grid for {nt_s} of {p.number_of_time_steps} "
+                   f"steps ({secs:.2f} s), scaled to a full image"),
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the RTM path shards by "
+              "independent simulations, so N GPUs run N independent copies.", file=sys.stderr)
+        sys.exit(2)
+
+    import rtmuq2d_amd as R
+    if R.lib().rtm_device_count() < 1:
+        log("no HIP device")
+        sys.exit(1)
+    p = c1_params()
+    X, Z, nt = p.X, p.Z, p.number_of_time_steps
+    log(f"[bench] c1 plain RTM {p.nx}x{p.nz} padded {X}x{Z} nt={nt} TS=1 FP32; "
+        f"plan footprint {R.footprint(p) / 2**30:.2f} GiB")
+    v_true, v_mig, w = c1_inputs(p)
+    plan = R.Plan(p)
+    # seismogram: forward on the true model with receiver extraction, then ZFP
+    plan.set_velocity(R.extend(p, v_true))
+    plan.set_wavelet(w)
+    t0 = time.time()
+    plan.forward(p.first_location_shot, p.depth_shot, record=True)
+    seis_true = plan.seismogram()
+    z = R.zfp_compress(seis_true, p.zfp_tolerance)
+    with tempfile.NamedTemporaryFile(suffix=".zfp", delete=False) as f:
+        f.write(z)
+    seis = R.zfp_decompress_file(f.name, p.number_of_receivers, nt, p.zfp_tolerance)
+    os.unlink(f.name)
+    log(f"[bench] seismogram modelled + ZFP ({len(z) / 1e6:.1f} MB) in {time.time() - t0:.2f} s")
+    plan.set_velocity(R.extend(p, v_mig))
+    plan.set_seismogram(seis)
+
+    def one_image():
+        plan.zero_image()
+        plan.migrate(p.first_location_shot, p.depth_shot)
+
+    for i in range(args.warmup):
+        one_image()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    plan.synchronize()
+    fwd_ms, bwd_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        one_image()
+        f_ms, b_ms, _ = plan.last_timing()
+        fwd_ms.append(f_ms)
+        bwd_ms.append(b_ms)
+        log(f"[bench] step {i + 1}/{args.steps}: fwd {f_ms * 1e3:.2f} us/step, bwd {b_ms * 1e3:.2f} us/step")
+    plan.synchronize()
+    secs = (time.perf_counter() - t0) / args.steps
+    I, _ = plan.image()
+    plan.close()
+
+    updates = 2.0 * nt * X * Z
+    step_ms = (np.mean(fwd_ms) + np.mean(bwd_ms)) / 2.0
+    bytes_per_launch = BYTES_PER_UPDATE * X * Z
+    achieved = bytes_per_launch / (step_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(3600.0 / secs, 3),
+        "unit": "images/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
+                        "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
+            "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
+        },
+        "gupdates_per_s": round(updates / secs / 1e9, 3),
+        "step_kernel_us": {"forward": round(float(np.mean(fwd_ms)) * 1e3, 3),
+                           "backward": round(float(np.mean(bwd_ms)) * 1e3, 3)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "image_norm": float(np.linalg.norm(I.astype(np.float64))),
+    }
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path on a bounded sample ...")
+        out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * rtm_hip.h -- C ABI of librtm_hip.so, the MI355X (gfx950) drop-in for the
+ * reverse-time-migration hot path of rtmUq2D.
+ *
+ * Two layers are exported, both with plain C types (no HIP or torch types):
+ *
+ *  1. The reference's own entry points, unchanged in name, signature, memory
+ *     ownership and observable behaviour.  Each cites the declaration it
+ *     replaces.  Sizes come from the process-wide rtm_params (set with
+ *     rtm_set_params; the defaults are SRC/fixed_parameters.h:8-70), just as
+ *     the reference reads them from its compile-time globals.
+ *
+ *  2. The device-resident API (rtm_plan_*, rtm_batch_*): wavefields, CPML
+ *     state and the source-snapshot store live on HBM for a whole
+ *     forward/backward pass; only inputs and images cross PCIe.  These have no
+ *     counterpart in the reference and return an int status (0 = success,
+ *     negative = error; the message is in rtm_last_error()).
+ *
+ * Layouts at this boundary are the reference's: padded fields [X][Z] with z
+ * fastest (SRC/stencilDFM.c:23), models and images [Nx][Nz] z fastest
+ * (SRC/image_conditions.c:25), seismograms [nrec][nt] t fastest
+ * (SRC/backward_modeling.c:109).  Device layouts are internal.
+ */
+#ifndef RTM_HIP_H
+#define RTM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTM_PATH_LEN 512
+
+/* Snapshot store precision (stepping is always FP32 in an FP32 plan). */
+enum rtm_snapshot_mode {
+    RTM_SNAP_FP32 = 0,      /* snapshot as computed                        */
+    RTM_SNAP_FP16 = 1,      /* FP16 with a per-snapshot max-abs scale      */
+    RTM_SNAP_BF16 = 2,      /* BF16 with a per-snapshot max-abs scale      */
+    RTM_SNAP_BOUNDARY = 3   /* boundary saving: 4-point ring, reconstruct  */
+};
+
+enum rtm_real { RTM_REAL_F32 = 0, RTM_REAL_F64 = 1 };
+
+/* Runtime form of SRC/fixed_parameters.h:8-70 plus TIME_SLICES
+ * (SRC/forward_modeling.h:4).  rtm_params_default() fills the reference's
+ * shipped values. */
+typedef struct rtm_params {
+    char velocity_file[RTM_PATH_LEN];        /* fixed_parameters.h:8  (printf pattern, sim id) */
+    char wavelet_file[RTM_PATH_LEN];         /* fixed_parameters.h:9  */
+    char seismogram_file[RTM_PATH_LEN];      /* fixed_parameters.h:10 (printf pattern, shot id) */
+    char stacked_file[RTM_PATH_LEN];         /* fixed_parameters.h:14 */
+    char stacked_laplacian_file[RTM_PATH_LEN]; /* fixed_parameters.h:15 */
+    float zfp_tolerance;                     /* fixed_parameters.h:19 */
+    float spacing_grid;                      /* h,  fixed_parameters.h:23 */
+    float sampling;                          /* dt, fixed_parameters.h:24 */
+    int number_of_simulations;               /* fixed_parameters.h:28 */
+    int nx, nz;                              /* fixed_parameters.h:31-32 */
+    int first_location_shot, depth_shot, spacing_shot, number_of_shots;           /* :37-40 */
+    int first_location_receiver, depth_receiver, spacing_receiver, number_of_receivers; /* :45-48 */
+    int number_of_time_steps;                /* fixed_parameters.h:53 */
+    int sampling_wavelet;                    /* fixed_parameters.h:57 */
+    float cut_frequency;                     /* fixed_parameters.h:58 (the CPML uses its own 60 Hz) */
+    int thickness_up, thickness_down, thickness_left, thickness_right; /* :67-70 */
+    int time_slices;                         /* TIME_SLICES, forward_modeling.h:4 */
+    /* device-side options, no reference counterpart */
+    int snapshot_mode;                       /* enum rtm_snapshot_mode */
+    int image_kahan;                         /* 1: Kahan-compensated FP32 image accumulation */
+    int device;                              /* HIP device ordinal */
+} rtm_params;
+
+void rtm_params_default(rtm_params *p);
+int rtm_set_params(const rtm_params *p);      /* process-wide params for the legacy symbols */
+int rtm_get_params(rtm_params *p);
+const char *rtm_last_error(void);             /* thread-local message of the last failure */
+int rtm_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * 1. Reference entry points (same names and signatures)
+ * ------------------------------------------------------------------------- */
+
+/* SRC/rtm_iso2d.h:11 -- one simulation: read inputs, shot loop, write images. */
+void rtm_routine(int simulationId);
+
+/* SRC/forward_modeling.h:11 -- source modeling; the snapshot store that the
+ * reference writes to TEMPORARY_FILES/sourceWavefield_%05d.bin is kept on HBM,
+ * keyed by simId, until adjointModeling(simId, ...) consumes it. */
+void isotropicAcousticModeling(int simId, int *shot, float *velocity, float *wavelet);
+
+/* SRC/backward_modeling.h:11 -- receiver modeling + imaging; images are
+ * accumulated (+=) into the caller's arrays; frees simId's snapshot store.
+ * Returns 1, as the reference does (SRC/backward_modeling.c:135,164). */
+int adjointModeling(int simId, int *shot, float *velocity, float *wavelet, float *seismogram,
+                    float *imageCondition, float *selfImageCondition);
+
+/* SRC/stencilDFM.h:15 -- one interior leapfrog step, host pointers (H2D, step, D2H). */
+void iso_acoustic_wave_equation(float *pond, float *waveField2, float *waveField3);
+
+/* SRC/absorbingBoundaryCondition.h:27 -- CPML damping profiles (host). */
+void attenuation_profile_cpml(float *a_Left, float *a_Right, float *a_Top, float *a_Bottom,
+                              float *b_Left, float *b_Right, float *b_Top, float *b_Bottom);
+
+/* SRC/absorbingBoundaryCondition.h:29 -- one CPML update, host pointers (H2D, step, D2H). */
+void aiso_attenuation_cpml(float *waveField1, float *waveField2, float *Coef,
+                           float *a_left, float *b_left, float *a_right, float *b_right,
+                           float *a_top, float *b_top, float *a_bottom, float *b_bottom,
+                           float *psi_borderLeft, float *psi_borderRight, float *psi_borderTop,
+                           float *psi_borderBottom, float *csi_borderLeft, float *csi_borderRight,
+                           float *csi_borderTop, float *csi_borderBottom);
+
+/* SRC/image_conditions.h:4-6 -- imaging conditions, host pointers. */
+void cross_correlation(float *imageCondition, float *sourceWavefield, float *receiverWavefield);
+void normalized_cross_correlation(float *imageCondition, float *selfImageCondition,
+                                  float *sourceWavefield, float *receiverWavefield);
+
+/* SRC/compression_tools.h:6,9 -- headerless ZFP 2-D fixed-accuracy streams
+ * (host, ZFP 0.5.5).  Decompression returns 0 on success, 1 on failure. */
+int zfp2d_lossyCompression(void **compression, float *array, int dimension1, int dimension2,
+                           float tolerance);
+int zfp2d_lossyDecompression(char *zfp_data_02, float *array, int dimension1, int dimension2,
+                             float tolerance);
+
+/* SRC/pi_file.h:5,8 -- raw float32 files; exit(1) on any error, as the reference. */
+int reading_file(char *path, float *matrix, int dim1, int dim2);
+int recording_file(char *path, float *matrix, int dim1, int dim2);
+
+/* SRC/generic_functions.h:5 -- pad the model with CPML strips by edge replication. */
+void extend_matrix(float *extendedModel, float *model);
+
+/* SRC/math_functions.h:15 -- 8th-order Laplacian post-filter of the image. */
+void laplacian_filter2D(float *lp, float *inputMatrix);
+
+/* ---------------------------------------------------------------------------
+ * 2. Device-resident API (no reference counterpart)
+ * ------------------------------------------------------------------------- */
+
+typedef struct rtm_plan rtm_plan;
+
+/* One simulation's device state: wavefields, coef, CPML memory variables,
+ * wavelet, seismogram, snapshot store and image.  real = enum rtm_real. */
+int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out);
+int rtm_plan_destroy(rtm_plan *plan);
+size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan allocates */
+
+int rtm_plan_set_velocity(rtm_plan *plan, const float *vpad);     /* padded [X][Z] */
+int rtm_plan_set_wavelet(rtm_plan *plan, const float *w, int n);
+int rtm_plan_set_seismogram(rtm_plan *plan, const float *d);      /* [nrec][nt] */
+int rtm_plan_zero_image(rtm_plan *plan);
+
+/* nt forward steps from rest with the source at model (shot_x, shot_z);
+ * stores snapshots; if record != 0 also records the seismogram on device. */
+int rtm_plan_forward(rtm_plan *plan, int shot_x, int shot_z, int record);
+/* nt backward steps from rest with receiver replacement and imaging;
+ * accumulates into the device image. */
+int rtm_plan_backward(rtm_plan *plan);
+/* forward + backward for one shot */
+int rtm_plan_migrate(rtm_plan *plan, int shot_x, int shot_z);
+
+int rtm_plan_get_seismogram(rtm_plan *plan, float *d);             /* [nrec][nt] */
+int rtm_plan_get_image(rtm_plan *plan, float *I, float *Is);      /* [Nx][Nz]; Is may be NULL */
+int rtm_plan_get_image_f64(rtm_plan *plan, double *I, double *Is);
+int rtm_plan_get_fields(rtm_plan *plan, float *u_prev, float *u_last);  /* padded [X][Z] */
+int rtm_plan_get_fields_f64(rtm_plan *plan, double *u_prev, double *u_last);
+int rtm_plan_synchronize(rtm_plan *plan);
+/* Average time per time step (ms) of the last forward and the last backward
+ * pass, from HIP events recorded on the plan's stream around the step
+ * launches; steps = time steps per pass. */
+int rtm_plan_last_timing(rtm_plan *plan, double *fwd_step_ms, double *bwd_step_ms, int *steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTM_HIP_H */

@@ -1,0 +1,299 @@
+"""rtmuq2d_amd -- MI355X-native drop-in for rtmUq2D's reverse-time-migration path.
+
+The product is ``lib/librtm_hip.so`` (C ABI declared in ``include/rtm_hip.h``):
+hand-written HIP kernels for gfx950 plus a host C layer exporting the
+reference's own entry points.  This module is a thin ctypes binding over that
+ABI for Python callers (tests, the benchmark, the serving loop).  There is no
+Python or CPU fallback for any computation: when the library is missing the
+import of a function raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "librtm_hip.so")
+ZFP_PATH = os.path.join(LIB_DIR, "libzfp.so")
+PATH_LEN = 512
+
+SNAP_FP32, SNAP_FP16, SNAP_BF16, SNAP_BOUNDARY = 0, 1, 2, 3
+REAL_F32, REAL_F64 = 0, 1
+
+
+class RtmParams(C.Structure):
+    """Mirror of ``rtm_params`` (include/rtm_hip.h); field order is the ABI."""
+
+    _fields_ = [
+        ("velocity_file", C.c_char * PATH_LEN),
+        ("wavelet_file", C.c_char * PATH_LEN),
+        ("seismogram_file", C.c_char * PATH_LEN),
+        ("stacked_file", C.c_char * PATH_LEN),
+        ("stacked_laplacian_file", C.c_char * PATH_LEN),
+        ("zfp_tolerance", C.c_float),
+        ("spacing_grid", C.c_float),
+        ("sampling", C.c_float),
+        ("number_of_simulations", C.c_int),
+        ("nx", C.c_int), ("nz", C.c_int),
+        ("first_location_shot", C.c_int), ("depth_shot", C.c_int),
+        ("spacing_shot", C.c_int), ("number_of_shots", C.c_int),
+        ("first_location_receiver", C.c_int), ("depth_receiver", C.c_int),
+        ("spacing_receiver", C.c_int), ("number_of_receivers", C.c_int),
+        ("number_of_time_steps", C.c_int),
+        ("sampling_wavelet", C.c_int),
+        ("cut_frequency", C.c_float),
+        ("thickness_up", C.c_int), ("thickness_down", C.c_int),
+        ("thickness_left", C.c_int), ("thickness_right", C.c_int),
+        ("time_slices", C.c_int),
+        ("snapshot_mode", C.c_int),
+        ("image_kahan", C.c_int),
+        ("device", C.c_int),
+    ]
+
+    # geometry helpers
+    @property
+    def X(self) -> int:
+        return self.thickness_left + self.nx + self.thickness_right
+
+    @property
+    def Z(self) -> int:
+        return self.thickness_up + self.nz + self.thickness_down
+
+    def copy(self) -> "RtmParams":
+        q = RtmParams()
+        C.pointer(q)[0] = self
+        return q
+
+    def update(self, **kw) -> "RtmParams":
+        for k, v in kw.items():
+            if isinstance(getattr(self, k), bytes):
+                v = v.encode() if isinstance(v, str) else v
+            setattr(self, k, v)
+        return self
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load librtm_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    P = C.POINTER
+    f32p, f64p, i32p = P(C.c_float), P(C.c_double), P(C.c_int)
+    plan_p = C.c_void_p
+    sig = {
+        "rtm_params_default": (None, [P(RtmParams)]),
+        "rtm_set_params": (C.c_int, [P(RtmParams)]),
+        "rtm_get_params": (C.c_int, [P(RtmParams)]),
+        "rtm_last_error": (C.c_char_p, []),
+        "rtm_device_count": (C.c_int, []),
+        "rtm_routine": (None, [C.c_int]),
+        "isotropicAcousticModeling": (None, [C.c_int, i32p, f32p, f32p]),
+        "adjointModeling": (C.c_int, [C.c_int, i32p, f32p, f32p, f32p, f32p, f32p]),
+        "iso_acoustic_wave_equation": (None, [f32p, f32p, f32p]),
+        "attenuation_profile_cpml": (None, [f32p] * 8),
+        "aiso_attenuation_cpml": (None, [f32p] * 19),
+        "cross_correlation": (None, [f32p] * 3),
+        "normalized_cross_correlation": (None, [f32p] * 4),
+        "zfp2d_lossyCompression": (C.c_int, [P(C.c_void_p), f32p, C.c_int, C.c_int, C.c_float]),
+        "zfp2d_lossyDecompression": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int, C.c_float]),
+        "reading_file": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
+        "recording_file": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
+        "extend_matrix": (None, [f32p, f32p]),
+        "laplacian_filter2D": (None, [f32p, f32p]),
+        "rtm_plan_create": (C.c_int, [P(RtmParams), C.c_int, P(plan_p)]),
+        "rtm_plan_destroy": (C.c_int, [plan_p]),
+        "rtm_plan_footprint": (C.c_size_t, [P(RtmParams), C.c_int]),
+        "rtm_plan_set_velocity": (C.c_int, [plan_p, f32p]),
+        "rtm_plan_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
+        "rtm_plan_set_seismogram": (C.c_int, [plan_p, f32p]),
+        "rtm_plan_zero_image": (C.c_int, [plan_p]),
+        "rtm_plan_forward": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int]),
+        "rtm_plan_backward": (C.c_int, [plan_p]),
+        "rtm_plan_migrate": (C.c_int, [plan_p, C.c_int, C.c_int]),
+        "rtm_plan_get_seismogram": (C.c_int, [plan_p, f32p]),
+        "rtm_plan_get_image": (C.c_int, [plan_p, f32p, f32p]),
+        "rtm_plan_get_image_f64": (C.c_int, [plan_p, f64p, f64p]),
+        "rtm_plan_get_fields": (C.c_int, [plan_p, f32p, f32p]),
+        "rtm_plan_get_fields_f64": (C.c_int, [plan_p, f64p, f64p]),
+        "rtm_plan_synchronize": (C.c_int, [plan_p]),
+        "rtm_plan_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class RtmError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().rtm_last_error()
+        raise RtmError(f"{what}: {msg.decode() if msg else 'error'} (rc={rc})")
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_params() -> RtmParams:
+    """The reference's shipped configuration (SRC/fixed_parameters.h:8-70)."""
+    p = RtmParams()
+    lib().rtm_params_default(C.byref(p))
+    return p
+
+
+def make_params(**kw) -> RtmParams:
+    return default_params().update(**kw)
+
+
+def set_params(p: RtmParams) -> None:
+    _check(lib().rtm_set_params(C.byref(p)), "rtm_set_params")
+
+
+def footprint(p: RtmParams, real: int = REAL_F32) -> int:
+    return int(lib().rtm_plan_footprint(C.byref(p), real))
+
+
+class Plan:
+    """One simulation's device-resident state (rtm_plan)."""
+
+    def __init__(self, p: RtmParams, real: int = REAL_F32):
+        self.p = p.copy()
+        self.real = real
+        h = C.c_void_p()
+        _check(lib().rtm_plan_create(C.byref(self.p), real, C.byref(h)), "rtm_plan_create")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().rtm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_velocity(self, vpad: np.ndarray) -> None:
+        v = np.ascontiguousarray(vpad, dtype=np.float32)
+        assert v.shape == (self.p.X, self.p.Z), v.shape
+        _check(lib().rtm_plan_set_velocity(self.h, fptr(v)), "set_velocity")
+
+    def set_wavelet(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().rtm_plan_set_wavelet(self.h, fptr(w), int(w.size)), "set_wavelet")
+
+    def set_seismogram(self, d: np.ndarray) -> None:
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        assert d.shape == (self.p.number_of_receivers, self.p.number_of_time_steps), d.shape
+        _check(lib().rtm_plan_set_seismogram(self.h, fptr(d)), "set_seismogram")
+
+    def zero_image(self) -> None:
+        _check(lib().rtm_plan_zero_image(self.h), "zero_image")
+
+    def forward(self, shot_x: int, shot_z: int, record: bool = False) -> None:
+        _check(lib().rtm_plan_forward(self.h, int(shot_x), int(shot_z), int(bool(record))), "forward")
+
+    def backward(self) -> None:
+        _check(lib().rtm_plan_backward(self.h), "backward")
+
+    def migrate(self, shot_x: int, shot_z: int) -> None:
+        _check(lib().rtm_plan_migrate(self.h, int(shot_x), int(shot_z)), "migrate")
+
+    def seismogram(self) -> np.ndarray:
+        d = np.empty((self.p.number_of_receivers, self.p.number_of_time_steps), np.float32)
+        _check(lib().rtm_plan_get_seismogram(self.h, fptr(d)), "get_seismogram")
+        return d
+
+    def image(self, f64: bool = False):
+        shape = (self.p.nx, self.p.nz)
+        if f64:
+            I, Is = np.empty(shape, np.float64), np.empty(shape, np.float64)
+            _check(lib().rtm_plan_get_image_f64(self.h, dptr(I), dptr(Is)), "get_image_f64")
+        else:
+            I, Is = np.empty(shape, np.float32), np.empty(shape, np.float32)
+            _check(lib().rtm_plan_get_image(self.h, fptr(I), fptr(Is)), "get_image")
+        return I, Is
+
+    def fields(self, f64: bool = False):
+        shape = (self.p.X, self.p.Z)
+        if f64:
+            a, b = np.empty(shape, np.float64), np.empty(shape, np.float64)
+            _check(lib().rtm_plan_get_fields_f64(self.h, dptr(a), dptr(b)), "get_fields_f64")
+        else:
+            a, b = np.empty(shape, np.float32), np.empty(shape, np.float32)
+            _check(lib().rtm_plan_get_fields(self.h, fptr(a), fptr(b)), "get_fields")
+        return a, b
+
+    def synchronize(self) -> None:
+        _check(lib().rtm_plan_synchronize(self.h), "synchronize")
+
+    def last_timing(self):
+        """(forward ms/step, backward ms/step, steps) of the last passes."""
+        f, b = C.c_double(), C.c_double()
+        n = C.c_int()
+        _check(lib().rtm_plan_last_timing(self.h, C.byref(f), C.byref(b), C.byref(n)), "last_timing")
+        return f.value, b.value, n.value
+
+
+def extend(p: RtmParams, model: np.ndarray) -> np.ndarray:
+    """extend_matrix for params p (SRC/generic_functions.c:5-48)."""
+    set_params(p)
+    m = np.ascontiguousarray(model, dtype=np.float32)
+    out = np.zeros((p.X, p.Z), np.float32)
+    lib().extend_matrix(fptr(out), fptr(m))
+    return out
+
+
+def zfp_compress(arr: np.ndarray, tol: float) -> bytes:
+    """Headerless ZFP fixed-accuracy stream of a [d1][d2] float array with the
+    reference's field convention (SRC/compression_tools.c:33)."""
+    a = np.ascontiguousarray(arr, dtype=np.float32)
+    d1, d2 = a.shape
+    buf = C.c_void_p()
+    n = lib().zfp2d_lossyCompression(C.byref(buf), fptr(a), d1, d2, C.c_float(tol))
+    if n <= 0 or not buf.value:
+        raise RtmError("zfp2d_lossyCompression failed: " + (lib().rtm_last_error() or b"").decode())
+    out = C.string_at(buf.value, n)
+    C.CDLL(None).free(buf)
+    return out
+
+
+def zfp_decompress_file(path: str, d1: int, d2: int, tol: float) -> np.ndarray:
+    a = np.zeros((d1, d2), np.float32)
+    rc = lib().zfp2d_lossyDecompression(path.encode(), fptr(a), d1, d2, C.c_float(tol))
+    if rc != 0:
+        raise RtmError(f"zfp2d_lossyDecompression({path}) failed")
+    return a

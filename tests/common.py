@@ -1,0 +1,129 @@
+"""Shared helpers for the test suite: configurations, seeded inputs in the
+reference's file formats, and runners for the reference build in oracle/_ref.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+import oracle as O  # noqa: E402  (test infrastructure)
+import build as oracle_build  # noqa: E402
+from rtmuq2d_amd import synth  # noqa: E402
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+REF_CONFIGS = oracle_build.REF_CONFIGS
+
+
+def ref_dir(name: str, flavour: str = "strict") -> str:
+    return os.path.join(oracle_build.REF_OUT, f"{name}_{flavour}")
+
+
+def ref_built(name: str, flavour: str = "strict") -> bool:
+    return os.path.exists(os.path.join(ref_dir(name, flavour), "librtmref.so"))
+
+
+def orc_params(cfg: dict, time_slices: int = 20) -> O.OrcParams:
+    return O.OrcParams(cfg["Nx"], cfg["Nz"], cfg["thicknessLeft"], cfg["thicknessRight"],
+                       cfg["thicknessUp"], cfg["thicknessDown"], cfg["spacingGrid"], cfg["sampling"],
+                       cfg["numberOfTimeStep"], cfg["sampligWavelet"], cfg["firstLocationReceiver"],
+                       cfg["depthReceiver"], cfg["spacingReceiver"], cfg["numberOfReceivers"],
+                       time_slices)
+
+
+def rtm_params(cfg: dict, time_slices: int = 20, **extra):
+    """rtmuq2d_amd.RtmParams for a reference configuration dict."""
+    import rtmuq2d_amd as R
+    p = R.make_params(
+        nx=cfg["Nx"], nz=cfg["Nz"], thickness_left=cfg["thicknessLeft"],
+        thickness_right=cfg["thicknessRight"], thickness_up=cfg["thicknessUp"],
+        thickness_down=cfg["thicknessDown"], spacing_grid=cfg["spacingGrid"],
+        sampling=cfg["sampling"], number_of_time_steps=cfg["numberOfTimeStep"],
+        sampling_wavelet=cfg["sampligWavelet"], first_location_shot=cfg["firstLocationShot"],
+        depth_shot=cfg["depthShot"], spacing_shot=cfg["spacingShot"],
+        number_of_shots=cfg["numberOfShots"], first_location_receiver=cfg["firstLocationReceiver"],
+        depth_receiver=cfg["depthReceiver"], spacing_receiver=cfg["spacingReceiver"],
+        number_of_receivers=cfg["numberOfReceivers"],
+        number_of_simulations=cfg["numberOfSimulations"], zfp_tolerance=cfg["zfp_tolerance"],
+        time_slices=time_slices,
+        velocity_file="./INPUTS/vp_%05d.bin", wavelet_file="./INPUTS/wavelet.bin",
+        seismogram_file="./INPUTS/seismogram_%05d.zfp",
+        stacked_file="./OUTPUTS/stackedBrute_%05d.bin",
+        stacked_laplacian_file="./OUTPUTS/stackedLaplacian_%05d.bin")
+    return p.update(**extra)
+
+
+def shot_positions(cfg: dict):
+    return [(cfg["firstLocationShot"] + s * cfg["spacingShot"], cfg["depthShot"])
+            for s in range(cfg["numberOfShots"])]
+
+
+def make_inputs(cfg: dict, seed: int = 7):
+    """Seeded inputs of one configuration:
+    true model, per-simulation migration models, wavelet, and per-shot
+    seismograms forward-modelled (oracle, FP32) on the true model."""
+    nx, nz = cfg["Nx"], cfg["Nz"]
+    dt, nt, nw = cfg["sampling"], cfg["numberOfTimeStep"], cfg["sampligWavelet"]
+    f0 = 15.0 if dt >= 0.001 else 20.0
+    v_true = synth.layered_anomaly(nx, nz, seed, sigma=max(2.0, min(15.0, nx / 8)))
+    sm = max(1.0, min(10.0, nx / 12))
+    models = [synth.migration_velocity(v_true, seed + s, sigma_smooth=sm,
+                                       sigma_field=max(2.0, min(20.0, nx / 6)))
+              for s in range(1, cfg["numberOfSimulations"] + 1)]
+    wav = synth.ricker(f0, dt, nw)
+    op = orc_params(cfg)
+    vpad = O.extend(op, v_true)
+    seis = []
+    for sx, sz in shot_positions(cfg):
+        _, d, _, _ = O.forward(op, sx, sz, vpad, wav, snaps=False, record=True)
+        seis.append(d)
+    return v_true, models, wav, seis
+
+
+def write_inputs(run: str, cfg: dict, models, wav, seis_bytes):
+    os.makedirs(os.path.join(run, "INPUTS"), exist_ok=True)
+    os.makedirs(os.path.join(run, "OUTPUTS"), exist_ok=True)
+    os.makedirs(os.path.join(run, "TEMPORARY_FILES"), exist_ok=True)
+    for s, m in enumerate(models, start=1):
+        np.ascontiguousarray(m, np.float32).tofile(os.path.join(run, "INPUTS", f"vp_{s:05d}.bin"))
+    np.ascontiguousarray(wav, np.float32).tofile(os.path.join(run, "INPUTS", "wavelet.bin"))
+    for s, b in enumerate(seis_bytes, start=1):
+        with open(os.path.join(run, "INPUTS", f"seismogram_{s:05d}.zfp"), "wb") as f:
+            f.write(b)
+
+
+def read_outputs(run: str, cfg: dict, sim: int):
+    nx, nz = cfg["Nx"], cfg["Nz"]
+    b = np.fromfile(os.path.join(run, "OUTPUTS", f"stackedBrute_{sim:05d}.bin"), np.float32)
+    lap = np.fromfile(os.path.join(run, "OUTPUTS", f"stackedLaplacian_{sim:05d}.bin"), np.float32)
+    return b.reshape(nx, nz), lap.reshape(nx, nz)
+
+
+def run_reference(run: str, name: str, flavour: str = "strict", first: int = 1, last: int = 1,
+                  threads: int | None = None) -> float:
+    """Run the reference's rtm_routine for sims [first, last] in `run`;
+    returns its own wall-clock seconds."""
+    env = dict(os.environ)
+    env["OMP_SCHEDULE"] = "static"
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    exe = os.path.join(ref_dir(name, flavour), "ref_main")
+    r = subprocess.run([exe, str(first), str(last)], cwd=run, env=env, capture_output=True, text=True,
+                       timeout=1800)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for line in r.stdout.splitlines():
+        if line.startswith("REF_SECONDS"):
+            return float(line.split()[1])
+    raise AssertionError("no REF_SECONDS line: " + r.stdout)
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))

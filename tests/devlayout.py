@@ -1,0 +1,130 @@
+"""Device-layout driver of the step arithmetic for the CPU harness (tests).
+
+Mirrors, in numpy, what the HIP plan does between launches (state reset,
+ping-pong of u and psi, per-step mode bits, snapshot indices) and calls the
+per-point update of rtmuq2d_amd/csrc/rtm_step.h compiled for the CPU
+(tests/cpu_step_harness.cpp).  Used to prove that the device arithmetic is
+bit-identical to the oracle before any GPU run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import common
+import oracle as O
+
+M_STENCIL, M_CPML, M_SOURCE, M_INJECT, M_SNAP, M_IMAGE, M_KAHAN = 1, 2, 4, 8, 16, 32, 64
+
+
+def build_harness(outdir: str) -> C.CDLL:
+    src = os.path.join(common.REPO, "tests", "cpu_step_harness.cpp")
+    inc = os.path.join(common.REPO, "rtmuq2d_amd", "csrc")
+    so = os.path.join(outdir, "libstep_harness.so")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I", inc,
+                    src, "-o", so], check=True)
+    return C.CDLL(so)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class DevSim:
+    def __init__(self, hx, op: O.OrcParams, vpad, dtype=np.float32):
+        self.hx, self.op, self.dt = hx, op, np.dtype(dtype)
+        self.fn = hx.hx_step_f64 if self.dt == np.float64 else hx.hx_step_f32
+        X, Z = op.X, op.Z
+        self.X, self.Z, self.P = X, Z, (X + 63) // 64 * 64
+        self.C = self.to_dev(O.coef(op, vpad, dtype))
+        aL, aR, aT, aB, bL, bR, bT, bB = O.profiles(op, dtype)
+        self.ax = np.zeros(X, dtype); self.bx = np.zeros(X, dtype)
+        self.az = np.zeros(Z, dtype); self.bz = np.zeros(Z, dtype)
+        self.ax[:op.t_left], self.bx[:op.t_left] = aL, bL
+        self.ax[X - op.t_right:], self.bx[X - op.t_right:] = aR, bR
+        self.az[:op.t_up], self.bz[:op.t_up] = aT, bT
+        self.az[Z - op.t_down:], self.bz[Z - op.t_down:] = aB, bB
+        if self.dt == np.float32:
+            h = np.float32(op.h)
+            self.hs = np.array([np.float32(1) / h, h * h, np.float32(1) / (h * h), 0], np.float32)
+        else:
+            h = float(op.h)
+            self.hs = np.array([1.0 / h, h * h, 1.0 / (h * h), 0.0], np.float64)
+        self.recmap = np.full(X, -1, np.int32)
+        zr = op.t_up + op.depth_receiver
+        for r in range(op.n_receivers):
+            self.recmap[op.t_left + op.first_receiver + (op.spacing_receiver - 1) * r] = r
+        self.rec_row = zr
+
+    def to_dev(self, a):
+        d = np.zeros((self.Z, self.P), self.dt)
+        d[:, :self.X] = np.asarray(a).T
+        return d
+
+    def from_dev(self, d):
+        return np.ascontiguousarray(d[:, :self.X].T)
+
+    def reset(self):
+        z = lambda: np.zeros((self.Z, self.P), self.dt)  # noqa: E731
+        self.u = [z(), z()]
+        self.psiX = [z(), z()]
+        self.psiZ = [z(), z()]
+        self.csiX, self.csiZ = z(), z()
+        self.cur = self.pc = 0
+
+    def step(self, mode, src_j=-1, src_w=0.0, seis_row=None, snap=None, I=None, Is=None, Ic=None,
+             Isc=None, edges=False):
+        op = self.op
+        g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx, op.nz,
+                      mode, src_j, self.rec_row, op.n_receivers, int(edges)], np.int32)
+        hs = self.hs.copy()
+        hs[3] = src_w
+        c, pc = self.cur, self.pc
+        self.fn(_p(g), _p(hs), _p(self.u[c]), _p(self.u[c ^ 1]), _p(self.C), _p(self.ax), _p(self.bx),
+                _p(self.az), _p(self.bz), _p(self.psiX[pc]), _p(self.psiX[pc ^ 1]), _p(self.csiX),
+                _p(self.psiZ[pc]), _p(self.psiZ[pc ^ 1]), _p(self.csiZ), _p(self.recmap),
+                _p(seis_row), _p(snap), _p(I), _p(Is), _p(Ic), _p(Isc))
+        self.cur ^= 1
+        self.pc ^= 1
+
+    def forward(self, shot_x, shot_z, wav):
+        op = self.op
+        self.reset()
+        TS = op.time_slices
+        snaps = np.zeros((op.nt // TS, op.nz, op.nx), self.dt)
+        src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
+        for n in range(op.nt):
+            mode = M_STENCIL | M_CPML
+            w = 0.0
+            if n < op.n_wavelet:
+                mode |= M_SOURCE
+                w = wav[n]
+            snap = None
+            if n % TS == 0:
+                mode |= M_SNAP
+                snap = snaps[n // TS]
+            self.step(mode, src_j, w, snap=snap)
+        return snaps
+
+    def backward(self, seis, snaps, kahan=False):
+        op = self.op
+        self.reset()
+        TS = op.time_slices
+        seis_t = np.ascontiguousarray(np.asarray(seis, np.float32).T)
+        I = np.zeros((op.nz, op.nx), self.dt)
+        Is, Ic, Isc = np.zeros_like(I), np.zeros_like(I), np.zeros_like(I)
+        for n in range(op.nt):
+            tau = op.nt - n - 1
+            mode = M_STENCIL | M_CPML | M_INJECT
+            snap = None
+            if tau % TS == 0:
+                mode |= M_IMAGE | (M_KAHAN if kahan else 0)
+                snap = snaps[tau // TS]
+            self.step(mode, seis_row=seis_t[tau], snap=snap, I=I, Is=Is, Ic=Ic, Isc=Isc)
+        return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)
+
+    def last_fields(self):
+        return self.from_dev(self.u[self.cur ^ 1]), self.from_dev(self.u[self.cur])

@@ -1,0 +1,237 @@
+"""GPU parity of the HIP path against the oracle and the reference's golden
+outputs (FP32 bit-exact; FP64 bit-exact against the FP64 oracle).
+
+Everything goes through the C ABI of librtm_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import common
+import oracle as O
+import rtmuq2d_amd as R
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def golden(name):
+    return np.load(os.path.join(common.GOLDEN, f"{name}.npz"))
+
+
+@pytest.mark.parametrize("name", ["tiny", "asym"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_forward_backward_bitexact(gpu_available, name, dtype):
+    say(f"[gpu] forward/backward {name} {np.dtype(dtype).name}")
+    cfg = common.REF_CONFIGS[name]
+    g = golden(name)
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    S, d_orc, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=dtype, record=True)
+    real = R.REAL_F64 if dtype == np.float64 else R.REAL_F32
+    with R.Plan(p, real) as plan:
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.forward(sx, sz, record=True)
+        a, b = plan.fields(f64=dtype == np.float64)
+        assert np.array_equal(a, lu) and np.array_equal(b, lU), "forward fields differ"
+        if dtype == np.float32:
+            assert np.array_equal(plan.seismogram(), d_orc), "recorded seismogram differs"
+        d = g["seismograms"][0]
+        I, Is, lu2, lU2 = O.backward(op, vpad, d, S)
+        plan.set_seismogram(d)
+        plan.zero_image()
+        plan.backward()
+        a, b = plan.fields(f64=dtype == np.float64)
+        assert np.array_equal(a, lu2) and np.array_equal(b, lU2), "backward fields differ"
+        I2, Is2 = plan.image(f64=dtype == np.float64)
+        assert np.array_equal(I2, I), f"image relL2={common.rel_l2(I2, I)}"
+        assert np.array_equal(Is2, Is)
+
+
+@pytest.mark.parametrize("name", ["tiny", "asym"])
+def test_rtm_routine_matches_reference_golden(gpu_available, name):
+    """The drop-in rtm_routine on the reference's files reproduces the
+    reference's own outputs bit for bit."""
+    say(f"[gpu] rtm_routine {name}")
+    cfg = common.REF_CONFIGS[name]
+    g = golden(name)
+    p = common.rtm_params(cfg)
+    zs = [g[f"zfp_{s}"].tobytes() for s in range(cfg["numberOfShots"])]
+    with tempfile.TemporaryDirectory() as run:
+        common.write_inputs(run, cfg, list(g["models"]), g["wavelet"], zs)
+        cwd = os.getcwd()
+        os.chdir(run)
+        try:
+            R.set_params(p)
+            for s in range(1, cfg["numberOfSimulations"] + 1):
+                R.lib().rtm_routine(s)
+        finally:
+            os.chdir(cwd)
+        for s in range(cfg["numberOfSimulations"]):
+            b, lap = common.read_outputs(run, cfg, s + 1)
+            assert np.array_equal(b, g["brute"][s]), f"sim {s + 1}: relL2={common.rel_l2(b, g['brute'][s])}"
+            assert np.array_equal(lap, g["laplacian"][s])
+
+
+def test_legacy_modeling_symbols(gpu_available):
+    """isotropicAcousticModeling/adjointModeling keep the reference's calling
+    convention (images accumulated with +=) and match the oracle."""
+    say("[gpu] legacy modeling symbols")
+    cfg = common.REF_CONFIGS["tiny"]
+    g = golden("tiny")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    R.set_params(p)
+    vpad = O.extend(op, g["models"][1])
+    sx, sz = common.shot_positions(cfg)[1]
+    d = g["seismograms"][1]
+    I0 = np.random.default_rng(3).standard_normal((cfg["Nx"], cfg["Nz"])).astype(np.float32)
+    Is0 = np.abs(I0)
+    I_orc, Is_orc = O.rtm_shot(op, sx, sz, vpad, g["wavelet"], d)
+    shot = np.array([sx, sz], np.int32)
+    wav = np.ascontiguousarray(g["wavelet"], np.float32)
+    R.lib().isotropicAcousticModeling(5, R.iptr(shot), R.fptr(vpad), R.fptr(wav))
+    I, Is = I0.copy(), Is0.copy()
+    rc = R.lib().adjointModeling(5, R.iptr(shot), R.fptr(vpad), R.fptr(wav),
+                                 R.fptr(np.ascontiguousarray(d)), R.fptr(I), R.fptr(Is))
+    assert rc == 1
+    assert np.array_equal(I, I0 + I_orc)
+    assert np.array_equal(Is, Is0 + Is_orc)
+
+
+def _rand_fields(cfg, seed):
+    rng = np.random.default_rng(seed)
+    X = cfg["thicknessLeft"] + cfg["Nx"] + cfg["thicknessRight"]
+    Z = cfg["thicknessUp"] + cfg["Nz"] + cfg["thicknessDown"]
+    f = lambda: rng.standard_normal((X, Z)).astype(np.float32)  # noqa: E731
+    return X, Z, f
+
+
+@pytest.mark.parametrize("name", ["tiny", "asym"])
+def test_legacy_step_symbols(gpu_available, name):
+    """iso_acoustic_wave_equation, aiso_attenuation_cpml and the imaging
+    conditions, host-pointer form, against the oracle on random state."""
+    say(f"[gpu] legacy per-step symbols {name}")
+    cfg = common.REF_CONFIGS[name]
+    op = common.orc_params(cfg)
+    R.set_params(common.rtm_params(cfg))
+    X, Z, f = _rand_fields(cfg, 11)
+    Cf = (np.abs(f()) * 0.1).astype(np.float32)
+    u, U = f(), f()
+    ref = O.stencil(op, Cf, u, U)
+    mine = U.copy()
+    R.lib().iso_acoustic_wave_equation(R.fptr(Cf), R.fptr(u), R.fptr(mine))
+    assert np.array_equal(mine, ref)
+
+    prof = O.profiles(op)
+    TL, TR, TU, TD = cfg["thicknessLeft"], cfg["thicknessRight"], cfg["thicknessUp"], cfg["thicknessDown"]
+    rng = np.random.default_rng(5)
+    shapes = [(TL, Z), (TR, Z), (X, TU), (X, TD)]
+    psi = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    csi = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    psi_o, csi_o = [a.copy() for a in psi], [a.copy() for a in csi]
+    Uref = O.cpml(op, u, U, Cf, prof, psi_o, csi_o)
+    Umine = U.copy()
+    aL, aR, aT, aB, bL, bR, bT, bB = prof
+    R.lib().aiso_attenuation_cpml(R.fptr(u), R.fptr(Umine), R.fptr(Cf), R.fptr(aL), R.fptr(bL),
+                                  R.fptr(aR), R.fptr(bR), R.fptr(aT), R.fptr(bT), R.fptr(aB), R.fptr(bB),
+                                  *[R.fptr(a) for a in psi], *[R.fptr(a) for a in csi])
+    assert np.array_equal(Umine, Uref)
+    for a, b in zip(psi, psi_o):
+        assert np.array_equal(a, b)
+    for a, b in zip(csi, csi_o):
+        assert np.array_equal(a, b)
+
+    S, Rw = f(), f()
+    I0 = rng.standard_normal((cfg["Nx"], cfg["Nz"])).astype(np.float32)
+    I_ref, Is_ref = I0.copy(), I0.copy()
+    O.image(op, I_ref, Is_ref, S, Rw)
+    I_m, Is_m = I0.copy(), I0.copy()
+    R.lib().normalized_cross_correlation(R.fptr(I_m), R.fptr(Is_m), R.fptr(S), R.fptr(Rw))
+    assert np.array_equal(I_m, I_ref) and np.array_equal(Is_m, Is_ref)
+    I_m = I0.copy()
+    R.lib().cross_correlation(R.fptr(I_m), R.fptr(S), R.fptr(Rw))
+    assert np.array_equal(I_m, I_ref)
+
+
+def test_source_receiver_delta(gpu_available):
+    """One forward step from rest puts w[0]*C at the source and nothing else;
+    one backward step puts d[r, nt-1] at each receiver, bit for bit."""
+    say("[gpu] delta indexing")
+    cfg = dict(common.REF_CONFIGS["asym"])
+    cfg["numberOfTimeStep"] = 1
+    p = common.rtm_params(cfg, time_slices=1)
+    op = common.orc_params(cfg, time_slices=1)
+    rng = np.random.default_rng(2)
+    vpad = O.extend(op, rng.uniform(1500, 3000, (cfg["Nx"], cfg["Nz"])).astype(np.float32))
+    Cf = O.coef(op, vpad)
+    wav = rng.standard_normal(cfg["sampligWavelet"]).astype(np.float32)
+    sx, sz = 17, 9
+    with R.Plan(p) as plan:
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.forward(sx, sz)
+        _, U = plan.fields()
+        xs, zs = sx + cfg["thicknessLeft"], sz + cfg["thicknessUp"]
+        expect = np.zeros_like(U)
+        expect[xs, zs] = np.float32(0) + wav[0] * Cf[xs, zs]
+        assert np.array_equal(U, expect)
+        d = rng.standard_normal((cfg["numberOfReceivers"], 1)).astype(np.float32)
+        plan.set_seismogram(d)
+        plan.zero_image()
+        plan.backward()
+        _, U = plan.fields()
+        expect = np.zeros_like(U)
+        zr = cfg["thicknessUp"] + cfg["depthReceiver"]
+        for r in range(cfg["numberOfReceivers"]):
+            expect[cfg["thicknessLeft"] + cfg["firstLocationReceiver"] + (cfg["spacingReceiver"] - 1) * r, zr] = d[r, 0]
+        assert np.array_equal(U, expect)
+
+
+def _c0_inputs():
+    cfg = common.REF_CONFIGS["c0"]
+    v_true, models, wav, seis = common.make_inputs(cfg)
+    return cfg, v_true, models, wav, seis
+
+
+def test_c0_image_matches_reference(gpu_available):
+    """Config c0 (600x200, CPML 40, nt=2000, TS=20): the GPU FP32 image is
+    bit-identical to the reference's (strict build, checksums committed) and
+    within relL2 <= 1e-5 of the reference built -O3 -march=native."""
+    say("[gpu] c0 end-to-end")
+    meta = json.load(open(os.path.join(common.GOLDEN, "c0.json")))
+    native = np.load(os.path.join(common.GOLDEN, "c0_native.npz"))["brute"]
+    cfg, v_true, models, wav, seis = _c0_inputs()
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    assert sha(models[0]) == meta["models"][0], "input generator drifted"
+    zs = R.zfp_compress(seis[0], cfg["zfp_tolerance"])
+    assert hashlib.sha256(zs).hexdigest() == meta["zfp"][0], "seismogram or ZFP stream drifted"
+    p = common.rtm_params(cfg)
+    with tempfile.TemporaryDirectory() as run:
+        common.write_inputs(run, cfg, models[:1], wav, [zs])
+        cwd = os.getcwd()
+        os.chdir(run)
+        try:
+            R.set_params(p)
+            R.lib().rtm_routine(1)
+        finally:
+            os.chdir(cwd)
+        b, lap = common.read_outputs(run, cfg, 1)
+    assert sha(b) == meta["stackedBrute"][0], f"relL2 vs native {common.rel_l2(b, native)}"
+    assert sha(lap) == meta["stackedLaplacian"][0]
+    assert common.rel_l2(b, native) <= 1e-5

@@ -1,0 +1,126 @@
+"""The C-ABI library: loads without a GPU, exports every symbol declared in
+include/*.h, and its host-side pieces (params, profiles, padding, the
+Laplacian post-filter, raw files, ZFP) match the oracle / reference."""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import common
+import oracle as O
+import rtmuq2d_amd as R
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(common.REPO, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
+                names.add(m.group(1))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol():
+    names = declared_functions()
+    assert "rtm_routine" in names and "aiso_attenuation_cpml" in names and "rtm_plan_forward" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", R.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    L = R.lib()
+    for n in names:
+        getattr(L, n)
+
+
+def test_default_params_are_the_reference_configuration():
+    p = R.default_params()
+    assert (p.nx, p.nz, p.number_of_time_steps, p.sampling_wavelet) == (2721, 561, 14000, 800)
+    assert (p.thickness_left, p.thickness_right, p.thickness_up, p.thickness_down) == (50, 50, 50, 50)
+    assert (p.first_location_receiver, p.spacing_receiver, p.number_of_receivers) == (1, 3, 1361)
+    assert abs(p.spacing_grid - 6.25) == 0 and abs(p.sampling - np.float32(0.0005)) == 0
+    assert p.time_slices == 20
+    assert p.velocity_file == b"./INPUTS/vp_marmousi2_%05d.bin"
+
+
+def test_params_validation_errors():
+    bad = R.make_params(number_of_time_steps=1001, time_slices=20)
+    with pytest.raises(R.RtmError, match="multiple of time_slices"):
+        R.set_params(bad)
+    with pytest.raises(R.RtmError, match="thickness"):
+        R.set_params(R.make_params(thickness_left=6))
+    with pytest.raises(R.RtmError):
+        R.Plan(bad)   # rejected on the host, before any device call
+
+
+@pytest.mark.parametrize("name", ["tiny", "asym"])
+def test_host_profiles_padding_laplacian(name):
+    cfg = common.REF_CONFIGS[name]
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    R.set_params(p)
+    prof = [np.zeros_like(a) for a in O.profiles(op)]
+    R.lib().attenuation_profile_cpml(*[R.fptr(a) for a in prof])
+    for a, b in zip(prof, O.profiles(op)):
+        assert np.array_equal(a, b)
+    model = np.random.default_rng(0).uniform(1500, 4500, (cfg["Nx"], cfg["Nz"])).astype(np.float32)
+    assert np.array_equal(R.extend(p, model), O.extend(op, model))
+    img = np.random.default_rng(1).standard_normal((cfg["Nx"], cfg["Nz"])).astype(np.float32)
+    lp = np.zeros_like(img)
+    R.lib().laplacian_filter2D(R.fptr(lp), R.fptr(img))
+    assert np.array_equal(lp, O.laplacian(op, img))
+
+
+def test_zfp_roundtrip_and_reference_streams():
+    """The host codec reproduces the reference's committed seismogram
+    streams byte for byte and decodes them within the stated tolerance."""
+    g = np.load(os.path.join(common.GOLDEN, "tiny.npz"))
+    cfg = common.REF_CONFIGS["tiny"]
+    d = g["seismograms"][0]
+    z = R.zfp_compress(d, cfg["zfp_tolerance"])
+    assert z == g["zfp_0"].tobytes()
+    with tempfile.NamedTemporaryFile(delete=False) as f:
+        f.write(z)
+    try:
+        dd = R.zfp_decompress_file(f.name, *d.shape, cfg["zfp_tolerance"])
+    finally:
+        os.unlink(f.name)
+    assert np.max(np.abs(dd - d)) <= cfg["zfp_tolerance"]
+    assert not np.array_equal(dd, d)   # lossy: the decoded array is what the RTM consumes
+
+
+def test_zfp_missing_file_reports_failure():
+    a = np.zeros((4, 8), np.float32)
+    rc = R.lib().zfp2d_lossyDecompression(b"/nonexistent/x.zfp", R.fptr(a), 4, 8, C.c_float(1e-4))
+    assert rc == 1
+
+
+def test_raw_files_roundtrip(tmp_path):
+    a = np.random.default_rng(3).standard_normal((7, 5)).astype(np.float32)
+    path = str(tmp_path / "m.bin").encode()
+    assert R.lib().recording_file(path, R.fptr(a), 7, 5) == 0
+    b = np.zeros_like(a)
+    assert R.lib().reading_file(path, R.fptr(b), 7, 5) == 0
+    assert np.array_equal(a, b)
+    assert np.array_equal(np.fromfile(path.decode(), np.float32).reshape(7, 5), a)
+
+
+def test_short_file_exits_like_the_reference(tmp_path):
+    """reading_file on a short file exits the process with status 1
+    (SRC/pi_file.c:25-28); run in a child process."""
+    path = tmp_path / "short.bin"
+    np.zeros(3, np.float32).tofile(path)
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np, rtmuq2d_amd as R; "
+            "a = np.zeros(10, np.float32); R.lib().reading_file(%r, R.fptr(a), 2, 5)" %
+            (common.REPO, str(path).encode()))
+    r = subprocess.run(["python", "-c", code], capture_output=True, text=True)
+    assert r.returncode == 1 and "Error reading file" in r.stdout
