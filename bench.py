@@ -141,6 +141,22 @@ def main() -> None:
         plan.zero_image()
         plan.migrate(p.first_location_shot, p.depth_shot)
 
+    # tile height x graph block sweep (one image each); the fastest is timed
+    best = None
+    tuning = []
+    for rz in (1, 2, 4, 8):
+        for block in (1, 100):
+            plan.set_tiling(rz, block)
+            one_image()
+            f_ms, b_ms, _ = plan.last_timing()
+            tuning.append({"rows_per_thread": rz, "graph_block": block,
+                           "fwd_us": round(f_ms * 1e3, 3), "bwd_us": round(b_ms * 1e3, 3)})
+            log(f"[bench] tiling rz={rz} block={block}: fwd {f_ms * 1e3:.2f} us/step, bwd {b_ms * 1e3:.2f} us/step")
+            if best is None or f_ms + b_ms < best[0]:
+                best = (f_ms + b_ms, rz, block)
+    plan.set_tiling(best[1], best[2])
+    log(f"[bench] using rz={best[1]} graph_block={best[2]}")
+
     for i in range(args.warmup):
         one_image()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
@@ -179,7 +195,9 @@ def main() -> None:
             "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
                         "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
             "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
+            "tiling": {"rows_per_thread": best[1], "graph_block": best[2]},
         },
+        "tiling_sweep": tuning,
         "gupdates_per_s": round(updates / secs / 1e9, 3),
         "step_kernel_us": {"forward": round(float(np.mean(fwd_ms)) * 1e3, 3),
                            "backward": round(float(np.mean(bwd_ms)) * 1e3, 3)},

@@ -142,6 +142,11 @@ int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out);
 int rtm_plan_destroy(rtm_plan *plan);
 size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan allocates */
 
+/* Tile height of the step kernel (rows_per_thread in {1,2,4,8}: tiles of
+ * 64 x 4*rows_per_thread points) and the number of time steps per captured
+ * hipGraph block (<= 1: one eager launch per step).  Defaults: 4, 100. */
+int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block);
+
 int rtm_plan_set_velocity(rtm_plan *plan, const float *vpad);     /* padded [X][Z] */
 int rtm_plan_set_wavelet(rtm_plan *plan, const float *w, int n);
 int rtm_plan_set_seismogram(rtm_plan *plan, const float *d);      /* [nrec][nt] */

@@ -1,189 +1,26 @@
-// rtm_device.hip -- HIP device layer of librtm_hip.so (gfx950).
+// rtm_device.hip -- the plan (one simulation on HBM) and the per-step legacy
+// entry points of librtm_hip.so (gfx950).
 //
-// Holds one simulation's state on HBM (rtm_plan) and runs the forward and
+// A plan holds one simulation's state on HBM and runs the forward and
 // backward time loops of rtmUq2D (SRC/forward_modeling.c:98-130,
-// SRC/backward_modeling.c:97-126) with one fused kernel launch per time step.
-// Everything between the velocity/seismogram upload and the image download
-// stays on the device: the reference's disk snapshots
-// (TEMPORARY_FILES/sourceWavefield_%05d.bin) become an HBM snapshot store.
-#include <hip/hip_runtime.h>
-
+// SRC/backward_modeling.c:97-126) with one fused tiled-kernel launch per
+// time step, replayed from hipGraph blocks.  Everything between the
+// velocity/seismogram upload and the image download stays on the device: the
+// reference's disk snapshots (TEMPORARY_FILES/sourceWavefield_%05d.bin)
+// become an HBM snapshot store.
 #include <cmath>
 #include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
+#include <cstdlib>
 #include <mutex>
 #include <string>
-#include <vector>
 
-#include "rtm_internal.h"
-#include "rtm_step.h"
+#include "rtm_sim.h"
 
-using rtm::StepArgs;
+using namespace rtmd;
 
 namespace {
-
 thread_local std::string g_err;
-
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            rtm_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
-                          __FILE__, __LINE__);                                         \
-            return -1;                                                                 \
-        }                                                                              \
-    } while (0)
-
-constexpr int kBX = 64;   // one wavefront along x
-constexpr int kBZ = 4;
-
-inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
-// ---------------------------------------------------------------------------
-// kernels
-// ---------------------------------------------------------------------------
-
-template <typename R>
-__global__ void __launch_bounds__(kBX * kBZ) k_step(StepArgs<R> A)
-{
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int k = blockIdx.y * kBZ + threadIdx.y;
-    if (i < rtm::kHalf || i >= A.X - rtm::kHalf || k < rtm::kHalf || k >= A.Z - rtm::kHalf) return;
-    rtm::step_point(A, i, k);
 }
-
-// Top-strip correction of the outer columns i < 4 and i >= X-4
-// (rtm::cpml_top_edge_point); 8 columns x (TU-8) rows.
-template <typename R>
-__global__ void k_top_edges(StepArgs<R> A)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int rows = A.TU - 2 * rtm::kHalf;
-    if (t >= 8 * rows) return;
-    const int c = t % 8, k = rtm::kHalf + t / 8;
-    const int i = c < 4 ? c : A.X - 8 + c;
-    rtm::cpml_top_edge_point(A, i, k);
-}
-
-template <typename R>
-static void launch_top_edges(const StepArgs<R> &A, hipStream_t s)
-{
-    const int n = 8 * (A.TU - 2 * rtm::kHalf);
-    hipLaunchKernelGGL(k_top_edges<R>, dim3((n + 63) / 64), dim3(64), 0, s, A);
-}
-
-// coef = (v*aux)*(v*aux) on the padded velocity (already in device layout),
-// SRC/forward_modeling.c:79.
-template <typename R>
-__global__ void k_coef(const float *__restrict__ v, R *__restrict__ C, R aux, int X, int Z, int P)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (i >= X || k >= Z) return;
-    const size_t j = (size_t)k * P + i;
-    const R vv = R(v[j]);
-    C[j] = (vv * aux) * (vv * aux);
-}
-
-// Points the fused kernel does not cover (outside 4 <= i < X-4, 4 <= k < Z-4):
-// receiver replacement, SRC/backward_modeling.c:106-110.
-template <typename R>
-__global__ void k_inject_list(R *__restrict__ U, const int *__restrict__ jlist,
-                              const int *__restrict__ rlist, int n, const float *__restrict__ seis_row)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) U[jlist[t]] = R(seis_row[rlist[t]]);
-}
-
-// ... and source add, SRC/forward_modeling.c:105-109.
-template <typename R>
-__global__ void k_source_point(R *__restrict__ U, const R *__restrict__ C, int j, R w)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0) U[j] = U[j] + w * C[j];
-}
-
-// Seismogram recording (forward): d[n*nrec + r] = U at receiver r.
-template <typename R>
-__global__ void k_record(const R *__restrict__ U, const int *__restrict__ rec_j, int nrec,
-                         float *__restrict__ row)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nrec) row[r] = float(U[rec_j[r]]);
-}
-
-// Imaging on padded fields (legacy normalized_cross_correlation),
-// SRC/image_conditions.c:20-33; image layout [nz][nx] x fastest.
-template <typename R>
-__global__ void k_image_padded(R *__restrict__ I, R *__restrict__ Is, const R *__restrict__ S,
-                               const R *__restrict__ Rw, int nx, int nz, int TL, int TU, int P)
-{
-    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kk = blockIdx.y;
-    if (ii >= nx || kk >= nz) return;
-    const size_t j = (size_t)(kk + TU) * P + (ii + TL);
-    const size_t q = (size_t)kk * nx + ii;
-    I[q] = I[q] + S[j] * Rw[j];
-    if (Is) Is[q] = Is[q] + S[j] * S[j];
-}
-
-// ---------------------------------------------------------------------------
-// host-side layout helpers (reference [X][Z] z-fastest <-> device [Z][P])
-// ---------------------------------------------------------------------------
-
-template <typename Src, typename Dst>
-void to_device_layout(const Src *ref, Dst *dev, int X, int Z, int P)
-{
-    for (int k = 0; k < Z; k++) {
-        Dst *row = dev + (size_t)k * P;
-        for (int i = 0; i < X; i++) row[i] = Dst(ref[(size_t)i * Z + k]);
-        for (int i = X; i < P; i++) row[i] = Dst(0);
-    }
-}
-
-template <typename Src, typename Dst>
-void from_device_layout(const Src *dev, Dst *ref, int X, int Z, int P)
-{
-    for (int i = 0; i < X; i++)
-        for (int k = 0; k < Z; k++) ref[(size_t)i * Z + k] = Dst(dev[(size_t)k * P + i]);
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// plan
-// ---------------------------------------------------------------------------
-
-struct rtm_plan {
-    rtm_params p{};
-    int real = RTM_REAL_F32;
-    size_t esz = 4;
-    int X = 0, Z = 0, P = 0;
-    size_t nfield = 0, nint = 0, nsnap = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void *u[2] = {nullptr, nullptr};
-    void *C = nullptr;
-    void *psiX[2] = {nullptr, nullptr}, *psiZ[2] = {nullptr, nullptr};
-    void *csiX = nullptr, *csiZ = nullptr;
-    void *ax = nullptr, *bx = nullptr, *az = nullptr, *bz = nullptr;
-    int *recmap = nullptr;          // [X]
-    int *rec_j = nullptr;           // [nrec] device index of each receiver
-    int *out_j = nullptr, *out_r = nullptr;   // receivers outside the fused region
-    int n_out = 0;
-    float *wav = nullptr;           // [n_w]
-    float *seis_t = nullptr;        // [nt][nrec] (backward input, t-major)
-    float *seis_rec = nullptr;      // [nt][nrec] (forward recording)
-    void *snaps = nullptr;          // [nsnap][nz][nx]
-    void *I = nullptr, *Is = nullptr, *Ic = nullptr, *Isc = nullptr;
-    std::vector<float> wav_host;
-    int cur = 0;                    // u[cur] = u^n after a loop
-    int psi_cur = 0;
-    double fwd_ms = 0.0, bwd_ms = 0.0;   // per-pass event time
-    int last_steps = 0;
-    bool have_velocity = false, have_seis = false, have_snaps = false;
-};
 
 extern "C" void rtm_set_error(const char *fmt, ...)
 {
@@ -204,104 +41,46 @@ extern "C" int rtm_device_count(void)
     return n;
 }
 
-static size_t plan_bytes(const rtm_params &p, size_t esz, size_t *nsnap_out)
-{
-    const int X = p.thickness_left + p.nx + p.thickness_right;
-    const int Z = p.thickness_up + p.nz + p.thickness_down;
-    const size_t P = (size_t)round_up(X, 64);
-    const size_t nfield = P * Z;
-    const size_t nint = (size_t)p.nx * p.nz;
-    const size_t nsnap = (size_t)(p.number_of_time_steps / p.time_slices);
-    if (nsnap_out) *nsnap_out = nsnap;
-    size_t b = 0;
-    b += 9 * nfield * esz;                          // u0,u1,C,psiX x2,psiZ x2,csiX,csiZ
-    b += (size_t)(2 * X + 2 * Z) * esz;             // profiles
-    b += (size_t)X * 4 + 3 * (size_t)p.number_of_receivers * 4;
-    b += (size_t)p.sampling_wavelet * 4;
-    b += 2 * (size_t)p.number_of_time_steps * p.number_of_receivers * 4;
-    b += nsnap * nint * esz;
-    b += 4 * nint * esz;                            // I, Is, Ic, Isc
-    return b;
-}
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+
+struct rtm_plan {
+    SimBuffers b;
+    Exec ex;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int rz = 4;                     // tile rows per thread
+    int graph_block = 100;          // steps per captured graph block (<= 1: eager)
+    int last_nt = 0;                // steps of the last pass
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    int last_steps = 0;
+};
 
 extern "C" size_t rtm_plan_footprint(const rtm_params *p, int real)
 {
     if (!p || rtm_params_check(p)) return 0;
-    return plan_bytes(*p, real == RTM_REAL_F64 ? 8 : 4, nullptr);
+    return sim_bytes(*p, real == RTM_REAL_F64 ? 8 : 4);
 }
 
 static void plan_free(rtm_plan *q)
 {
-    void *all[] = {q->u[0], q->u[1], q->C, q->psiX[0], q->psiX[1], q->psiZ[0], q->psiZ[1],
-                   q->csiX, q->csiZ, q->ax, q->bx, q->az, q->bz, q->recmap, q->rec_j, q->out_j,
-                   q->out_r, q->wav, q->seis_t, q->seis_rec, q->snaps, q->I, q->Is, q->Ic, q->Isc};
-    for (void *ptr : all)
-        if (ptr) (void)hipFree(ptr);
+    q->ex.release();
+    q->b.release();
     if (q->ev0) (void)hipEventDestroy(q->ev0);
     if (q->ev1) (void)hipEventDestroy(q->ev1);
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
-template <typename R>
-static int upload_profiles(rtm_plan *q)
+static int plan_set_tiling(rtm_plan *q, int rz, int graph_block)
 {
-    const rtm_params &p = q->p;
-    const int TL = p.thickness_left, TR = p.thickness_right, TU = p.thickness_up,
-              TD = p.thickness_down;
-    std::vector<R> aL(TL), aR(TR), aT(TU), aB(TD), bL(TL), bR(TR), bT(TU), bB(TD);
-    if constexpr (sizeof(R) == 4)
-        rtm_profiles_f32(&p, aL.data(), aR.data(), aT.data(), aB.data(), bL.data(), bR.data(),
-                         bT.data(), bB.data());
-    else
-        rtm_profiles_f64(&p, aL.data(), aR.data(), aT.data(), aB.data(), bL.data(), bR.data(),
-                         bT.data(), bB.data());
-    // full-length profiles: x strips use the left/right arrays, z strips the
-    // top/bottom arrays, indexed as SRC/absorbingBoundaryCondition.c:181,273,227,320
-    std::vector<R> ax(q->X, R(0)), bx(q->X, R(0)), az(q->Z, R(0)), bz(q->Z, R(0));
-    for (int i = 0; i < TL; i++) { ax[i] = aL[i]; bx[i] = bL[i]; }
-    for (int i = 0; i < TR; i++) { ax[q->X - TR + i] = aR[i]; bx[q->X - TR + i] = bR[i]; }
-    for (int k = 0; k < TU; k++) { az[k] = aT[k]; bz[k] = bT[k]; }
-    for (int k = 0; k < TD; k++) { az[q->Z - TD + k] = aB[k]; bz[q->Z - TD + k] = bB[k]; }
-    HIP_TRY(hipMemcpy(q->ax, ax.data(), sizeof(R) * q->X, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(q->bx, bx.data(), sizeof(R) * q->X, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(q->az, az.data(), sizeof(R) * q->Z, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(q->bz, bz.data(), sizeof(R) * q->Z, hipMemcpyHostToDevice));
-    return 0;
-}
-
-// receiver tables: fused map on the receiver row, list of the rest, and the
-// device index of every receiver (for recording).  Positions follow
-// pos_r = first + (spacing-1)*r (SRC/backward_modeling.c:108); when several
-// receivers share a point the last one wins, as in the reference's loop.
-static int upload_receivers(rtm_plan *q)
-{
-    const rtm_params &p = q->p;
-    const int nrec = p.number_of_receivers;
-    const int zr = p.thickness_up + p.depth_receiver;
-    std::vector<int> recmap(q->X, -1), rec_j(nrec);
-    std::map<long, int> outside;   // device index -> last receiver
-    for (int r = 0; r < nrec; r++) {
-        const int col = p.thickness_left + p.first_location_receiver + (p.spacing_receiver - 1) * r;
-        if (col < 0 || col >= q->X || zr < 0 || zr >= q->Z) {
-            rtm_set_error("receiver %d at padded (%d,%d) is outside the %dx%d grid", r, col, zr,
-                          q->X, q->Z);
-            return -1;
-        }
-        rec_j[r] = zr * q->P + col;
-        const bool fused = col >= rtm::kHalf && col < q->X - rtm::kHalf && zr >= rtm::kHalf &&
-                           zr < q->Z - rtm::kHalf;
-        if (fused) recmap[col] = r;
-        else outside[(long)zr * q->P + col] = r;
-    }
-    HIP_TRY(hipMemcpy(q->recmap, recmap.data(), sizeof(int) * q->X, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(q->rec_j, rec_j.data(), sizeof(int) * nrec, hipMemcpyHostToDevice));
-    std::vector<int> oj, orr;
-    for (auto &kv : outside) { oj.push_back((int)kv.first); orr.push_back(kv.second); }
-    q->n_out = (int)oj.size();
-    if (q->n_out) {
-        HIP_TRY(hipMemcpy(q->out_j, oj.data(), sizeof(int) * q->n_out, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(q->out_r, orr.data(), sizeof(int) * q->n_out, hipMemcpyHostToDevice));
-    }
+    std::vector<int4> tiles;
+    append_tiles(tiles, 0, q->b.p, rz);
+    q->ex.release();
+    if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz)) return -1;
+    if (q->ex.set_tiles(tiles, q->stream)) return -1;
+    q->rz = rz;
+    q->graph_block = graph_block;
     return 0;
 }
 
@@ -315,63 +94,21 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
         return -1;
     }
     if (p->snapshot_mode != RTM_SNAP_FP32) {
-        rtm_set_error("rtm_plan_create: snapshot mode %d is not available in this build", p->snapshot_mode);
+        rtm_set_error("rtm_plan_create: snapshot mode %d is served by the batch API", p->snapshot_mode);
         return -1;
     }
     HIP_TRY(hipSetDevice(p->device));
     rtm_plan *q = new rtm_plan();
-    q->p = *p;
-    q->real = real;
-    q->esz = real == RTM_REAL_F64 ? 8 : 4;
-    q->X = p->thickness_left + p->nx + p->thickness_right;
-    q->Z = p->thickness_up + p->nz + p->thickness_down;
-    q->P = round_up(q->X, 64);
-    q->nfield = (size_t)q->P * q->Z;
-    q->nint = (size_t)p->nx * p->nz;
-    q->nsnap = (size_t)(p->number_of_time_steps / p->time_slices);
-    const size_t nrec = (size_t)p->number_of_receivers, nt = (size_t)p->number_of_time_steps;
-    auto fail = [&](void) { plan_free(q); delete q; return -1; };
-#define ALLOC(ptr, bytes)                                                                \
-    do {                                                                                 \
-        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 4);              \
-        if (e_ != hipSuccess) {                                                          \
-            rtm_set_error("hipMalloc(%zu) failed: %s", (size_t)(bytes), hipGetErrorString(e_)); \
-            return fail();                                                               \
-        }                                                                                \
-    } while (0)
+    auto fail = [&]() { plan_free(q); delete q; return -1; };
     if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&q->ev0) != hipSuccess || hipEventCreate(&q->ev1) != hipSuccess) {
         rtm_set_error("rtm_plan_create: stream/event creation failed");
         return fail();
     }
-    const size_t fb = q->nfield * q->esz;
-    ALLOC(q->u[0], fb); ALLOC(q->u[1], fb); ALLOC(q->C, fb);
-    ALLOC(q->psiX[0], fb); ALLOC(q->psiX[1], fb); ALLOC(q->psiZ[0], fb); ALLOC(q->psiZ[1], fb);
-    ALLOC(q->csiX, fb); ALLOC(q->csiZ, fb);
-    ALLOC(q->ax, q->X * q->esz); ALLOC(q->bx, q->X * q->esz);
-    ALLOC(q->az, q->Z * q->esz); ALLOC(q->bz, q->Z * q->esz);
-    ALLOC(q->recmap, sizeof(int) * q->X);
-    ALLOC(q->rec_j, sizeof(int) * nrec);
-    ALLOC(q->out_j, sizeof(int) * nrec);
-    ALLOC(q->out_r, sizeof(int) * nrec);
-    ALLOC(q->wav, sizeof(float) * (size_t)p->sampling_wavelet);
-    ALLOC(q->seis_t, sizeof(float) * nt * nrec);
-    ALLOC(q->seis_rec, sizeof(float) * nt * nrec);
-    ALLOC(q->snaps, q->nsnap * q->nint * q->esz);
-    ALLOC(q->I, q->nint * q->esz); ALLOC(q->Is, q->nint * q->esz);
-    ALLOC(q->Ic, q->nint * q->esz); ALLOC(q->Isc, q->nint * q->esz);
-#undef ALLOC
-    int rc = real == RTM_REAL_F64 ? upload_profiles<double>(q) : upload_profiles<float>(q);
-    if (rc == 0) rc = upload_receivers(q);
-    if (rc) return fail();
-    if (hipMemset(q->I, 0, q->nint * q->esz) != hipSuccess ||
-        hipMemset(q->Is, 0, q->nint * q->esz) != hipSuccess ||
-        hipMemset(q->Ic, 0, q->nint * q->esz) != hipSuccess ||
-        hipMemset(q->Isc, 0, q->nint * q->esz) != hipSuccess ||
-        hipMemset(q->seis_rec, 0, sizeof(float) * nt * nrec) != hipSuccess) {
-        rtm_set_error("rtm_plan_create: hipMemset failed");
-        return fail();
-    }
+    if (q->b.alloc(*p, real)) return fail();
+    const char *env = getenv("RTM_TILE_RZ");
+    const int rz = env && *env ? atoi(env) : 4;
+    if (plan_set_tiling(q, rz, 100)) return fail();
     *out = q;
     return 0;
 }
@@ -379,180 +116,187 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
 extern "C" int rtm_plan_destroy(rtm_plan *q)
 {
     if (!q) return 0;
-    (void)hipSetDevice(q->p.device);
+    (void)hipSetDevice(q->b.p.device);
     (void)hipStreamSynchronize(q->stream);
     plan_free(q);
     delete q;
     return 0;
 }
 
-template <typename R>
-static int set_velocity_t(rtm_plan *q, const float *vpad)
+extern "C" int rtm_plan_set_tiling(rtm_plan *q, int rows_per_thread, int graph_block)
 {
-    std::vector<float> v(q->nfield);
-    to_device_layout(vpad, v.data(), q->X, q->Z, q->P);
-    float *dv = nullptr;
-    HIP_TRY(hipMalloc(&dv, sizeof(float) * q->nfield));
-    hipError_t e = hipMemcpyAsync(dv, v.data(), sizeof(float) * q->nfield, hipMemcpyHostToDevice, q->stream);
-    if (e == hipSuccess) {
-        // aux = dt/h rounded in R, as SRC/forward_modeling.c:71
-        const R aux = sizeof(R) == 4 ? R(q->p.sampling / q->p.spacing_grid)
-                                     : R((double)q->p.sampling / (double)q->p.spacing_grid);
-        dim3 grid((q->X + 255) / 256, q->Z);
-        hipLaunchKernelGGL(k_coef<R>, grid, dim3(256), 0, q->stream, dv, (R *)q->C, aux, q->X, q->Z, q->P);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(q->stream);
-    (void)hipFree(dv);
-    if (e != hipSuccess) { rtm_set_error("set_velocity: %s", hipGetErrorString(e)); return -1; }
-    q->have_velocity = true;
-    return 0;
+    if (!q) { rtm_set_error("rtm_plan_set_tiling: null plan"); return -1; }
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return plan_set_tiling(q, rows_per_thread, graph_block);
 }
 
 extern "C" int rtm_plan_set_velocity(rtm_plan *q, const float *vpad)
 {
     if (!q || !vpad) { rtm_set_error("rtm_plan_set_velocity: null argument"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    return q->real == RTM_REAL_F64 ? set_velocity_t<double>(q, vpad) : set_velocity_t<float>(q, vpad);
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.set_velocity(vpad, q->stream);
 }
 
 extern "C" int rtm_plan_set_wavelet(rtm_plan *q, const float *w, int n)
 {
     if (!q || !w) { rtm_set_error("rtm_plan_set_wavelet: null argument"); return -1; }
-    if (n != q->p.sampling_wavelet) {
-        rtm_set_error("rtm_plan_set_wavelet: %d samples given, sampling_wavelet is %d", n,
-                      q->p.sampling_wavelet);
-        return -1;
-    }
-    q->wav_host.assign(w, w + n);
-    HIP_TRY(hipSetDevice(q->p.device));
-    HIP_TRY(hipMemcpy(q->wav, w, sizeof(float) * n, hipMemcpyHostToDevice));
-    return 0;
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.set_wavelet(w, n);
 }
 
 extern "C" int rtm_plan_set_seismogram(rtm_plan *q, const float *d)
 {
     if (!q || !d) { rtm_set_error("rtm_plan_set_seismogram: null argument"); return -1; }
-    const int nrec = q->p.number_of_receivers, nt = q->p.number_of_time_steps;
-    std::vector<float> t((size_t)nt * nrec);
-    for (int r = 0; r < nrec; r++)
-        for (int n = 0; n < nt; n++) t[(size_t)n * nrec + r] = d[(size_t)r * nt + n];
-    HIP_TRY(hipSetDevice(q->p.device));
-    HIP_TRY(hipMemcpy(q->seis_t, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
-    q->have_seis = true;
-    return 0;
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.set_seismogram(d, q->stream);
 }
 
 extern "C" int rtm_plan_zero_image(rtm_plan *q)
 {
     if (!q) { rtm_set_error("rtm_plan_zero_image: null plan"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    const size_t b = q->nint * q->esz;
-    HIP_TRY(hipMemsetAsync(q->I, 0, b, q->stream));
-    HIP_TRY(hipMemsetAsync(q->Is, 0, b, q->stream));
-    HIP_TRY(hipMemsetAsync(q->Ic, 0, b, q->stream));
-    HIP_TRY(hipMemsetAsync(q->Isc, 0, b, q->stream));
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    q->b.zero_image(q->stream);
     HIP_TRY(hipStreamSynchronize(q->stream));
     return 0;
 }
 
-// zero wavefields and CPML state: every modeling pass starts from rest
-// (calloc at SRC/forward_modeling.c:43-67)
-static int reset_state(rtm_plan *q)
-{
-    const size_t fb = q->nfield * q->esz;
-    void *f[] = {q->u[0], q->u[1], q->psiX[0], q->psiX[1], q->psiZ[0], q->psiZ[1], q->csiX, q->csiZ};
-    for (void *ptr : f) HIP_TRY(hipMemsetAsync(ptr, 0, fb, q->stream));
-    q->cur = 0;
-    q->psi_cur = 0;
-    return 0;
-}
+// ---- per-point (v1) loops: injection points outside the stepped region ----
 
 template <typename R>
-static StepArgs<R> base_args(rtm_plan *q)
+static StepArgs<R> v1_args(const SimBuffers &b)
 {
+    const SimDesc<R> d = b.desc<R>();
     StepArgs<R> A{};
-    const rtm_params &p = q->p;
-    A.X = q->X; A.Z = q->Z; A.P = q->P;
-    A.TL = p.thickness_left; A.TR = p.thickness_right; A.TU = p.thickness_up; A.TD = p.thickness_down;
-    A.nx = p.nx; A.nz = p.nz; A.nrec = p.number_of_receivers;
-    // h1 = 1/h, h2 = h*h, h3 = 1/(h*h), rounded as SRC/absorbingBoundaryCondition.c:159-161
-    if (sizeof(R) == 4) {
-        const float h = p.spacing_grid;
-        A.h1 = R(1.0f / h); A.h2 = R(h * h); A.h3 = R(1.0f / (h * h));
-    } else {
-        const double h = p.spacing_grid;
-        A.h1 = R(1.0 / h); A.h2 = R(h * h); A.h3 = R(1.0 / (h * h));
-    }
-    A.C = (const R *)q->C;
-    A.ax = (const R *)q->ax; A.bx = (const R *)q->bx; A.az = (const R *)q->az; A.bz = (const R *)q->bz;
-    A.csiX = (R *)q->csiX; A.csiZ = (R *)q->csiZ;
+    A.X = d.X; A.Z = d.Z; A.P = d.P;
+    A.TL = d.TL; A.TR = d.TR; A.TU = d.TU; A.TD = d.TD;
+    A.nx = d.nx; A.nz = d.nz; A.nrec = d.nrec;
+    A.h1 = d.h1; A.h2 = d.h2; A.h3 = d.h3;
+    A.C = d.C;
+    A.ax = d.ax; A.bx = d.bx; A.az = d.az; A.bz = d.bz;
+    A.csiX = d.csiX; A.csiZ = d.csiZ;
     A.src_j = -1;
-    A.rec_row = p.thickness_up + p.depth_receiver;
-    A.recmap = q->recmap;
-    A.I = (R *)q->I; A.Is = (R *)q->Is; A.Ic = (R *)q->Ic; A.Isc = (R *)q->Isc;
+    A.rec_row = d.rec_row;
+    A.recmap = d.recmap;
+    A.I = d.I; A.Is = d.Is; A.Ic = d.Ic; A.Isc = d.Isc;
     return A;
 }
 
+// u at step n is u[n & 1], psi^n is psi[n & 1] (the tiled path's convention)
 template <typename R>
-static void bind_step(rtm_plan *q, StepArgs<R> &A)
+static void v1_bind(const SimBuffers &b, StepArgs<R> &A, int n)
 {
-    A.u = (const R *)q->u[q->cur];
-    A.U = (R *)q->u[q->cur ^ 1];
-    A.psiXi = (const R *)q->psiX[q->psi_cur];
-    A.psiXo = (R *)q->psiX[q->psi_cur ^ 1];
-    A.psiZi = (const R *)q->psiZ[q->psi_cur];
-    A.psiZo = (R *)q->psiZ[q->psi_cur ^ 1];
+    A.u = (const R *)b.u[n & 1];
+    A.U = (R *)b.u[(n + 1) & 1];
+    A.psiXi = (const R *)b.psiX[n & 1];
+    A.psiXo = (R *)b.psiX[(n + 1) & 1];
+    A.psiZi = (const R *)b.psiZ[n & 1];
+    A.psiZo = (R *)b.psiZ[(n + 1) & 1];
 }
 
-static dim3 step_grid(const rtm_plan *q) { return dim3((q->X + kBX - 1) / kBX, (q->Z + kBZ - 1) / kBZ); }
-
-// Forward modeling loop, SRC/forward_modeling.c:98-130.
 template <typename R>
-static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
+static void v1_forward(rtm_plan *q, int src_j, bool src_inside, int record)
 {
-    const rtm_params &p = q->p;
-    const int xs = shot_x + p.thickness_left, zs = shot_z + p.thickness_up;
-    if (xs < 0 || xs >= q->X || zs < 0 || zs >= q->Z) {
-        rtm_set_error("source at padded (%d,%d) is outside the %dx%d grid", xs, zs, q->X, q->Z);
-        return -1;
-    }
-    const bool src_fused = xs >= rtm::kHalf && xs < q->X - rtm::kHalf && zs >= rtm::kHalf &&
-                           zs < q->Z - rtm::kHalf;
-    const int src_j = zs * q->P + xs;
-    if (reset_state(q)) return -1;
-    StepArgs<R> A = base_args<R>(q);
-    const dim3 grid = step_grid(q), block(kBX, kBZ);
-    const int TS = p.time_slices;
-    HIP_TRY(hipEventRecord(q->ev0, q->stream));
+    const SimBuffers &b = q->b;
+    const rtm_params &p = b.p;
+    StepArgs<R> A = v1_args<R>(b);
+    const dim3 grid((b.X + kBX - 1) / kBX, (b.Z + kBZ - 1) / kBZ), block(kBX, kBZ);
     for (int n = 0; n < p.number_of_time_steps; n++) {
-        bind_step(q, A);
+        v1_bind(b, A, n);
         A.mode = rtm::M_STENCIL | rtm::M_CPML;
         const bool src = n < p.sampling_wavelet;
-        if (src && src_fused) {
+        if (src && src_inside) {
             A.mode |= rtm::M_SOURCE;
             A.src_j = src_j;
-            A.src_w = R(q->wav_host[n]);
+            A.src_w = R(b.wav_host[n]);
         }
-        if (n % TS == 0) {
+        if (n % p.time_slices == 0) {
             A.mode |= rtm::M_SNAP;
-            A.snap = (R *)q->snaps + (size_t)(n / TS) * q->nint;
+            A.snap = (R *)b.snaps + (size_t)(n / p.time_slices) * b.nint;
         }
         hipLaunchKernelGGL(k_step<R>, grid, block, 0, q->stream, A);
-        if (!src_fused) {
+        if (!src_inside) {
             // a source outside the stepped region can make the outer columns
             // non-zero, so their top-strip correction is applied too
             launch_top_edges(A, q->stream);
             if (src)
-                hipLaunchKernelGGL(k_source_point<R>, dim3(1), dim3(64), 0, q->stream,
-                                   (R *)q->u[q->cur ^ 1], (const R *)q->C, src_j, R(q->wav_host[n]));
+                hipLaunchKernelGGL(k_source_point<R>, dim3(1), dim3(64), 0, q->stream, A.U, A.C, src_j, b.wav, n);
         }
         if (record)
-            hipLaunchKernelGGL(k_record<R>, dim3((p.number_of_receivers + 255) / 256), dim3(256), 0,
-                               q->stream, (const R *)q->u[q->cur ^ 1], q->rec_j, p.number_of_receivers,
-                               q->seis_rec + (size_t)n * p.number_of_receivers);
-        q->cur ^= 1;
-        q->psi_cur ^= 1;
+            hipLaunchKernelGGL(k_record<R>, dim3((p.number_of_receivers + 255) / 256), dim3(256), 0, q->stream,
+                               (const R *)A.U, b.rec_j, p.number_of_receivers,
+                               b.seis_rec + (size_t)n * p.number_of_receivers);
+    }
+}
+
+template <typename R>
+static void v1_backward(rtm_plan *q)
+{
+    const SimBuffers &b = q->b;
+    const rtm_params &p = b.p;
+    StepArgs<R> A = v1_args<R>(b);
+    const dim3 grid((b.X + kBX - 1) / kBX, (b.Z + kBZ - 1) / kBZ), block(kBX, kBZ);
+    const int TS = p.time_slices, nt = p.number_of_time_steps, nrec = p.number_of_receivers;
+    for (int n = 0; n < nt; n++) {
+        const int tau = nt - n - 1;
+        v1_bind(b, A, n);
+        A.mode = rtm::M_STENCIL | rtm::M_CPML | rtm::M_INJECT;
+        A.seis_row = b.seis_t + (size_t)tau * nrec;
+        // snapshot tau/TS pairs with this step; the reference's counter
+        // (SRC/backward_modeling.c:96,119) equals tau/TS when nt % TS == 0
+        if (tau % TS == 0) {
+            A.mode |= rtm::M_IMAGE | (p.image_kahan ? rtm::M_KAHAN : 0);
+            A.snap = (R *)b.snaps + (size_t)(tau / TS) * b.nint;
+        }
+        hipLaunchKernelGGL(k_step<R>, grid, block, 0, q->stream, A);
+        if (b.n_out) {
+            // receivers outside the stepped region: outer-column top-strip
+            // correction, then their replacement.  None of these points is in
+            // the image region, so the fused imaging above is unaffected.
+            launch_top_edges(A, q->stream);
+            hipLaunchKernelGGL(k_inject_list<R>, dim3((b.n_out + 255) / 256), dim3(256), 0, q->stream, A.U,
+                               b.out_j, b.out_r, b.n_out, A.seis_row);
+        }
+    }
+}
+
+// ---- passes ----
+
+template <typename R>
+static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
+{
+    SimBuffers &b = q->b;
+    const rtm_params &p = b.p;
+    const int xs = shot_x + p.thickness_left, zs = shot_z + p.thickness_up;
+    if (xs < 0 || xs >= b.X || zs < 0 || zs >= b.Z) {
+        rtm_set_error("source at padded (%d,%d) is outside the %dx%d grid", xs, zs, b.X, b.Z);
+        return -1;
+    }
+    const bool src_inside = xs >= rtm::kHalf && xs < b.X - rtm::kHalf && zs >= rtm::kHalf && zs < b.Z - rtm::kHalf;
+    const int src_j = zs * b.P + xs;
+    const int nt = p.number_of_time_steps;
+    b.reset_state(q->stream);
+    const bool tiled = src_inside;   // the tiled kernel covers only the stepped region
+    HIP_TRY(hipEventRecord(q->ev0, q->stream));
+    if (tiled) {
+        SimDesc<R> d = b.desc<R>();
+        d.phase = rtm::PH_FORWARD;
+        d.flags = rtm::F_SNAP | (record && !b.rec_dup && b.n_out == 0 ? rtm::F_RECORD : 0);
+        d.src_j = src_j;
+        if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
+        if (record && !(d.flags & rtm::F_RECORD)) {
+            // receivers sharing a point or outside: record each one per step
+            for (int n = 0; n < nt; n++) {
+                q->ex.launch_one(n, q->stream);
+                hipLaunchKernelGGL(k_record<R>, dim3((p.number_of_receivers + 255) / 256), dim3(256), 0,
+                                   q->stream, (const R *)b.u[(n + 1) & 1], b.rec_j, p.number_of_receivers,
+                                   b.seis_rec + (size_t)n * p.number_of_receivers);
+            }
+        } else if (q->ex.run(nt, q->graph_block, q->stream)) {
+            return -1;
+        }
+    } else {
+        v1_forward<R>(q, src_j, src_inside, record);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(q->ev1, q->stream));
@@ -560,45 +304,30 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
     q->fwd_ms = ms;
-    q->last_steps = p.number_of_time_steps;
-    q->have_snaps = true;
+    q->last_steps = nt;
+    q->last_nt = nt;
+    b.have_snaps = true;
     return 0;
 }
 
-// Backward modeling + imaging loop, SRC/backward_modeling.c:97-126.
 template <typename R>
 static int backward_t(rtm_plan *q)
 {
-    const rtm_params &p = q->p;
-    if (!q->have_snaps) { rtm_set_error("rtm_plan_backward: no snapshots (run forward first)"); return -1; }
-    if (!q->have_seis) { rtm_set_error("rtm_plan_backward: no seismogram"); return -1; }
-    if (reset_state(q)) return -1;
-    StepArgs<R> A = base_args<R>(q);
-    const dim3 grid = step_grid(q), block(kBX, kBZ);
-    const int TS = p.time_slices, nt = p.number_of_time_steps, nrec = p.number_of_receivers;
+    SimBuffers &b = q->b;
+    const rtm_params &p = b.p;
+    if (!b.have_snaps) { rtm_set_error("rtm_plan_backward: no snapshots (run forward first)"); return -1; }
+    if (!b.have_seis) { rtm_set_error("rtm_plan_backward: no seismogram"); return -1; }
+    const int nt = p.number_of_time_steps;
+    b.reset_state(q->stream);
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
-    for (int n = 0; n < nt; n++) {
-        const int tau = nt - n - 1;
-        bind_step(q, A);
-        A.mode = rtm::M_STENCIL | rtm::M_CPML | rtm::M_INJECT;
-        A.seis_row = q->seis_t + (size_t)tau * nrec;
-        // snapshot tau/TS pairs with this step; the reference's counter
-        // (SRC/backward_modeling.c:96,119) equals tau/TS when nt % TS == 0
-        if (tau % TS == 0) {
-            A.mode |= rtm::M_IMAGE | (p.image_kahan ? rtm::M_KAHAN : 0);
-            A.snap = (R *)q->snaps + (size_t)(tau / TS) * q->nint;
-        }
-        hipLaunchKernelGGL(k_step<R>, grid, block, 0, q->stream, A);
-        if (q->n_out) {
-            // receivers outside the stepped region: outer-column top-strip
-            // correction, then their replacement.  None of these points is in
-            // the image region, so the fused imaging above is unaffected.
-            launch_top_edges(A, q->stream);
-            hipLaunchKernelGGL(k_inject_list<R>, dim3((q->n_out + 255) / 256), dim3(256), 0, q->stream,
-                               (R *)q->u[q->cur ^ 1], q->out_j, q->out_r, q->n_out, A.seis_row);
-        }
-        q->cur ^= 1;
-        q->psi_cur ^= 1;
+    if (b.n_out == 0) {
+        SimDesc<R> d = b.desc<R>();
+        d.phase = rtm::PH_BACKWARD;
+        d.flags = rtm::F_IMAGE | (p.image_kahan ? rtm::F_KAHAN : 0);
+        if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
+        if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+    } else {
+        v1_backward<R>(q);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(q->ev1, q->stream));
@@ -607,32 +336,33 @@ static int backward_t(rtm_plan *q)
     HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
     q->bwd_ms = ms;
     q->last_steps = nt;
+    q->last_nt = nt;
     return 0;
 }
 
 static int ready(rtm_plan *q, const char *who)
 {
     if (!q) { rtm_set_error("%s: null plan", who); return -1; }
-    if (!q->have_velocity) { rtm_set_error("%s: velocity not set", who); return -1; }
-    if (hipSetDevice(q->p.device) != hipSuccess) { rtm_set_error("%s: hipSetDevice failed", who); return -1; }
+    if (!q->b.have_velocity) { rtm_set_error("%s: velocity not set", who); return -1; }
+    if (hipSetDevice(q->b.p.device) != hipSuccess) { rtm_set_error("%s: hipSetDevice failed", who); return -1; }
     return 0;
 }
 
 extern "C" int rtm_plan_forward(rtm_plan *q, int shot_x, int shot_z, int record)
 {
     if (ready(q, "rtm_plan_forward")) return -1;
-    if (q->wav_host.size() != (size_t)q->p.sampling_wavelet) {
+    if (q->b.wav_host.size() != (size_t)q->b.p.sampling_wavelet) {
         rtm_set_error("rtm_plan_forward: wavelet not set");
         return -1;
     }
-    return q->real == RTM_REAL_F64 ? forward_t<double>(q, shot_x, shot_z, record)
-                                   : forward_t<float>(q, shot_x, shot_z, record);
+    return q->b.real == RTM_REAL_F64 ? forward_t<double>(q, shot_x, shot_z, record)
+                                     : forward_t<float>(q, shot_x, shot_z, record);
 }
 
 extern "C" int rtm_plan_backward(rtm_plan *q)
 {
     if (ready(q, "rtm_plan_backward")) return -1;
-    return q->real == RTM_REAL_F64 ? backward_t<double>(q) : backward_t<float>(q);
+    return q->b.real == RTM_REAL_F64 ? backward_t<double>(q) : backward_t<float>(q);
 }
 
 extern "C" int rtm_plan_migrate(rtm_plan *q, int shot_x, int shot_z)
@@ -645,11 +375,11 @@ extern "C" int rtm_plan_migrate(rtm_plan *q, int shot_x, int shot_z)
 extern "C" int rtm_plan_get_seismogram(rtm_plan *q, float *d)
 {
     if (!q || !d) { rtm_set_error("rtm_plan_get_seismogram: null argument"); return -1; }
-    const int nrec = q->p.number_of_receivers, nt = q->p.number_of_time_steps;
+    const int nrec = q->b.p.number_of_receivers, nt = q->b.p.number_of_time_steps;
     std::vector<float> t((size_t)nt * nrec);
-    HIP_TRY(hipSetDevice(q->p.device));
+    HIP_TRY(hipSetDevice(q->b.p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
-    HIP_TRY(hipMemcpy(t.data(), q->seis_rec, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(t.data(), q->b.seis_rec, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
     for (int r = 0; r < nrec; r++)
         for (int n = 0; n < nt; n++) d[(size_t)r * nt + n] = t[(size_t)n * nrec + r];
     return 0;
@@ -658,14 +388,15 @@ extern "C" int rtm_plan_get_seismogram(rtm_plan *q, float *d)
 template <typename R, typename O>
 static int get_image_t(rtm_plan *q, O *I, O *Is)
 {
-    const int nx = q->p.nx, nz = q->p.nz;
-    std::vector<R> h(q->nint);
+    const SimBuffers &b = q->b;
+    const int nx = b.p.nx, nz = b.p.nz;
+    std::vector<R> h(b.nint);
     HIP_TRY(hipStreamSynchronize(q->stream));
-    void *src[2] = {q->I, q->Is};
+    void *src[2] = {b.I, b.Is};
     O *dst[2] = {I, Is};
     for (int t = 0; t < 2; t++) {
         if (!dst[t]) continue;
-        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * q->nint, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * b.nint, hipMemcpyDeviceToHost));
         for (int i = 0; i < nx; i++)
             for (int k = 0; k < nz; k++) dst[t][(size_t)i * nz + k] = O(h[(size_t)k * nx + i]);
     }
@@ -675,29 +406,31 @@ static int get_image_t(rtm_plan *q, O *I, O *Is)
 extern "C" int rtm_plan_get_image(rtm_plan *q, float *I, float *Is)
 {
     if (!q || !I) { rtm_set_error("rtm_plan_get_image: null argument"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    return q->real == RTM_REAL_F64 ? get_image_t<double, float>(q, I, Is) : get_image_t<float, float>(q, I, Is);
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.real == RTM_REAL_F64 ? get_image_t<double, float>(q, I, Is) : get_image_t<float, float>(q, I, Is);
 }
 
 extern "C" int rtm_plan_get_image_f64(rtm_plan *q, double *I, double *Is)
 {
     if (!q || !I) { rtm_set_error("rtm_plan_get_image_f64: null argument"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    return q->real == RTM_REAL_F64 ? get_image_t<double, double>(q, I, Is) : get_image_t<float, double>(q, I, Is);
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.real == RTM_REAL_F64 ? get_image_t<double, double>(q, I, Is) : get_image_t<float, double>(q, I, Is);
 }
 
 template <typename R, typename O>
 static int get_fields_t(rtm_plan *q, O *u_prev, O *u_last)
 {
-    std::vector<R> h(q->nfield);
+    const SimBuffers &b = q->b;
+    std::vector<R> h(b.nfield);
     HIP_TRY(hipStreamSynchronize(q->stream));
-    // after a loop u[cur] holds the last field and u[cur^1] the one before
-    void *src[2] = {q->u[q->cur ^ 1], q->u[q->cur]};
+    // after nt steps u[nt & 1] holds u^{nt} and the other buffer u^{nt-1}
+    const int nt = q->last_nt;
+    void *src[2] = {b.u[(nt + 1) & 1], b.u[nt & 1]};
     O *dst[2] = {u_prev, u_last};
     for (int t = 0; t < 2; t++) {
         if (!dst[t]) continue;
-        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * q->nfield, hipMemcpyDeviceToHost));
-        from_device_layout(h.data(), dst[t], q->X, q->Z, q->P);
+        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * b.nfield, hipMemcpyDeviceToHost));
+        from_device_layout(h.data(), dst[t], b.X, b.Z, b.P);
     }
     return 0;
 }
@@ -705,23 +438,23 @@ static int get_fields_t(rtm_plan *q, O *u_prev, O *u_last)
 extern "C" int rtm_plan_get_fields(rtm_plan *q, float *u_prev, float *u_last)
 {
     if (!q) { rtm_set_error("rtm_plan_get_fields: null plan"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    return q->real == RTM_REAL_F64 ? get_fields_t<double, float>(q, u_prev, u_last)
-                                   : get_fields_t<float, float>(q, u_prev, u_last);
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.real == RTM_REAL_F64 ? get_fields_t<double, float>(q, u_prev, u_last)
+                                     : get_fields_t<float, float>(q, u_prev, u_last);
 }
 
 extern "C" int rtm_plan_get_fields_f64(rtm_plan *q, double *u_prev, double *u_last)
 {
     if (!q) { rtm_set_error("rtm_plan_get_fields_f64: null plan"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
-    return q->real == RTM_REAL_F64 ? get_fields_t<double, double>(q, u_prev, u_last)
-                                   : get_fields_t<float, double>(q, u_prev, u_last);
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.real == RTM_REAL_F64 ? get_fields_t<double, double>(q, u_prev, u_last)
+                                     : get_fields_t<float, double>(q, u_prev, u_last);
 }
 
 extern "C" int rtm_plan_synchronize(rtm_plan *q)
 {
     if (!q) { rtm_set_error("rtm_plan_synchronize: null plan"); return -1; }
-    HIP_TRY(hipSetDevice(q->p.device));
+    HIP_TRY(hipSetDevice(q->b.p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
     return 0;
 }

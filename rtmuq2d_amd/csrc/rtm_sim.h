@@ -1,0 +1,285 @@
+// rtm_sim.h -- device-side building blocks shared by the plan (one
+// simulation) and the batch/serving layers (many simulations per launch):
+// kernels, the per-simulation buffer set, and the step executor that walks a
+// tile list with one launch per time step (optionally replayed from a
+// hipGraph of a block of steps).  Internal to librtm_hip.so.
+#ifndef RTM_SIM_H
+#define RTM_SIM_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "rtm_internal.h"
+#include "rtm_step.h"
+#include "rtm_tile.h"
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            rtm_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
+                          __FILE__, __LINE__);                                         \
+            return -1;                                                                 \
+        }                                                                              \
+    } while (0)
+
+namespace rtmd {
+
+using rtm::SimDesc;
+using rtm::StepArgs;
+
+constexpr int kBX = 64;   // v1 kernel block: one wavefront along x ...
+constexpr int kBZ = 4;    // ... by 4 rows
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// The tiled step: one workgroup per tile of the flat tile list, any number of
+// simulations per launch.  Workgroup b takes tile (b % 8) * per + b / 8 so
+// that each XCD (workgroups are dealt round-robin to the 8 XCDs) walks a
+// contiguous run of neighbouring tiles and their shared halo rows stay in its
+// L2.  `s` is the step within the current block; sim i is at n0[i] + s.
+template <typename R, int RZ>
+__global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
+                                               const int *__restrict__ n0,
+                                               const int4 *__restrict__ tiles,
+                                               const int *__restrict__ ntiles_p, int s)
+{
+    __shared__ rtm::TileLds<R, RZ> L;
+    const int ntiles = *ntiles_p;
+    const int per = (ntiles + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= ntiles) return;
+    const int4 tile = tiles[t];
+    const SimDesc<R> &S = sims[tile.x];
+    const rtm::StepView<R> V = rtm::step_view(S, n0[tile.x] + s);
+    if (!V.active) return;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    rtm::tile_load(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+    __syncthreads();
+    if (tile.w) {
+        rtm::tile_psi(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+        __syncthreads();
+    }
+    rtm::tile_points(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+}
+
+// n0[i] += block for every simulation (the last node of a captured block)
+static __global__ void k_advance(int *__restrict__ n0, int nsims, int block)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsims) n0[i] += block;
+}
+
+// zero fill (used instead of hipMemsetAsync so that the same code is safe to
+// capture into a graph)
+template <typename T>
+__global__ void k_fill_zero(T *__restrict__ p, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) p[t] = T(0);
+}
+
+template <typename T>
+inline void fill_zero(T *p, size_t n, hipStream_t s)
+{
+    if (!p || !n) return;
+    const size_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+}
+
+// The per-point kernel (rtm::step_point), used by the per-step legacy entry
+// points and for injection points outside the stepped region.
+template <typename R>
+__global__ void __launch_bounds__(kBX * kBZ) k_step(StepArgs<R> A)
+{
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int k = blockIdx.y * kBZ + threadIdx.y;
+    if (i < rtm::kHalf || i >= A.X - rtm::kHalf || k < rtm::kHalf || k >= A.Z - rtm::kHalf) return;
+    rtm::step_point(A, i, k);
+}
+
+// Top-strip correction of the outer columns i < 4 and i >= X-4
+// (rtm::cpml_top_edge_point); 8 columns x (TU-8) rows.
+template <typename R>
+__global__ void k_top_edges(StepArgs<R> A)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rows = A.TU - 2 * rtm::kHalf;
+    if (t >= 8 * rows) return;
+    const int c = t % 8, k = rtm::kHalf + t / 8;
+    const int i = c < 4 ? c : A.X - 8 + c;
+    rtm::cpml_top_edge_point(A, i, k);
+}
+
+template <typename R>
+inline void launch_top_edges(const StepArgs<R> &A, hipStream_t s)
+{
+    const int n = 8 * (A.TU - 2 * rtm::kHalf);
+    hipLaunchKernelGGL(k_top_edges<R>, dim3((n + 63) / 64), dim3(64), 0, s, A);
+}
+
+// coef = (v*aux)*(v*aux) on the padded velocity (already in device layout),
+// SRC/forward_modeling.c:79.
+template <typename R>
+__global__ void k_coef(const float *__restrict__ v, R *__restrict__ C, R aux, int X, int Z, int P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= X || k >= Z) return;
+    const size_t j = (size_t)k * P + i;
+    const R vv = R(v[j]);
+    C[j] = (vv * aux) * (vv * aux);
+}
+
+// receiver replacement at points outside the stepped region,
+// SRC/backward_modeling.c:106-110
+template <typename R>
+__global__ void k_inject_list(R *__restrict__ U, const int *__restrict__ jlist,
+                              const int *__restrict__ rlist, int n, const float *__restrict__ seis_row)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) U[jlist[t]] = R(seis_row[rlist[t]]);
+}
+
+// source add at a point outside the stepped region, SRC/forward_modeling.c:105-109
+template <typename R>
+__global__ void k_source_point(R *__restrict__ U, const R *__restrict__ C, int j, const float *__restrict__ w, int n)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) U[j] = U[j] + R(w[n]) * C[j];
+}
+
+// seismogram recording for every receiver (also when several share a point)
+template <typename R>
+__global__ void k_record(const R *__restrict__ U, const int *__restrict__ rec_j, int nrec,
+                         float *__restrict__ row)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrec) row[r] = float(U[rec_j[r]]);
+}
+
+// imaging of padded fields (legacy normalized_cross_correlation),
+// SRC/image_conditions.c:20-33; image layout [nz][nx] x fastest.
+template <typename R>
+__global__ void k_image_padded(R *__restrict__ I, R *__restrict__ Is, const R *__restrict__ S,
+                               const R *__restrict__ Rw, int nx, int nz, int TL, int TU, int P)
+{
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kk = blockIdx.y;
+    if (ii >= nx || kk >= nz) return;
+    const size_t j = (size_t)(kk + TU) * P + (ii + TL);
+    const size_t q = (size_t)kk * nx + ii;
+    I[q] = I[q] + S[j] * Rw[j];
+    if (Is) Is[q] = Is[q] + S[j] * S[j];
+}
+
+// ---------------------------------------------------------------------------
+// host-side layout helpers (reference [X][Z] z-fastest <-> device [Z][P])
+// ---------------------------------------------------------------------------
+
+template <typename Src, typename Dst>
+void to_device_layout(const Src *ref, Dst *dev, int X, int Z, int P)
+{
+    for (int k = 0; k < Z; k++) {
+        Dst *row = dev + (size_t)k * P;
+        for (int i = 0; i < X; i++) row[i] = Dst(ref[(size_t)i * Z + k]);
+        for (int i = X; i < P; i++) row[i] = Dst(0);
+    }
+}
+
+template <typename Src, typename Dst>
+void from_device_layout(const Src *dev, Dst *ref, int X, int Z, int P)
+{
+    for (int i = 0; i < X; i++)
+        for (int k = 0; k < Z; k++) ref[(size_t)i * Z + k] = Dst(dev[(size_t)k * P + i]);
+}
+
+// ---------------------------------------------------------------------------
+// one simulation's device buffers
+// ---------------------------------------------------------------------------
+
+struct SimBuffers {
+    rtm_params p{};
+    int real = RTM_REAL_F32;
+    size_t esz = 4;
+    int X = 0, Z = 0, P = 0;
+    size_t nfield = 0, nint = 0, nsnap = 0;
+    void *u[2] = {nullptr, nullptr};
+    void *C = nullptr;
+    void *psiX[2] = {nullptr, nullptr}, *psiZ[2] = {nullptr, nullptr};
+    void *csiX = nullptr, *csiZ = nullptr;
+    void *ax = nullptr, *bx = nullptr, *az = nullptr, *bz = nullptr;
+    int *recmap = nullptr;              // [X]
+    int *rec_j = nullptr;               // [nrec] device index of each receiver
+    int *out_j = nullptr, *out_r = nullptr;
+    int n_out = 0;                      // receivers outside the stepped region
+    bool rec_dup = false;               // several receivers share a point
+    float *wav = nullptr;               // [n_w]
+    float *seis_t = nullptr;            // [nt][nrec] backward input (t-major)
+    float *seis_rec = nullptr;          // [nt][nrec] forward recording
+    void *snaps = nullptr;              // [nsnap][nz][nx]
+    void *I = nullptr, *Is = nullptr, *Ic = nullptr, *Isc = nullptr;
+    std::vector<float> wav_host;
+    bool have_velocity = false, have_seis = false, have_snaps = false;
+    std::vector<void *> owned;
+
+    int alloc(const rtm_params &pp, int real_type);
+    void release();
+    int upload_profiles();
+    int upload_receivers();
+    int set_velocity(const float *vpad, hipStream_t s);
+    int set_wavelet(const float *w, int n);
+    int set_seismogram(const float *d, hipStream_t s);
+    void reset_state(hipStream_t s);      // zero u, psi, csi (a pass starts at rest)
+    void zero_image(hipStream_t s);
+    template <typename R>
+    SimDesc<R> desc() const;
+};
+
+// plan memory in bytes for params p (the admission footprint)
+size_t sim_bytes(const rtm_params &p, size_t esz);
+
+// tiles {sim, x0, z0, class} covering the stepped region of a grid
+void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz);
+
+// ---------------------------------------------------------------------------
+// executor: one launch per time step over a tile list, hipGraph blocks
+// ---------------------------------------------------------------------------
+
+struct Exec {
+    int real = RTM_REAL_F32;
+    int rz = 4;                         // rows per thread (tile height 4*rz)
+    int cap_sims = 0, cap_tiles = 0;
+    int nsims = 0, ntiles = 0;
+    void *d_desc = nullptr;             // SimDesc<R>[cap_sims]
+    int *d_n0 = nullptr;                // [cap_sims]
+    int4 *d_tiles = nullptr;            // [cap_tiles]
+    int *d_ntiles = nullptr;
+    std::vector<int> h_n0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int gblock = 0;
+
+    int init(int real_type, int max_sims, int max_tiles, int rows_per_thread);
+    void release();
+    int set_tiles(const std::vector<int4> &tiles, hipStream_t s);
+    int upload_descs(const void *descs, int n, hipStream_t s);   // SimDesc<R>[n]
+    int set_n0(const std::vector<int> &n0, hipStream_t s);
+    // `steps` launches from the current n0 (eager, or in graph blocks of
+    // `block` steps when block > 1); n0 advances by `steps` for all sims
+    int run(int steps, int block, hipStream_t s);
+    int launch_one(int s_in_block, hipStream_t s);
+    int drop_graph();
+};
+
+}  // namespace rtmd
+
+#endif

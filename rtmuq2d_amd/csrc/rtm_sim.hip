@@ -1,0 +1,421 @@
+// rtm_sim.hip -- per-simulation device buffers and the tiled step executor.
+#include "rtm_sim.h"
+
+namespace rtmd {
+
+size_t sim_bytes(const rtm_params &p, size_t esz)
+{
+    const int X = p.thickness_left + p.nx + p.thickness_right;
+    const int Z = p.thickness_up + p.nz + p.thickness_down;
+    const size_t P = (size_t)round_up(X, 64);
+    const size_t nfield = P * Z;
+    const size_t nint = (size_t)p.nx * p.nz;
+    const size_t nsnap = (size_t)(p.number_of_time_steps / p.time_slices);
+    const size_t nrec = (size_t)p.number_of_receivers, nt = (size_t)p.number_of_time_steps;
+    size_t b = 0;
+    b += 9 * nfield * esz;                          // u x2, C, psiX x2, psiZ x2, csiX, csiZ
+    b += (size_t)(2 * X + 2 * Z) * esz;             // profiles
+    b += (size_t)X * 4 + 3 * nrec * 4;              // receiver tables
+    b += (size_t)p.sampling_wavelet * 4;
+    b += 2 * nt * nrec * 4;                         // input and recorded traces
+    b += nsnap * nint * esz;                        // snapshot store
+    b += 4 * nint * esz;                            // I, Is and their compensations
+    return b;
+}
+
+int SimBuffers::alloc(const rtm_params &pp, int real_type)
+{
+    p = pp;
+    real = real_type;
+    esz = real == RTM_REAL_F64 ? 8 : 4;
+    X = p.thickness_left + p.nx + p.thickness_right;
+    Z = p.thickness_up + p.nz + p.thickness_down;
+    P = round_up(X, 64);
+    nfield = (size_t)P * Z;
+    nint = (size_t)p.nx * p.nz;
+    nsnap = (size_t)(p.number_of_time_steps / p.time_slices);
+    const size_t nrec = (size_t)p.number_of_receivers, nt = (size_t)p.number_of_time_steps;
+    auto get = [&](void **ptr, size_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 4);
+        if (e != hipSuccess) {
+            rtm_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return -1;
+        }
+        owned.push_back(*ptr);
+        return 0;
+    };
+    const size_t fb = nfield * esz;
+    int rc = 0;
+    rc |= get(&u[0], fb); rc |= rc ? 0 : get(&u[1], fb);
+    if (!rc) rc = get(&C, fb);
+    if (!rc) rc = get(&psiX[0], fb);
+    if (!rc) rc = get(&psiX[1], fb);
+    if (!rc) rc = get(&psiZ[0], fb);
+    if (!rc) rc = get(&psiZ[1], fb);
+    if (!rc) rc = get(&csiX, fb);
+    if (!rc) rc = get(&csiZ, fb);
+    if (!rc) rc = get(&ax, X * esz);
+    if (!rc) rc = get(&bx, X * esz);
+    if (!rc) rc = get(&az, Z * esz);
+    if (!rc) rc = get(&bz, Z * esz);
+    if (!rc) rc = get((void **)&recmap, sizeof(int) * X);
+    if (!rc) rc = get((void **)&rec_j, sizeof(int) * nrec);
+    if (!rc) rc = get((void **)&out_j, sizeof(int) * nrec);
+    if (!rc) rc = get((void **)&out_r, sizeof(int) * nrec);
+    if (!rc) rc = get((void **)&wav, sizeof(float) * (size_t)p.sampling_wavelet);
+    if (!rc) rc = get((void **)&seis_t, sizeof(float) * nt * nrec);
+    if (!rc) rc = get((void **)&seis_rec, sizeof(float) * nt * nrec);
+    if (!rc) rc = get(&snaps, nsnap * nint * esz);
+    if (!rc) rc = get(&I, nint * esz);
+    if (!rc) rc = get(&Is, nint * esz);
+    if (!rc) rc = get(&Ic, nint * esz);
+    if (!rc) rc = get(&Isc, nint * esz);
+    if (!rc) rc = upload_profiles();
+    if (!rc) rc = upload_receivers();
+    if (!rc) {
+        // velocity-independent state starts at zero
+        void *z[] = {I, Is, Ic, Isc, C};
+        for (void *ptr : z)
+            if (hipMemset(ptr, 0, ptr == C ? fb : nint * esz) != hipSuccess) rc = -1;
+        if (hipMemset(seis_rec, 0, sizeof(float) * nt * nrec) != hipSuccess ||
+            hipMemset(seis_t, 0, sizeof(float) * nt * nrec) != hipSuccess)
+            rc = -1;
+        if (rc) rtm_set_error("SimBuffers: hipMemset failed");
+    }
+    if (rc) release();
+    return rc;
+}
+
+void SimBuffers::release()
+{
+    for (void *ptr : owned) (void)hipFree(ptr);
+    owned.clear();
+    u[0] = u[1] = C = psiX[0] = psiX[1] = psiZ[0] = psiZ[1] = csiX = csiZ = nullptr;
+    ax = bx = az = bz = nullptr;
+    recmap = rec_j = out_j = out_r = nullptr;
+    wav = seis_t = seis_rec = nullptr;
+    snaps = I = Is = Ic = Isc = nullptr;
+}
+
+template <typename R>
+static int upload_profiles_t(SimBuffers &q)
+{
+    const rtm_params &p = q.p;
+    const int TL = p.thickness_left, TR = p.thickness_right, TU = p.thickness_up, TD = p.thickness_down;
+    std::vector<R> aL(TL), aR(TR), aT(TU), aB(TD), bL(TL), bR(TR), bT(TU), bB(TD);
+    if constexpr (sizeof(R) == 4)
+        rtm_profiles_f32(&p, aL.data(), aR.data(), aT.data(), aB.data(), bL.data(), bR.data(), bT.data(), bB.data());
+    else
+        rtm_profiles_f64(&p, aL.data(), aR.data(), aT.data(), aB.data(), bL.data(), bR.data(), bT.data(), bB.data());
+    // full-length profiles: x strips use the left/right arrays, z strips the
+    // top/bottom arrays, indexed as SRC/absorbingBoundaryCondition.c:181,273,227,320
+    std::vector<R> ax(q.X, R(0)), bx(q.X, R(0)), az(q.Z, R(0)), bz(q.Z, R(0));
+    for (int i = 0; i < TL; i++) { ax[i] = aL[i]; bx[i] = bL[i]; }
+    for (int i = 0; i < TR; i++) { ax[q.X - TR + i] = aR[i]; bx[q.X - TR + i] = bR[i]; }
+    for (int k = 0; k < TU; k++) { az[k] = aT[k]; bz[k] = bT[k]; }
+    for (int k = 0; k < TD; k++) { az[q.Z - TD + k] = aB[k]; bz[q.Z - TD + k] = bB[k]; }
+    HIP_TRY(hipMemcpy(q.ax, ax.data(), sizeof(R) * q.X, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(q.bx, bx.data(), sizeof(R) * q.X, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(q.az, az.data(), sizeof(R) * q.Z, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(q.bz, bz.data(), sizeof(R) * q.Z, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int SimBuffers::upload_profiles()
+{
+    return real == RTM_REAL_F64 ? upload_profiles_t<double>(*this) : upload_profiles_t<float>(*this);
+}
+
+// Receiver tables.  Positions follow pos_r = first + (spacing-1)*r
+// (SRC/backward_modeling.c:108); when several receivers share a point the
+// last one wins, as in the reference's loop.
+int SimBuffers::upload_receivers()
+{
+    const int nrec = p.number_of_receivers;
+    const int zr = p.thickness_up + p.depth_receiver;
+    std::vector<int> rm(X, -1), rj(nrec);
+    std::map<long, int> outside;
+    rec_dup = false;
+    for (int r = 0; r < nrec; r++) {
+        const int col = p.thickness_left + p.first_location_receiver + (p.spacing_receiver - 1) * r;
+        if (col < 0 || col >= X || zr < 0 || zr >= Z) {
+            rtm_set_error("receiver %d at padded (%d,%d) is outside the %dx%d grid", r, col, zr, X, Z);
+            return -1;
+        }
+        rj[r] = zr * P + col;
+        const bool inside = col >= rtm::kHalf && col < X - rtm::kHalf && zr >= rtm::kHalf && zr < Z - rtm::kHalf;
+        if (inside) {
+            if (rm[col] >= 0) rec_dup = true;
+            rm[col] = r;
+        } else {
+            if (outside.count((long)zr * P + col)) rec_dup = true;
+            outside[(long)zr * P + col] = r;
+        }
+    }
+    HIP_TRY(hipMemcpy(recmap, rm.data(), sizeof(int) * X, hipMemcpyHostToDevice));
+    if (nrec) HIP_TRY(hipMemcpy(rec_j, rj.data(), sizeof(int) * nrec, hipMemcpyHostToDevice));
+    std::vector<int> oj, orr;
+    for (auto &kv : outside) { oj.push_back((int)kv.first); orr.push_back(kv.second); }
+    n_out = (int)oj.size();
+    if (n_out) {
+        HIP_TRY(hipMemcpy(out_j, oj.data(), sizeof(int) * n_out, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(out_r, orr.data(), sizeof(int) * n_out, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+template <typename R>
+static int set_velocity_t(SimBuffers &q, const float *vpad, hipStream_t s)
+{
+    std::vector<float> v(q.nfield);
+    to_device_layout(vpad, v.data(), q.X, q.Z, q.P);
+    float *dv = nullptr;
+    HIP_TRY(hipMalloc(&dv, sizeof(float) * q.nfield));
+    hipError_t e = hipMemcpyAsync(dv, v.data(), sizeof(float) * q.nfield, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        // aux = dt/h rounded in R, as SRC/forward_modeling.c:71
+        const R aux = sizeof(R) == 4 ? R(q.p.sampling / q.p.spacing_grid)
+                                     : R((double)q.p.sampling / (double)q.p.spacing_grid);
+        hipLaunchKernelGGL(k_coef<R>, dim3((q.X + 255) / 256, q.Z), dim3(256), 0, s, dv, (R *)q.C, aux, q.X,
+                           q.Z, q.P);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dv);
+    if (e != hipSuccess) { rtm_set_error("set_velocity: %s", hipGetErrorString(e)); return -1; }
+    q.have_velocity = true;
+    return 0;
+}
+
+int SimBuffers::set_velocity(const float *vpad, hipStream_t s)
+{
+    return real == RTM_REAL_F64 ? set_velocity_t<double>(*this, vpad, s) : set_velocity_t<float>(*this, vpad, s);
+}
+
+int SimBuffers::set_wavelet(const float *w, int n)
+{
+    if (n != p.sampling_wavelet) {
+        rtm_set_error("wavelet: %d samples given, sampling_wavelet is %d", n, p.sampling_wavelet);
+        return -1;
+    }
+    wav_host.assign(w, w + n);
+    if (n) HIP_TRY(hipMemcpy(wav, w, sizeof(float) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int SimBuffers::set_seismogram(const float *d, hipStream_t s)
+{
+    const int nrec = p.number_of_receivers, nt = p.number_of_time_steps;
+    std::vector<float> t((size_t)nt * nrec);
+    for (int r = 0; r < nrec; r++)
+        for (int n = 0; n < nt; n++) t[(size_t)n * nrec + r] = d[(size_t)r * nt + n];
+    HIP_TRY(hipMemcpyAsync(seis_t, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    have_seis = true;
+    return 0;
+}
+
+void SimBuffers::reset_state(hipStream_t s)
+{
+    if (real == RTM_REAL_F64) {
+        double *f[] = {(double *)u[0], (double *)u[1], (double *)psiX[0], (double *)psiX[1],
+                       (double *)psiZ[0], (double *)psiZ[1], (double *)csiX, (double *)csiZ};
+        for (double *ptr : f) fill_zero(ptr, nfield, s);
+    } else {
+        float *f[] = {(float *)u[0], (float *)u[1], (float *)psiX[0], (float *)psiX[1],
+                      (float *)psiZ[0], (float *)psiZ[1], (float *)csiX, (float *)csiZ};
+        for (float *ptr : f) fill_zero(ptr, nfield, s);
+    }
+}
+
+void SimBuffers::zero_image(hipStream_t s)
+{
+    if (real == RTM_REAL_F64) {
+        double *f[] = {(double *)I, (double *)Is, (double *)Ic, (double *)Isc};
+        for (double *ptr : f) fill_zero(ptr, nint, s);
+    } else {
+        float *f[] = {(float *)I, (float *)Is, (float *)Ic, (float *)Isc};
+        for (float *ptr : f) fill_zero(ptr, nint, s);
+    }
+}
+
+template <typename R>
+SimDesc<R> SimBuffers::desc() const
+{
+    SimDesc<R> d{};
+    d.X = X; d.Z = Z; d.P = P;
+    d.TL = p.thickness_left; d.TR = p.thickness_right; d.TU = p.thickness_up; d.TD = p.thickness_down;
+    d.nx = p.nx; d.nz = p.nz;
+    d.nt = p.number_of_time_steps; d.TS = p.time_slices; d.nw = p.sampling_wavelet;
+    d.nrec = p.number_of_receivers;
+    d.phase = rtm::PH_IDLE;
+    d.flags = 0;
+    d.src_j = -1;
+    d.rec_row = p.thickness_up + p.depth_receiver;
+    // h1 = 1/h, h2 = h*h, h3 = 1/(h*h), rounded as SRC/absorbingBoundaryCondition.c:159-161
+    if (sizeof(R) == 4) {
+        const float h = p.spacing_grid;
+        d.h1 = R(1.0f / h); d.h2 = R(h * h); d.h3 = R(1.0f / (h * h));
+    } else {
+        const double h = p.spacing_grid;
+        d.h1 = R(1.0 / h); d.h2 = R(h * h); d.h3 = R(1.0 / (h * h));
+    }
+    d.ubuf[0] = (R *)u[0]; d.ubuf[1] = (R *)u[1];
+    d.C = (const R *)C;
+    d.ax = (const R *)ax; d.bx = (const R *)bx; d.az = (const R *)az; d.bz = (const R *)bz;
+    d.psiX[0] = (R *)psiX[0]; d.psiX[1] = (R *)psiX[1];
+    d.psiZ[0] = (R *)psiZ[0]; d.psiZ[1] = (R *)psiZ[1];
+    d.csiX = (R *)csiX; d.csiZ = (R *)csiZ;
+    d.recmap = recmap;
+    d.wav = wav;
+    d.seis_t = seis_t;
+    d.seis_rec = seis_rec;
+    d.snaps = (R *)snaps;
+    d.I = (R *)I; d.Is = (R *)Is; d.Ic = (R *)Ic; d.Isc = (R *)Isc;
+    return d;
+}
+
+template SimDesc<float> SimBuffers::desc<float>() const;
+template SimDesc<double> SimBuffers::desc<double>() const;
+
+void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
+{
+    const int X = p.thickness_left + p.nx + p.thickness_right;
+    const int Z = p.thickness_up + p.nz + p.thickness_down;
+    const int TZ = 4 * rz;
+    SimDesc<float> d{};
+    d.X = X; d.Z = Z;
+    d.TL = p.thickness_left; d.TR = p.thickness_right; d.TU = p.thickness_up; d.TD = p.thickness_down;
+    // z fastest: vertically adjacent tiles (which share halo rows) are
+    // neighbours in the list and so land on the same XCD
+    for (int x0 = rtm::kHalf; x0 < X - rtm::kHalf; x0 += 64)
+        for (int z0 = rtm::kHalf; z0 < Z - rtm::kHalf; z0 += TZ)
+            out.push_back(make_int4(sim, x0, z0, rtm::tile_class(d, x0, z0, TZ)));
+}
+
+// ---------------------------------------------------------------------------
+// executor
+// ---------------------------------------------------------------------------
+
+int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread)
+{
+    real = real_type;
+    rz = rows_per_thread;
+    if (rz != 1 && rz != 2 && rz != 4 && rz != 8) {
+        rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
+        return -1;
+    }
+    cap_sims = max_sims > 0 ? max_sims : 1;
+    cap_tiles = max_tiles > 0 ? max_tiles : 1;
+    const size_t dsz = real == RTM_REAL_F64 ? sizeof(SimDesc<double>) : sizeof(SimDesc<float>);
+    HIP_TRY(hipMalloc(&d_desc, dsz * cap_sims));
+    HIP_TRY(hipMalloc((void **)&d_n0, sizeof(int) * cap_sims));
+    HIP_TRY(hipMalloc((void **)&d_tiles, sizeof(int4) * cap_tiles));
+    HIP_TRY(hipMalloc((void **)&d_ntiles, sizeof(int)));
+    HIP_TRY(hipMemset(d_desc, 0, dsz * cap_sims));
+    HIP_TRY(hipMemset(d_n0, 0, sizeof(int) * cap_sims));
+    HIP_TRY(hipMemset(d_ntiles, 0, sizeof(int)));
+    h_n0.assign(cap_sims, 0);
+    return 0;
+}
+
+int Exec::drop_graph()
+{
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    gblock = 0;
+    return 0;
+}
+
+void Exec::release()
+{
+    drop_graph();
+    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles};
+    for (void *p : all)
+        if (p) (void)hipFree(p);
+    d_desc = nullptr; d_n0 = nullptr; d_tiles = nullptr; d_ntiles = nullptr;
+}
+
+int Exec::set_tiles(const std::vector<int4> &tiles, hipStream_t s)
+{
+    if ((int)tiles.size() > cap_tiles) {
+        rtm_set_error("tile list of %zu exceeds the executor capacity %d", tiles.size(), cap_tiles);
+        return -1;
+    }
+    ntiles = (int)tiles.size();
+    if (ntiles) HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * ntiles, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int Exec::upload_descs(const void *descs, int n, hipStream_t s)
+{
+    if (n > cap_sims) { rtm_set_error("%d simulations exceed the executor capacity %d", n, cap_sims); return -1; }
+    nsims = n;
+    const size_t dsz = real == RTM_REAL_F64 ? sizeof(SimDesc<double>) : sizeof(SimDesc<float>);
+    HIP_TRY(hipMemcpyAsync(d_desc, descs, dsz * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
+{
+    const int n = (int)n0.size() < cap_sims ? (int)n0.size() : cap_sims;
+    for (int i = 0; i < n; i++) h_n0[i] = n0[i];
+    HIP_TRY(hipMemcpyAsync(d_n0, h_n0.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+template <typename R>
+static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
+{
+    const int grid = 8 * ((e.cap_tiles + 7) / 8);
+    const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
+    switch (e.rz) {
+    case 1: hipLaunchKernelGGL((k_tiles<R, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    case 2: hipLaunchKernelGGL((k_tiles<R, 2>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    case 8: hipLaunchKernelGGL((k_tiles<R, 8>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    default: hipLaunchKernelGGL((k_tiles<R, 4>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    }
+}
+
+int Exec::launch_one(int s_in_block, hipStream_t s)
+{
+    if (real == RTM_REAL_F64) launch_tiles<double>(*this, s_in_block, s);
+    else launch_tiles<float>(*this, s_in_block, s);
+    return 0;
+}
+
+int Exec::run(int steps, int block, hipStream_t s)
+{
+    if (steps <= 0) return 0;
+    int done = 0;
+    if (block > 1 && steps >= block) {
+        if (!gexec || gblock != block) {
+            drop_graph();
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int t = 0; t < block; t++) launch_one(t, s);
+            hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
+            HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            gblock = block;
+        }
+        while (steps - done >= block) {
+            HIP_TRY(hipGraphLaunch(gexec, s));
+            done += block;
+        }
+    }
+    const int rem = steps - done;
+    for (int t = 0; t < rem; t++) launch_one(t, s);
+    if (rem) hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, rem);
+    HIP_TRY(hipGetLastError());
+    for (int i = 0; i < cap_sims; i++) h_n0[i] += steps;
+    return 0;
+}
+
+}  // namespace rtmd

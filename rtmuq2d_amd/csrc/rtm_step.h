@@ -97,42 +97,88 @@ struct StepArgs {
     R *Ic, *Isc;              /* Kahan compensations (M_KAHAN)           */
 };
 
-/* sum_{m=4..1} B_m (f[j+m*s] - f[j-m*s]), as SRC/absorbingBoundaryCondition.c:183-186 */
+/* ---- value forms: every kernel variant evaluates exactly these trees ---- */
+
+/* sum_{m=4..1} B_m (f(+m) - f(-m)), as SRC/absorbingBoundaryCondition.c:183-186;
+ * p[m] = f(+m), q[m] = f(-m) */
 template <typename R>
-RTM_HD R first_diff(const R *f, ptrdiff_t j, ptrdiff_t s)
+RTM_HD R d1_val(R p1, R p2, R p3, R p4, R q1, R q2, R q3, R q4)
 {
-    return ((R(kB4) * (f[j + 4 * s] - f[j - 4 * s]) + R(kB3) * (f[j + 3 * s] - f[j - 3 * s]))
-            + R(kB2) * (f[j + 2 * s] - f[j - 2 * s])) + R(kB1) * (f[j + s] - f[j - s]);
+    return ((R(kB4) * (p4 - q4) + R(kB3) * (p3 - q3)) + R(kB2) * (p2 - q2)) + R(kB1) * (p1 - q1);
 }
 
-/* sum_{m=4..1} A_m (f[j+m*s] + f[j-m*s]) + A0 f[j], as :208-212 */
+/* sum_{m=4..1} A_m (f(+m) + f(-m)) + A0 f(0), as :208-212 */
 template <typename R>
-RTM_HD R second_diff(const R *f, ptrdiff_t j, ptrdiff_t s)
+RTM_HD R d2_val(R c, R p1, R p2, R p3, R p4, R q1, R q2, R q3, R q4)
 {
-    return (((R(kA4) * (f[j + 4 * s] + f[j - 4 * s]) + R(kA3) * (f[j + 3 * s] + f[j - 3 * s]))
-             + R(kA2) * (f[j + 2 * s] + f[j - 2 * s])) + R(kA1) * (f[j + s] + f[j - s]))
-           + R(kA0) * f[j];
+    return (((R(kA4) * (p4 + q4) + R(kA3) * (p3 + q3)) + R(kA2) * (p2 + q2)) + R(kA1) * (p1 + q1))
+           + R(kA0) * c;
 }
 
 /* Interior update, SRC/stencilDFM.c:23-28; operand order inside each
  * bracket is (x-, x+, z-, z+). */
 template <typename R>
-RTM_HD R stencil_point(const R *u, R C, R Uold, ptrdiff_t j, ptrdiff_t P)
+RTM_HD R stencil_val(R C, R Uold, R uc, R xm1, R xp1, R zm1, R zp1, R xm2, R xp2, R zm2, R zp2,
+                     R xm3, R xp3, R zm3, R zp3, R xm4, R xp4, R zm4, R zp4)
 {
-    const R uc = u[j];
-    const R s1 = ((u[j - 1] + u[j + 1]) + u[j - P]) + u[j + P];
-    const R s2 = ((u[j - 2] + u[j + 2]) + u[j - 2 * P]) + u[j + 2 * P];
-    const R s3 = ((u[j - 3] + u[j + 3]) + u[j - 3 * P]) + u[j + 3 * P];
-    const R s4 = ((u[j - 4] + u[j + 4]) + u[j - 4 * P]) + u[j + 4 * P];
+    const R s1 = ((xm1 + xp1) + zm1) + zp1;
+    const R s2 = ((xm2 + xp2) + zm2) + zp2;
+    const R s3 = ((xm3 + xp3) + zm3) + zp3;
+    const R s4 = ((xm4 + xp4) + zm4) + zp4;
     const R lap = ((((R(2.0f) * R(kA0)) * uc + R(kA1) * s1) + R(kA2) * s2) + R(kA3) * s3) + R(kA4) * s4;
     return (C * lap + R(2.0f) * uc) - Uold;
 }
 
-/* psi^{n+1} at one point of a strip: b psi^n + (a D1(u)) / h, as :180-187 */
+/* psi^{n+1} = b psi^n + (a D1(u)) / h, as :180-187 */
+template <typename R>
+RTM_HD R psi_val(R a, R b, R psi_old, R d1, R h1)
+{
+    return b * psi_old + (a * d1) * h1;
+}
+
+/* csi^{n+1} = b csi^n + a (D2(u) / h^2 + dpsi), as :205-213 */
+template <typename R>
+RTM_HD R csi_val(R a, R b, R csi_old, R d2, R dpsi, R h3)
+{
+    return b * csi_old + a * (d2 * h3 + dpsi);
+}
+
+/* (C h^2)(dpsi + csi), the correction added to U, as :215 */
+template <typename R>
+RTM_HD R corr_val(R C, R h2, R dpsi, R cs)
+{
+    return (C * h2) * (dpsi + cs);
+}
+
+/* ---- pointer forms over a strided array ---- */
+
+template <typename R>
+RTM_HD R first_diff(const R *f, ptrdiff_t j, ptrdiff_t s)
+{
+    return d1_val(f[j + s], f[j + 2 * s], f[j + 3 * s], f[j + 4 * s],
+                  f[j - s], f[j - 2 * s], f[j - 3 * s], f[j - 4 * s]);
+}
+
+template <typename R>
+RTM_HD R second_diff(const R *f, ptrdiff_t j, ptrdiff_t s)
+{
+    return d2_val(f[j], f[j + s], f[j + 2 * s], f[j + 3 * s], f[j + 4 * s],
+                  f[j - s], f[j - 2 * s], f[j - 3 * s], f[j - 4 * s]);
+}
+
+template <typename R>
+RTM_HD R stencil_point(const R *u, R C, R Uold, ptrdiff_t j, ptrdiff_t P)
+{
+    return stencil_val(C, Uold, u[j], u[j - 1], u[j + 1], u[j - P], u[j + P],
+                       u[j - 2], u[j + 2], u[j - 2 * P], u[j + 2 * P],
+                       u[j - 3], u[j + 3], u[j - 3 * P], u[j + 3 * P],
+                       u[j - 4], u[j + 4], u[j - 4 * P], u[j + 4 * P]);
+}
+
 template <typename R>
 RTM_HD R psi_next(R a, R b, R psi_old, const R *u, ptrdiff_t j, ptrdiff_t s, R h1)
 {
-    return b * psi_old + (a * first_diff(u, j, s)) * h1;
+    return psi_val(a, b, psi_old, first_diff(u, j, s), h1);
 }
 
 template <typename R>
@@ -175,11 +221,10 @@ RTM_HD R cpml_correction(const StepArgs<R> &A, int i, int k, ptrdiff_t j, int ax
         pp[m] = okp ? psi_next(a[cp], b[cp], psi_in[j + m * s], A.u, j + m * s, s, A.h1) : psi_in[j + m * s];
     }
     psi_out[j] = psi_next(a[c], b[c], psi_in[j], A.u, j, s, A.h1);
-    const R dpsi = (((R(kB4) * (pp[4] - pm[4]) + R(kB3) * (pp[3] - pm[3])) + R(kB2) * (pp[2] - pm[2]))
-                    + R(kB1) * (pp[1] - pm[1])) * A.h1;
-    const R cs = b[c] * csi[j] + a[c] * (second_diff(A.u, j, s) * A.h3 + dpsi);
+    const R dpsi = d1_val(pp[1], pp[2], pp[3], pp[4], pm[1], pm[2], pm[3], pm[4]) * A.h1;
+    const R cs = csi_val(a[c], b[c], csi[j], second_diff(A.u, j, s), dpsi, A.h3);
     csi[j] = cs;
-    return (A.C[j] * A.h2) * (dpsi + cs);
+    return corr_val(A.C[j], A.h2, dpsi, cs);
 }
 
 /* Top-strip correction in the outer columns i < 4 and i >= X-4, 4 <= k < TU-4.
@@ -194,9 +239,9 @@ RTM_HD void cpml_top_edge_point(const StepArgs<R> &A, int i, int k)
 {
     const ptrdiff_t s = (ptrdiff_t)A.P, j = (ptrdiff_t)k * A.P + i;
     const R dpsi = first_diff(A.psiZi, j, s) * A.h1;
-    const R cs = A.bz[k] * A.csiZ[j] + A.az[k] * (second_diff(A.u, j, s) * A.h3 + dpsi);
+    const R cs = csi_val(A.az[k], A.bz[k], A.csiZ[j], second_diff(A.u, j, s), dpsi, A.h3);
     A.csiZ[j] = cs;
-    A.U[j] = A.U[j] + (A.C[j] * A.h2) * (dpsi + cs);
+    A.U[j] = A.U[j] + corr_val(A.C[j], A.h2, dpsi, cs);
 }
 
 /* Kahan-compensated acc += v (fp-contract is off, so the compensation

@@ -5,6 +5,7 @@
 #include <cstddef>
 
 #include "rtm_step.h"
+#include "rtm_tile.h"
 
 template <typename R>
 static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const R *ax, const R *bx,
@@ -29,7 +30,67 @@ static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const 
             for (int c = 0; c < 8; c++) rtm::cpml_top_edge_point(A, c < 4 ? c : A.X - 8 + c, k);
 }
 
+// Emulation of k_tiles for one simulation at step n: every tile, phase by
+// phase, all 256 threads of a phase before the next (the barriers).
+template <typename R, int RZ>
+static void run_tiles(const rtm::SimDesc<R> &S, int n)
+{
+    const rtm::StepView<R> V = rtm::step_view(S, n);
+    if (!V.active) return;
+    auto *L = new rtm::TileLds<R, RZ>();
+    const int TZ = 4 * RZ;
+    for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64)
+        for (int z0 = rtm::kHalf; z0 < S.Z - rtm::kHalf; z0 += TZ) {
+            const int cls = rtm::tile_class(S, x0, z0, TZ);
+            for (int ty = 0; ty < 4; ty++)
+                for (int tx = 0; tx < 64; tx++) rtm::tile_load(*L, S, V, x0, z0, cls, tx, ty);
+            if (cls)
+                for (int ty = 0; ty < 4; ty++)
+                    for (int tx = 0; tx < 64; tx++) rtm::tile_psi(*L, S, V, x0, z0, cls, tx, ty);
+            for (int ty = 0; ty < 4; ty++)
+                for (int tx = 0; tx < 64; tx++) rtm::tile_points(*L, S, V, x0, z0, cls, tx, ty);
+        }
+    delete L;
+}
+
+template <typename R>
+static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *recmap, const float *wav,
+                        const float *seis_t, float *seis_rec)
+{
+    rtm::SimDesc<R> S{};
+    S.X = g[0]; S.Z = g[1]; S.P = g[2];
+    S.TL = g[3]; S.TR = g[4]; S.TU = g[5]; S.TD = g[6];
+    S.nx = g[7]; S.nz = g[8]; S.nt = g[9]; S.TS = g[10]; S.nw = g[11]; S.nrec = g[12];
+    S.phase = g[13]; S.flags = g[14]; S.src_j = g[15]; S.rec_row = g[16];
+    const int n = g[17], rz = g[18];
+    S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
+    S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
+    S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];
+    S.psiX[0] = ptr[7]; S.psiX[1] = ptr[8]; S.psiZ[0] = ptr[9]; S.psiZ[1] = ptr[10];
+    S.csiX = ptr[11]; S.csiZ = ptr[12];
+    S.snaps = ptr[13]; S.I = ptr[14]; S.Is = ptr[15]; S.Ic = ptr[16]; S.Isc = ptr[17];
+    S.recmap = recmap; S.wav = wav; S.seis_t = seis_t; S.seis_rec = seis_rec;
+    switch (rz) {
+    case 1: run_tiles<R, 1>(S, n); break;
+    case 2: run_tiles<R, 2>(S, n); break;
+    case 8: run_tiles<R, 8>(S, n); break;
+    default: run_tiles<R, 4>(S, n); break;
+    }
+}
+
 extern "C" {
+void hx_tiles_f32(const int *g, const float *hs, float *const *ptr, const int *recmap, const float *wav,
+                  const float *seis_t, float *seis_rec)
+{
+    tiles_entry<float>(g, hs, ptr, recmap, wav, seis_t, seis_rec);
+}
+
+void hx_tiles_f64(const int *g, const double *hs, double *const *ptr, const int *recmap,
+                  const float *wav, const float *seis_t, float *seis_rec)
+{
+    tiles_entry<double>(g, hs, ptr, recmap, wav, seis_t, seis_rec);
+}
+
 void hx_step_f32(const int *g, const float *hs, const float *u, float *U, const float *Cf,
                  const float *ax, const float *bx, const float *az, const float *bz, const float *pXi,
                  float *pXo, float *cX, const float *pZi, float *pZo, float *cZ, const int *recmap,

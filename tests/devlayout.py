@@ -128,3 +128,43 @@ class DevSim:
 
     def last_fields(self):
         return self.from_dev(self.u[self.cur ^ 1]), self.from_dev(self.u[self.cur])
+
+    # ---- the tiled kernel (rtm_tile.h), emulated phase by phase ----
+
+    def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False):
+        op = self.op
+        self.reset()
+        nint = op.nx * op.nz
+        TS = op.time_slices
+        fn = self.hx.hx_tiles_f64 if self.dt == np.float64 else self.hx.hx_tiles_f32
+        wav = np.zeros(max(op.n_wavelet, 1), np.float32) if wav is None else np.ascontiguousarray(wav, np.float32)
+        seis_t = (np.zeros((op.nt, op.n_receivers), np.float32) if seis is None
+                  else np.ascontiguousarray(np.asarray(seis, np.float32).T))
+        seis_rec = np.zeros((op.nt, op.n_receivers), np.float32)
+        if snaps is None:
+            snaps = np.zeros((op.nt // TS, op.nz, op.nx), self.dt)
+        I = np.zeros((op.nz, op.nx), self.dt)
+        Is, Ic, Isc = np.zeros_like(I), np.zeros_like(I), np.zeros_like(I)
+        arrs = [self.u[0], self.u[1], self.C, self.ax, self.bx, self.az, self.bz, self.psiX[0],
+                self.psiX[1], self.psiZ[0], self.psiZ[1], self.csiX, self.csiZ, snaps, I, Is, Ic, Isc]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        hs = self.hs[:3].copy()
+        for n in range(op.nt):
+            g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx,
+                          op.nz, op.nt, TS, op.n_wavelet, op.n_receivers, phase, flags, src_j,
+                          self.rec_row, n, rz], np.int32)
+            fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
+        self.cur = op.nt & 1
+        return snaps, I, Is, seis_rec
+
+    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False):
+        op = self.op
+        src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
+        flags = 4 | (2 if record else 0)
+        snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record)
+        return snaps, np.ascontiguousarray(rec.T)
+
+    def backward_tiled(self, seis, snaps, rz=4, kahan=False):
+        flags = 8 | (1 if kahan else 0)
+        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps)
+        return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)

@@ -41,6 +41,33 @@ def test_forward_backward_bitexact(hx, name, dtype):
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
+@pytest.mark.parametrize("name", ["tiny", "asym"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("rz", [1, 2, 4, 8])
+def test_tiled_kernel_bitexact(hx, name, dtype, rz):
+    """The LDS-tiled kernel body (rtm_tile.h), its phases run in sequence for
+    all threads of each tile, against the oracle: fields, snapshots,
+    recorded traces and images."""
+    cfg = common.REF_CONFIGS[name]
+    g = np.load(os.path.join(common.GOLDEN, f"{name}.npz"))
+    op = common.orc_params(cfg)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    S, drec, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=dtype, record=True)
+    ds = DL.DevSim(hx, op, vpad, dtype)
+    S2, rec = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, record=True)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu) and np.array_equal(b, lU)
+    assert np.array_equal(np.ascontiguousarray(S2.transpose(0, 2, 1)), S)
+    assert np.array_equal(rec, drec.astype(np.float32))
+    d = g["seismograms"][0]
+    I, Is, lu2, lU2 = O.backward(op, vpad, d, S)
+    I2, Is2 = ds.backward_tiled(d, S2, rz=rz)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
 def _strips_to_full(op, P, psi, csi):
     X, Z = op.X, op.Z
     XR, ZD = X - op.t_right, Z - op.t_down

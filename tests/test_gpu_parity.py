@@ -63,6 +63,59 @@ def test_forward_backward_bitexact(gpu_available, name, dtype):
         assert np.array_equal(Is2, Is)
 
 
+@pytest.mark.parametrize("rz", [1, 2, 4, 8])
+@pytest.mark.parametrize("block", [0, 7, 100])
+def test_tilings_and_graph_blocks_bitexact(gpu_available, rz, block):
+    """Every tile height and graph-block size gives the same bits (tiny)."""
+    say(f"[gpu] tiling rz={rz} graph_block={block}")
+    cfg = common.REF_CONFIGS["tiny"]
+    g = golden("tiny")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    vpad = O.extend(op, g["models"][1])
+    sx, sz = common.shot_positions(cfg)[1]
+    d = g["seismograms"][1]
+    I_orc, Is_orc = O.rtm_shot(op, sx, sz, vpad, g["wavelet"], d)
+    with R.Plan(p) as plan:
+        plan.set_tiling(rz, block)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.set_seismogram(d)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I, Is = plan.image()
+    assert np.array_equal(I, I_orc) and np.array_equal(Is, Is_orc)
+
+
+def test_injection_outside_stepped_region(gpu_available):
+    """Source and receivers on row 2 of the padded grid (outside the stepped
+    region 4 <= k < Z-4, inside the top CPML): the per-point path with the
+    outer-column top pass and list injection, against the oracle."""
+    say("[gpu] injection outside the stepped region")
+    cfg = dict(common.REF_CONFIGS["tiny"])
+    cfg["depthReceiver"] = 2 - cfg["thicknessUp"]
+    cfg["firstLocationReceiver"] = -cfg["thicknessLeft"]      # first receiver at column 0
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    g = golden("tiny")
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = 10, 2 - cfg["thicknessUp"]
+    S, d, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], record=True)
+    I, Is = O.rtm_shot(op, sx, sz, vpad, g["wavelet"], d)
+    with R.Plan(p) as plan:
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.forward(sx, sz, record=True)
+        a, b = plan.fields()
+        assert np.array_equal(a, lu) and np.array_equal(b, lU)
+        assert np.array_equal(plan.seismogram(), d)
+        plan.set_seismogram(d)
+        plan.zero_image()
+        plan.backward()
+        I2, Is2 = plan.image()
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
 @pytest.mark.parametrize("name", ["tiny", "asym"])
 def test_rtm_routine_matches_reference_golden(gpu_available, name):
     """The drop-in rtm_routine on the reference's files reproduces the
