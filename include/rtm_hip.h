@@ -172,6 +172,30 @@ int rtm_plan_synchronize(rtm_plan *plan);
  * launches; steps = time steps per pass. */
 int rtm_plan_last_timing(rtm_plan *plan, double *fwd_step_ms, double *bwd_step_ms, int *steps);
 
+/* Ensemble of nsims realizations of one geometry (the reference's
+ * simulations, SRC/mpi_rtm_iso2d.c:39-51, here on one device): every
+ * realization has its own velocity, wavefields, CPML state, snapshot store
+ * and image; the shot gather (wavelet, traces) is shared.  All realizations
+ * advance together, one tiled launch per time step in hipGraph blocks. */
+typedef struct rtm_batch rtm_batch;
+
+int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_batch **out);
+int rtm_batch_destroy(rtm_batch *batch);
+size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real);
+int rtm_batch_size(rtm_batch *batch);
+int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block);
+int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
+int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
+int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */
+int rtm_batch_zero_images(rtm_batch *batch);
+int rtm_batch_migrate(rtm_batch *batch, int shot_x, int shot_z);
+int rtm_batch_get_image(rtm_batch *batch, int sim, float *I, float *Is);     /* [Nx][Nz] */
+int rtm_batch_get_image_f64(rtm_batch *batch, int sim, double *I, double *Is);
+/* mean and unbiased variance of the realizations' images, FP64 Welford in
+ * realization order, [Nx][Nz] */
+int rtm_batch_welford(rtm_batch *batch, double *mean, double *var);
+int rtm_batch_last_timing(rtm_batch *batch, double *fwd_step_ms, double *bwd_step_ms, int *steps);
+
 #ifdef __cplusplus
 }
 #endif

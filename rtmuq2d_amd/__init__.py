@@ -127,6 +127,20 @@ def lib() -> C.CDLL:
         "rtm_plan_get_fields_f64": (C.c_int, [plan_p, f64p, f64p]),
         "rtm_plan_synchronize": (C.c_int, [plan_p]),
         "rtm_plan_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
+        "rtm_batch_create": (C.c_int, [P(RtmParams), C.c_int, C.c_int, P(plan_p)]),
+        "rtm_batch_destroy": (C.c_int, [plan_p]),
+        "rtm_batch_footprint": (C.c_size_t, [P(RtmParams), C.c_int, C.c_int]),
+        "rtm_batch_size": (C.c_int, [plan_p]),
+        "rtm_batch_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int]),
+        "rtm_batch_set_velocity": (C.c_int, [plan_p, C.c_int, f32p]),
+        "rtm_batch_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
+        "rtm_batch_set_seismogram": (C.c_int, [plan_p, f32p]),
+        "rtm_batch_zero_images": (C.c_int, [plan_p]),
+        "rtm_batch_migrate": (C.c_int, [plan_p, C.c_int, C.c_int]),
+        "rtm_batch_get_image": (C.c_int, [plan_p, C.c_int, f32p, f32p]),
+        "rtm_batch_get_image_f64": (C.c_int, [plan_p, C.c_int, f64p, f64p]),
+        "rtm_batch_welford": (C.c_int, [plan_p, f64p, f64p]),
+        "rtm_batch_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -301,3 +315,81 @@ def zfp_decompress_file(path: str, d1: int, d2: int, tol: float) -> np.ndarray:
     if rc != 0:
         raise RtmError(f"zfp2d_lossyDecompression({path}) failed")
     return a
+
+
+class Batch:
+    """An ensemble of realizations of one geometry on the device (rtm_batch)."""
+
+    def __init__(self, p: RtmParams, nsims: int, real: int = REAL_F32):
+        self.p = p.copy()
+        self.n = int(nsims)
+        self.real = real
+        h = C.c_void_p()
+        _check(lib().rtm_batch_create(C.byref(self.p), self.n, real, C.byref(h)), "rtm_batch_create")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().rtm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tiling(self, rows_per_thread: int = 8, graph_block: int = 100) -> None:
+        _check(lib().rtm_batch_set_tiling(self.h, int(rows_per_thread), int(graph_block)), "set_tiling")
+
+    def set_velocity(self, sim: int, vpad: np.ndarray) -> None:
+        v = np.ascontiguousarray(vpad, dtype=np.float32)
+        assert v.shape == (self.p.X, self.p.Z), v.shape
+        _check(lib().rtm_batch_set_velocity(self.h, int(sim), fptr(v)), "set_velocity")
+
+    def set_wavelet(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().rtm_batch_set_wavelet(self.h, fptr(w), int(w.size)), "set_wavelet")
+
+    def set_seismogram(self, d: np.ndarray) -> None:
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        assert d.shape == (self.p.number_of_receivers, self.p.number_of_time_steps), d.shape
+        _check(lib().rtm_batch_set_seismogram(self.h, fptr(d)), "set_seismogram")
+
+    def zero_images(self) -> None:
+        _check(lib().rtm_batch_zero_images(self.h), "zero_images")
+
+    def migrate(self, shot_x: int, shot_z: int) -> None:
+        _check(lib().rtm_batch_migrate(self.h, int(shot_x), int(shot_z)), "migrate")
+
+    def image(self, sim: int, f64: bool = False):
+        shape = (self.p.nx, self.p.nz)
+        if f64:
+            I, Is = np.empty(shape, np.float64), np.empty(shape, np.float64)
+            _check(lib().rtm_batch_get_image_f64(self.h, int(sim), dptr(I), dptr(Is)), "get_image_f64")
+        else:
+            I, Is = np.empty(shape, np.float32), np.empty(shape, np.float32)
+            _check(lib().rtm_batch_get_image(self.h, int(sim), fptr(I), fptr(Is)), "get_image")
+        return I, Is
+
+    def welford(self):
+        shape = (self.p.nx, self.p.nz)
+        m, v = np.empty(shape, np.float64), np.empty(shape, np.float64)
+        _check(lib().rtm_batch_welford(self.h, dptr(m), dptr(v)), "welford")
+        return m, v
+
+    def last_timing(self):
+        f, b = C.c_double(), C.c_double()
+        n = C.c_int()
+        _check(lib().rtm_batch_last_timing(self.h, C.byref(f), C.byref(b), C.byref(n)), "last_timing")
+        return f.value, b.value, n.value
+
+
+def batch_footprint(p: RtmParams, nsims: int, real: int = REAL_F32) -> int:
+    return int(lib().rtm_batch_footprint(C.byref(p), int(nsims), real))

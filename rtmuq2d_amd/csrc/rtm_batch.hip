@@ -1,0 +1,315 @@
+// rtm_batch.hip -- an ensemble of simulations on one device, advanced in
+// lockstep by one tiled launch per time step (hipGraph blocks of steps).
+//
+// This is the on-device form of the reference's "embarrassingly parallel
+// over realizations" driver (SRC/mpi_rtm_iso2d.c:39-51: MPI ranks each run a
+// range of simulation ids with no messages): every realization keeps its own
+// wavefields, CPML state, snapshot store and image on HBM, while the shot
+// gather (wavelet, traces, receiver tables, CPML profiles) is shared.  The
+// images are reduced across realizations with a Welford mean/variance in
+// FP64, in realization order.
+#include <cstdlib>
+
+#include "rtm_sim.h"
+
+using namespace rtmd;
+
+namespace {
+
+// mean and M2 over N images in realization order (Welford), FP64
+template <typename R>
+__global__ void k_welford(const R *const *__restrict__ imgs, int N, size_t n, double *__restrict__ mean,
+                          double *__restrict__ m2)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double m = 0.0, s = 0.0;
+        for (int k = 0; k < N; k++) {
+            const double x = (double)imgs[k][q];
+            const double d = x - m;
+            m = m + d / (double)(k + 1);
+            s = s + d * (x - m);
+        }
+        mean[q] = m;
+        m2[q] = s;
+    }
+}
+
+}  // namespace
+
+struct rtm_batch {
+    std::vector<SimBuffers> sims;
+    Exec ex;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int real = RTM_REAL_F32;
+    int rz = 4, graph_block = 100;
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    int last_steps = 0;
+};
+
+static void batch_free(rtm_batch *q)
+{
+    q->ex.release();
+    for (auto &s : q->sims) s.release();
+    q->sims.clear();
+    if (q->ev0) (void)hipEventDestroy(q->ev0);
+    if (q->ev1) (void)hipEventDestroy(q->ev1);
+    if (q->stream) (void)hipStreamDestroy(q->stream);
+}
+
+static int batch_set_tiling(rtm_batch *q, int rz, int block)
+{
+    std::vector<int4> tiles;
+    for (int s = 0; s < (int)q->sims.size(); s++) append_tiles(tiles, s, q->sims[s].p, rz);
+    q->ex.release();
+    if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz)) return -1;
+    if (q->ex.set_tiles(tiles, q->stream)) return -1;
+    q->rz = rz;
+    q->graph_block = block;
+    return 0;
+}
+
+extern "C" size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real)
+{
+    if (!p || nsims < 1 || rtm_params_check(p)) return 0;
+    const size_t esz = real == RTM_REAL_F64 ? 8 : 4;
+    const size_t one = sim_bytes(*p, esz);
+    const size_t shared = 2 * (size_t)p->number_of_time_steps * p->number_of_receivers * 4;
+    return one + (size_t)(nsims - 1) * (one - shared);
+}
+
+extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_batch **out)
+{
+    if (!p || !out || nsims < 1) { rtm_set_error("rtm_batch_create: bad argument"); return -1; }
+    *out = nullptr;
+    if (rtm_params_check(p)) return -1;
+    if (real != RTM_REAL_F32 && real != RTM_REAL_F64) { rtm_set_error("rtm_batch_create: bad real type"); return -1; }
+    if (p->snapshot_mode != RTM_SNAP_FP32) {
+        rtm_set_error("rtm_batch_create: snapshot mode %d not available", p->snapshot_mode);
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(p->device));
+    rtm_batch *q = new rtm_batch();
+    q->real = real;
+    auto fail = [&]() { batch_free(q); delete q; return -1; };
+    if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&q->ev0) != hipSuccess || hipEventCreate(&q->ev1) != hipSuccess) {
+        rtm_set_error("rtm_batch_create: stream/event creation failed");
+        return fail();
+    }
+    q->sims.resize(nsims);
+    for (int s = 0; s < nsims; s++)
+        if (q->sims[s].alloc(*p, real, s ? &q->sims[0] : nullptr)) return fail();
+    const char *env = getenv("RTM_TILE_RZ");
+    if (batch_set_tiling(q, env && *env ? atoi(env) : 8, 100)) return fail();
+    *out = q;
+    return 0;
+}
+
+extern "C" int rtm_batch_destroy(rtm_batch *q)
+{
+    if (!q) return 0;
+    (void)hipSetDevice(q->sims.empty() ? 0 : q->sims[0].p.device);
+    (void)hipStreamSynchronize(q->stream);
+    batch_free(q);
+    delete q;
+    return 0;
+}
+
+extern "C" int rtm_batch_set_tiling(rtm_batch *q, int rows_per_thread, int graph_block)
+{
+    if (!q) { rtm_set_error("rtm_batch_set_tiling: null batch"); return -1; }
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return batch_set_tiling(q, rows_per_thread, graph_block);
+}
+
+extern "C" int rtm_batch_size(rtm_batch *q) { return q ? (int)q->sims.size() : 0; }
+
+extern "C" int rtm_batch_set_velocity(rtm_batch *q, int sim, const float *vpad)
+{
+    if (!q || !vpad || sim < 0 || sim >= (int)q->sims.size()) {
+        rtm_set_error("rtm_batch_set_velocity: bad argument");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(q->sims[sim].p.device));
+    return q->sims[sim].set_velocity(vpad, q->stream);
+}
+
+extern "C" int rtm_batch_set_wavelet(rtm_batch *q, const float *w, int n)
+{
+    if (!q || !w) { rtm_set_error("rtm_batch_set_wavelet: null argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    if (q->sims[0].set_wavelet(w, n)) return -1;
+    for (auto &s : q->sims) s.wav_host = q->sims[0].wav_host;
+    return 0;
+}
+
+extern "C" int rtm_batch_set_seismogram(rtm_batch *q, const float *d)
+{
+    if (!q || !d) { rtm_set_error("rtm_batch_set_seismogram: null argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    if (q->sims[0].set_seismogram(d, q->stream)) return -1;
+    for (auto &s : q->sims) s.have_seis = true;
+    return 0;
+}
+
+extern "C" int rtm_batch_zero_images(rtm_batch *q)
+{
+    if (!q) { rtm_set_error("rtm_batch_zero_images: null batch"); return -1; }
+    for (auto &s : q->sims) s.zero_image(q->stream);
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return 0;
+}
+
+template <typename R>
+static int batch_pass(rtm_batch *q, int phase, int src_j)
+{
+    const int N = (int)q->sims.size();
+    std::vector<SimDesc<R>> d(N);
+    for (int s = 0; s < N; s++) {
+        SimBuffers &b = q->sims[s];
+        b.reset_state(q->stream);
+        d[s] = b.desc<R>();
+        d[s].phase = phase;
+        if (phase == rtm::PH_FORWARD) {
+            d[s].flags = rtm::F_SNAP;
+            d[s].src_j = src_j;
+        } else {
+            d[s].flags = rtm::F_IMAGE | (b.p.image_kahan ? rtm::F_KAHAN : 0);
+        }
+    }
+    if (q->ex.upload_descs(d.data(), N, q->stream)) return -1;
+    if (q->ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
+    const int nt = q->sims[0].p.number_of_time_steps;
+    HIP_TRY(hipEventRecord(q->ev0, q->stream));
+    if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+    HIP_TRY(hipEventRecord(q->ev1, q->stream));
+    HIP_TRY(hipEventSynchronize(q->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
+    (phase == rtm::PH_FORWARD ? q->fwd_ms : q->bwd_ms) = ms;
+    q->last_steps = nt;
+    return 0;
+}
+
+// Forward then backward for every realization, one shot, images accumulated.
+extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
+{
+    if (!q) { rtm_set_error("rtm_batch_migrate: null batch"); return -1; }
+    const SimBuffers &b0 = q->sims[0];
+    const rtm_params &p = b0.p;
+    HIP_TRY(hipSetDevice(p.device));
+    for (auto &s : q->sims)
+        if (!s.have_velocity) { rtm_set_error("rtm_batch_migrate: a realization has no velocity"); return -1; }
+    if (!b0.have_seis || b0.wav_host.size() != (size_t)p.sampling_wavelet) {
+        rtm_set_error("rtm_batch_migrate: wavelet or seismogram not set");
+        return -1;
+    }
+    const int xs = shot_x + p.thickness_left, zs = shot_z + p.thickness_up;
+    if (xs < rtm::kHalf || xs >= b0.X - rtm::kHalf || zs < rtm::kHalf || zs >= b0.Z - rtm::kHalf || b0.n_out) {
+        rtm_set_error("rtm_batch_migrate: source and receivers must lie in the stepped region "
+                      "4 <= i < X-4, 4 <= k < Z-4 (use a plan otherwise)");
+        return -1;
+    }
+    const int src_j = zs * b0.P + xs;
+    int rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_FORWARD, src_j)
+                                     : batch_pass<float>(q, rtm::PH_FORWARD, src_j);
+    if (rc == 0)
+        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_BACKWARD, -1)
+                                     : batch_pass<float>(q, rtm::PH_BACKWARD, -1);
+    for (auto &s : q->sims) s.have_snaps = true;
+    return rc;
+}
+
+template <typename R, typename O>
+static int batch_image_t(rtm_batch *q, int sim, O *I, O *Is)
+{
+    const SimBuffers &b = q->sims[sim];
+    const int nx = b.p.nx, nz = b.p.nz;
+    std::vector<R> h(b.nint);
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    void *src[2] = {b.I, b.Is};
+    O *dst[2] = {I, Is};
+    for (int t = 0; t < 2; t++) {
+        if (!dst[t]) continue;
+        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * b.nint, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nx; i++)
+            for (int k = 0; k < nz; k++) dst[t][(size_t)i * nz + k] = O(h[(size_t)k * nx + i]);
+    }
+    return 0;
+}
+
+extern "C" int rtm_batch_get_image(rtm_batch *q, int sim, float *I, float *Is)
+{
+    if (!q || !I || sim < 0 || sim >= (int)q->sims.size()) { rtm_set_error("rtm_batch_get_image: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[sim].p.device));
+    return q->real == RTM_REAL_F64 ? batch_image_t<double, float>(q, sim, I, Is) : batch_image_t<float, float>(q, sim, I, Is);
+}
+
+extern "C" int rtm_batch_get_image_f64(rtm_batch *q, int sim, double *I, double *Is)
+{
+    if (!q || !I || sim < 0 || sim >= (int)q->sims.size()) { rtm_set_error("rtm_batch_get_image_f64: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[sim].p.device));
+    return q->real == RTM_REAL_F64 ? batch_image_t<double, double>(q, sim, I, Is) : batch_image_t<float, double>(q, sim, I, Is);
+}
+
+// Mean and variance (M2 / (N-1); 0 when N = 1) of the realizations' images,
+// FP64 Welford in realization order; [Nx][Nz] layout.
+template <typename R>
+static int batch_welford_t(rtm_batch *q, double *mean, double *var)
+{
+    const int N = (int)q->sims.size();
+    const SimBuffers &b0 = q->sims[0];
+    const size_t n = b0.nint;
+    std::vector<const R *> ptrs(N);
+    for (int s = 0; s < N; s++) ptrs[s] = (const R *)q->sims[s].I;
+    const R **d_ptrs = nullptr;
+    double *d_mean = nullptr, *d_m2 = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&d_ptrs, sizeof(R *) * N) != hipSuccess || hipMalloc(&d_mean, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&d_m2, sizeof(double) * n) != hipSuccess) {
+        rtm_set_error("rtm_batch_welford: hipMalloc failed");
+        rc = -1;
+    }
+    std::vector<double> hm(n), hv(n);
+    if (!rc && hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(R *) * N, hipMemcpyHostToDevice, q->stream) != hipSuccess) rc = -1;
+    if (!rc) {
+        hipLaunchKernelGGL(k_welford<R>, dim3(1024), dim3(256), 0, q->stream, d_ptrs, N, n, d_mean, d_m2);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(q->stream) != hipSuccess ||
+            hipMemcpy(hm.data(), d_mean, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hv.data(), d_m2, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+            rtm_set_error("rtm_batch_welford: device error");
+            rc = -1;
+        }
+    }
+    if (d_ptrs) (void)hipFree(d_ptrs);
+    if (d_mean) (void)hipFree(d_mean);
+    if (d_m2) (void)hipFree(d_m2);
+    if (rc) return rc;
+    const int nx = b0.p.nx, nz = b0.p.nz;
+    for (int i = 0; i < nx; i++)
+        for (int k = 0; k < nz; k++) {
+            const size_t q2 = (size_t)k * nx + i;
+            mean[(size_t)i * nz + k] = hm[q2];
+            if (var) var[(size_t)i * nz + k] = N > 1 ? hv[q2] / (double)(N - 1) : 0.0;
+        }
+    return 0;
+}
+
+extern "C" int rtm_batch_welford(rtm_batch *q, double *mean, double *var)
+{
+    if (!q || !mean) { rtm_set_error("rtm_batch_welford: null argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    return q->real == RTM_REAL_F64 ? batch_welford_t<double>(q, mean, var) : batch_welford_t<float>(q, mean, var);
+}
+
+extern "C" int rtm_batch_last_timing(rtm_batch *q, double *fwd_step_ms, double *bwd_step_ms, int *steps)
+{
+    if (!q) { rtm_set_error("rtm_batch_last_timing: null batch"); return -1; }
+    const double n = q->last_steps ? (double)q->last_steps : 1.0;
+    if (fwd_step_ms) *fwd_step_ms = q->fwd_ms / n;
+    if (bwd_step_ms) *bwd_step_ms = q->bwd_ms / n;
+    if (steps) *steps = q->last_steps;
+    return 0;
+}

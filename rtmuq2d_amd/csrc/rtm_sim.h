@@ -62,14 +62,17 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const SimDesc<R> &S = sims[tile.x];
     const rtm::StepView<R> V = rtm::step_view(S, n0[tile.x] + s);
     if (!V.active) return;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    rtm::tile_load(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+    const int tx = threadIdx.x & 63;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
+    rtm::TileRegs<R, RZ> G;
+    rtm::tile_fetch(G, S, V, tile.y, tile.z, tile.w, tx, ty);
+    rtm::tile_stage(L, G, tx, ty);
     __syncthreads();
     if (tile.w) {
-        rtm::tile_psi(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+        rtm::tile_psi(L, G, S, tile.y, tile.z, tile.w, tx, ty);
         __syncthreads();
     }
-    rtm::tile_points(L, S, V, tile.y, tile.z, tile.w, tx, ty);
+    rtm::tile_points(L, G, S, V, tile.y, tile.z, tile.w, tx, ty);
 }
 
 // n0[i] += block for every simulation (the last node of a captured block)
@@ -231,7 +234,10 @@ struct SimBuffers {
     bool have_velocity = false, have_seis = false, have_snaps = false;
     std::vector<void *> owned;
 
-    int alloc(const rtm_params &pp, int real_type);
+    // `share` (same geometry): take its profiles, receiver tables, wavelet
+    // and trace buffers instead of allocating copies (an ensemble over one
+    // shot gather)
+    int alloc(const rtm_params &pp, int real_type, const SimBuffers *share = nullptr);
     void release();
     int upload_profiles();
     int upload_receivers();

@@ -23,7 +23,7 @@ size_t sim_bytes(const rtm_params &p, size_t esz)
     return b;
 }
 
-int SimBuffers::alloc(const rtm_params &pp, int real_type)
+int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *share)
 {
     p = pp;
     real = real_type;
@@ -54,32 +54,43 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type)
     if (!rc) rc = get(&psiZ[1], fb);
     if (!rc) rc = get(&csiX, fb);
     if (!rc) rc = get(&csiZ, fb);
-    if (!rc) rc = get(&ax, X * esz);
-    if (!rc) rc = get(&bx, X * esz);
-    if (!rc) rc = get(&az, Z * esz);
-    if (!rc) rc = get(&bz, Z * esz);
-    if (!rc) rc = get((void **)&recmap, sizeof(int) * X);
-    if (!rc) rc = get((void **)&rec_j, sizeof(int) * nrec);
-    if (!rc) rc = get((void **)&out_j, sizeof(int) * nrec);
-    if (!rc) rc = get((void **)&out_r, sizeof(int) * nrec);
-    if (!rc) rc = get((void **)&wav, sizeof(float) * (size_t)p.sampling_wavelet);
-    if (!rc) rc = get((void **)&seis_t, sizeof(float) * nt * nrec);
-    if (!rc) rc = get((void **)&seis_rec, sizeof(float) * nt * nrec);
     if (!rc) rc = get(&snaps, nsnap * nint * esz);
     if (!rc) rc = get(&I, nint * esz);
     if (!rc) rc = get(&Is, nint * esz);
     if (!rc) rc = get(&Ic, nint * esz);
     if (!rc) rc = get(&Isc, nint * esz);
-    if (!rc) rc = upload_profiles();
-    if (!rc) rc = upload_receivers();
+    if (!rc && share) {
+        ax = share->ax; bx = share->bx; az = share->az; bz = share->bz;
+        recmap = share->recmap; rec_j = share->rec_j; out_j = share->out_j; out_r = share->out_r;
+        n_out = share->n_out; rec_dup = share->rec_dup;
+        wav = share->wav; seis_t = share->seis_t; seis_rec = share->seis_rec;
+        wav_host = share->wav_host;
+        have_seis = share->have_seis;
+    } else if (!rc) {
+        if (!rc) rc = get(&ax, X * esz);
+        if (!rc) rc = get(&bx, X * esz);
+        if (!rc) rc = get(&az, Z * esz);
+        if (!rc) rc = get(&bz, Z * esz);
+        if (!rc) rc = get((void **)&recmap, sizeof(int) * X);
+        if (!rc) rc = get((void **)&rec_j, sizeof(int) * nrec);
+        if (!rc) rc = get((void **)&out_j, sizeof(int) * nrec);
+        if (!rc) rc = get((void **)&out_r, sizeof(int) * nrec);
+        if (!rc) rc = get((void **)&wav, sizeof(float) * (size_t)p.sampling_wavelet);
+        if (!rc) rc = get((void **)&seis_t, sizeof(float) * nt * nrec);
+        if (!rc) rc = get((void **)&seis_rec, sizeof(float) * nt * nrec);
+        if (!rc) rc = upload_profiles();
+        if (!rc) rc = upload_receivers();
+        if (!rc && (hipMemset(seis_rec, 0, sizeof(float) * nt * nrec) != hipSuccess ||
+                    hipMemset(seis_t, 0, sizeof(float) * nt * nrec) != hipSuccess)) {
+            rtm_set_error("SimBuffers: hipMemset failed");
+            rc = -1;
+        }
+    }
     if (!rc) {
         // velocity-independent state starts at zero
         void *z[] = {I, Is, Ic, Isc, C};
         for (void *ptr : z)
             if (hipMemset(ptr, 0, ptr == C ? fb : nint * esz) != hipSuccess) rc = -1;
-        if (hipMemset(seis_rec, 0, sizeof(float) * nt * nrec) != hipSuccess ||
-            hipMemset(seis_t, 0, sizeof(float) * nt * nrec) != hipSuccess)
-            rc = -1;
         if (rc) rtm_set_error("SimBuffers: hipMemset failed");
     }
     if (rc) release();

@@ -139,56 +139,153 @@ RTM_HD int tile_class(const SimDesc<R> &S, int x0, int z0, int TZ)
     return cls;
 }
 
-/* Phase 1: u^n into LDS.  tx in [0,64), ty in [0,4). */
+/* Per-thread values fetched from HBM before the first barrier, so that the
+ * whole step costs one memory round trip: the thread's share of the u tile,
+ * and the C, U_old, csi, psi and profile values its later phases need. */
 template <typename R, int RZ>
-RTM_HD void tile_load(TileLds<R, RZ> &L, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
-                      int cls, int tx, int ty)
+struct TileRegs {
+    R u0[RZ + 4], u1[RZ + 4];   /* LDS rows ty+4t, columns tx and 64+tx */
+    R C[RZ], Uold[RZ];          /* own points: rows ty*RZ+rr, column tx */
+    R csx[RZ], csz[RZ];
+    R px0[RZ], px1[RZ];         /* psiX^n at the psi-phase points       */
+    R pz[RZ + 2];               /* psiZ^n at the psi-phase points       */
+    R ax0, bx0, ax1, bx1;       /* x profiles at psi columns tx-4, tx+60 */
+    R axo, bxo;                 /* x profiles at the own column         */
+    R sv[RZ], Iv[RZ], Isv[RZ];  /* imaging: snapshot and image accumulators */
+    R Icv[RZ], Iscv[RZ];        /* Kahan compensations                  */
+    int recr;                   /* receiver at the own column (rec_row), or -1 */
+    R recv;                     /* its trace sample (backward)          */
+};
+
+/* Phase 1a: issue every global load of the step.  tx in [0,64), ty in [0,4). */
+template <typename R, int RZ>
+RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
+                       int cls, int tx, int ty)
 {
     constexpr int TZ = 4 * RZ;
     const int hx = (cls & T_XSTRIP) ? 8 : 4, hz = (cls & T_ZSTRIP) ? 8 : 4;
-    const int w = 64 + 2 * hx;
-    for (int r = ty; r < TZ + 2 * hz; r += 4) {
-        const int k = z0 - hz + r;
-        const bool krow = k >= 0 && k < S.Z;
-        const R *row = V.u + (ptrdiff_t)k * S.P;
-        for (int c = tx; c < w; c += 64) {
-            const int i = x0 - hx + c;
-            L.u[r + 8 - hz][c + 8 - hx] = (krow && i >= 0 && i < S.X) ? row[i] : R(0);
+    const int i0 = x0 - 8 + tx, i1 = x0 + 56 + tx;          /* LDS columns tx and 64+tx */
+    const bool c0 = tx >= 8 - hx && i0 >= 0 && i0 < S.X;
+    const bool c1 = tx < 8 + hx && i1 < S.X;
+#pragma unroll
+    for (int t = 0; t < RZ + 4; t++) {
+        const int r = ty + 4 * t, k = z0 - 8 + r;
+        const bool rok = r >= 8 - hz && r < TZ + 8 + hz && k >= 0 && k < S.Z;
+        const ptrdiff_t base = (ptrdiff_t)k * S.P;
+        G.u0[t] = (rok && c0) ? V.u[base + i0] : R(0);
+        G.u1[t] = (rok && c1) ? V.u[base + i1] : R(0);
+    }
+    const int i = x0 + tx;
+    const bool col_ok = i < S.X - kHalf;
+    const bool inX = col_ok && (i < S.TL - kHalf || i >= S.X - S.TR + kHalf);
+    G.axo = inX ? S.ax[i] : R(0);
+    G.bxo = inX ? S.bx[i] : R(0);
+#pragma unroll
+    for (int rr = 0; rr < RZ; rr++) {
+        const int k = z0 + ty * RZ + rr;
+        const bool ok = col_ok && k < S.Z - kHalf;
+        const ptrdiff_t j = (ptrdiff_t)k * S.P + i;
+        G.C[rr] = ok ? S.C[j] : R(0);
+        G.Uold[rr] = ok ? V.U[j] : R(0);
+        G.csx[rr] = (ok && inX) ? S.csiX[j] : R(0);
+        const bool inZ = ok && (k < S.TU - kHalf || k >= S.Z - S.TD + kHalf);
+        G.csz[rr] = inZ ? S.csiZ[j] : R(0);
+        /* imaging operands (backward steps that image) */
+        const int ii = i - S.TL, kk = k - S.TU;
+        const bool img = ok && V.snap && !V.forward && ii >= 0 && ii < S.nx && kk >= 0 && kk < S.nz;
+        const ptrdiff_t q = (ptrdiff_t)kk * S.nx + ii;
+        G.sv[rr] = img ? V.snap[q] : R(0);
+        G.Iv[rr] = img ? S.I[q] : R(0);
+        G.Isv[rr] = img ? S.Is[q] : R(0);
+        const bool kah = img && (S.flags & F_KAHAN);
+        G.Icv[rr] = kah ? S.Ic[q] : R(0);
+        G.Iscv[rr] = kah ? S.Isc[q] : R(0);
+    }
+    /* receiver on the own column, if the receiver row is one of the own rows */
+    const int rrow = S.rec_row - (z0 + ty * RZ);
+    const bool has_rec = col_ok && rrow >= 0 && rrow < RZ && S.rec_row < S.Z - kHalf &&
+                         (!V.forward || V.rec_row_out);
+    G.recr = has_rec ? S.recmap[i] : -1;
+    G.recv = (!V.forward && G.recr >= 0) ? R(V.seis_row[G.recr]) : R(0);
+    if (cls & T_XSTRIP) {
+        const int ia = x0 - 4 + tx, ib = x0 + 60 + tx;     /* psi columns c = tx, 64+tx */
+        const bool oka = x_strip(S, ia), okb = tx < 8 && x_strip(S, ib);
+        G.ax0 = oka ? S.ax[ia] : R(0);
+        G.bx0 = oka ? S.bx[ia] : R(0);
+        G.ax1 = okb ? S.ax[ib] : R(0);
+        G.bx1 = okb ? S.bx[ib] : R(0);
+#pragma unroll
+        for (int t = 0; t < RZ; t++) {
+            const int k = z0 + ty + 4 * t;
+            const bool kok = k < S.Z - kHalf;
+            G.px0[t] = (kok && oka) ? V.psiXi[(ptrdiff_t)k * S.P + ia] : R(0);
+            G.px1[t] = (kok && okb) ? V.psiXi[(ptrdiff_t)k * S.P + ib] : R(0);
         }
+    }
+    if (cls & T_ZSTRIP) {
+#pragma unroll
+        for (int t = 0; t < RZ + 2; t++) {
+            const int k = z0 - 4 + ty + 4 * t;
+            G.pz[t] = (col_ok && z_strip(S, k)) ? V.psiZi[(ptrdiff_t)k * S.P + i] : R(0);
+        }
+    }
+}
+
+/* Phase 1b: the fetched u into LDS (rows z0-8 .., columns x0-8 ..). */
+template <typename R, int RZ>
+RTM_HD void tile_stage(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, int tx, int ty)
+{
+#pragma unroll
+    for (int t = 0; t < RZ + 4; t++) {
+        L.u[ty + 4 * t][tx] = G.u0[t];
+        if (tx < 16) L.u[ty + 4 * t][64 + tx] = G.u1[t];
     }
 }
 
 /* Phase 2 (strip tiles): psi^{n+1} over the tile plus a radius-4 halo. */
 template <typename R, int RZ>
-RTM_HD void tile_psi(TileLds<R, RZ> &L, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
+RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S, int x0, int z0,
                      int cls, int tx, int ty)
 {
-    constexpr int TZ = 4 * RZ;
     if (cls & T_XSTRIP) {
-        for (int r = ty; r < TZ; r += 4) {
-            const int k = z0 + r;
-            for (int c = tx; c < 72; c += 64) {
-                const int i = x0 - 4 + c;
+        const bool oka = x_strip(S, x0 - 4 + tx), okb = tx < 8 && x_strip(S, x0 + 60 + tx);
+#pragma unroll
+        for (int t = 0; t < RZ; t++) {
+            const int r = ty + 4 * t;
+            const bool kok = z0 + r < S.Z - kHalf;
+            {
+                const R *lu = &L.u[r + 8][tx + 4];
                 R v = R(0);
-                if (k < S.Z - kHalf && x_strip(S, i)) {
-                    const R *lu = &L.u[r + 8][c + 4];
+                if (kok && oka) {
                     const R d1 = d1_val(lu[1], lu[2], lu[3], lu[4], lu[-1], lu[-2], lu[-3], lu[-4]);
-                    v = psi_val(S.ax[i], S.bx[i], V.psiXi[(ptrdiff_t)k * S.P + i], d1, S.h1);
+                    v = psi_val(G.ax0, G.bx0, G.px0[t], d1, S.h1);
                 }
-                L.px[r][c] = v;
+                L.px[r][tx] = v;
+            }
+            if (tx < 8) {
+                const R *lu = &L.u[r + 8][tx + 68];
+                R v = R(0);
+                if (kok && okb) {
+                    const R d1 = d1_val(lu[1], lu[2], lu[3], lu[4], lu[-1], lu[-2], lu[-3], lu[-4]);
+                    v = psi_val(G.ax1, G.bx1, G.px1[t], d1, S.h1);
+                }
+                L.px[r][64 + tx] = v;
             }
         }
     }
     if (cls & T_ZSTRIP) {
         const int i = x0 + tx;
-        for (int r = ty; r < TZ + 8; r += 4) {
+        const int lc = tx + 8;
+#pragma unroll
+        for (int t = 0; t < RZ + 2; t++) {
+            const int r = ty + 4 * t;
             const int k = z0 - 4 + r;
             R v = R(0);
             if (i < S.X - kHalf && z_strip(S, k)) {
-                const int lr = r + 4, lc = tx + 8;
+                const int lr = r + 4;
                 const R d1 = d1_val(L.u[lr + 1][lc], L.u[lr + 2][lc], L.u[lr + 3][lc], L.u[lr + 4][lc],
                                     L.u[lr - 1][lc], L.u[lr - 2][lc], L.u[lr - 3][lc], L.u[lr - 4][lc]);
-                v = psi_val(S.az[k], S.bz[k], V.psiZi[(ptrdiff_t)k * S.P + i], d1, S.h1);
+                v = psi_val(S.az[k], S.bz[k], G.pz[t], d1, S.h1);
             }
             L.pz[r][tx] = v;
         }
@@ -197,8 +294,8 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const SimDesc<R> &S, const StepView<R> &
 
 /* Phase 3: the fused update of this thread's RZ points. */
 template <typename R, int RZ>
-RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const StepView<R> &V, int x0,
-                        int z0, int cls, int tx, int ty)
+RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
+                        const StepView<R> &V, int x0, int z0, int cls, int tx, int ty)
 {
     const int i = x0 + tx;
     if (i >= S.X - kHalf) return;
@@ -220,8 +317,8 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const Step
         const R xm3 = lu[-3], xp3 = lu[3], xm4 = lu[-4], xp4 = lu[4];
         const R zm1 = col[rr + 3], zp1 = col[rr + 5], zm2 = col[rr + 2], zp2 = col[rr + 6];
         const R zm3 = col[rr + 1], zp3 = col[rr + 7], zm4 = col[rr + 0], zp4 = col[rr + 8];
-        const R Cj = S.C[j];
-        R U = stencil_val(Cj, V.U[j], uc, xm1, xp1, zm1, zp1, xm2, xp2, zm2, zp2, xm3, xp3, zm3, zp3,
+        const R Cj = G.C[rr];
+        R U = stencil_val(Cj, G.Uold[rr], uc, xm1, xp1, zm1, zp1, xm2, xp2, zm2, zp2, xm3, xp3, zm3, zp3,
                           xm4, xp4, zm4, zp4);
         if (cls) {
             const bool inT = k < S.TU - kHalf, inB = k >= S.Z - S.TD + kHalf;
@@ -231,7 +328,7 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const Step
                 V.psiXo[j] = p[0];
                 const R dpsi = d1_val(p[1], p[2], p[3], p[4], p[-1], p[-2], p[-3], p[-4]) * S.h1;
                 const R d2 = d2_val(uc, xp1, xp2, xp3, xp4, xm1, xm2, xm3, xm4);
-                const R cs = csi_val(S.ax[i], S.bx[i], S.csiX[j], d2, dpsi, S.h3);
+                const R cs = csi_val(G.axo, G.bxo, G.csx[rr], d2, dpsi, S.h3);
                 S.csiX[j] = cs;
                 corrX = corr_val(Cj, S.h2, dpsi, cs);
             }
@@ -242,7 +339,7 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const Step
                                       L.pz[pr - 1][tx], L.pz[pr - 2][tx], L.pz[pr - 3][tx], L.pz[pr - 4][tx])
                                * S.h1;
                 const R d2 = d2_val(uc, zp1, zp2, zp3, zp4, zm1, zm2, zm3, zm4);
-                const R cs = csi_val(S.az[k], S.bz[k], S.csiZ[j], d2, dpsi, S.h3);
+                const R cs = csi_val(S.az[k], S.bz[k], G.csz[rr], d2, dpsi, S.h3);
                 S.csiZ[j] = cs;
                 corrZ = corr_val(Cj, S.h2, dpsi, cs);
             }
@@ -252,17 +349,14 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const Step
             if (inR) U = U + corrX;
             if (inB) U = U + corrZ;
         }
+        const bool rec_here = k == S.rec_row && G.recr >= 0;
         if (V.forward) {
             if (V.src && j == S.src_j) U = U + V.src_w * Cj;
-        } else if (k == S.rec_row) {
-            const int r = S.recmap[i];
-            if (r >= 0) U = R(V.seis_row[r]);
+        } else if (rec_here) {
+            U = G.recv;
         }
         V.U[j] = U;
-        if (V.forward && V.rec_row_out && k == S.rec_row) {
-            const int r = S.recmap[i];
-            if (r >= 0) V.rec_row_out[r] = float(U);
-        }
+        if (V.forward && rec_here) V.rec_row_out[G.recr] = float(U);
         if (V.snap) {
             const int ii = i - S.TL, kk = k - S.TU;
             if (ii >= 0 && ii < S.nx && kk >= 0 && kk < S.nz) {
@@ -270,15 +364,15 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const SimDesc<R> &S, const Step
                 if (V.forward) {
                     V.snap[q] = U;
                 } else {
-                    const R s = V.snap[q];
+                    const R s = G.sv[rr];
                     if (S.flags & F_KAHAN) {
-                        R I = S.I[q], Ic = S.Ic[q], Is = S.Is[q], Isc = S.Isc[q];
+                        R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
                         kahan_add(I, Ic, s * U);
                         kahan_add(Is, Isc, s * s);
                         S.I[q] = I; S.Ic[q] = Ic; S.Is[q] = Is; S.Isc[q] = Isc;
                     } else {
-                        S.I[q] = S.I[q] + s * U;
-                        S.Is[q] = S.Is[q] + s * s;
+                        S.I[q] = G.Iv[rr] + s * U;
+                        S.Is[q] = G.Isv[rr] + s * s;
                     }
                 }
             }

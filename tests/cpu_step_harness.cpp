@@ -38,18 +38,24 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n)
     const rtm::StepView<R> V = rtm::step_view(S, n);
     if (!V.active) return;
     auto *L = new rtm::TileLds<R, RZ>();
+    auto *G = new rtm::TileRegs<R, RZ>[256];
     const int TZ = 4 * RZ;
     for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64)
         for (int z0 = rtm::kHalf; z0 < S.Z - rtm::kHalf; z0 += TZ) {
             const int cls = rtm::tile_class(S, x0, z0, TZ);
+            // the LDS holds stale data from the previous tile, as on the GPU
             for (int ty = 0; ty < 4; ty++)
-                for (int tx = 0; tx < 64; tx++) rtm::tile_load(*L, S, V, x0, z0, cls, tx, ty);
+                for (int tx = 0; tx < 64; tx++) {
+                    rtm::tile_fetch(G[ty * 64 + tx], S, V, x0, z0, cls, tx, ty);
+                    rtm::tile_stage(*L, G[ty * 64 + tx], tx, ty);
+                }
             if (cls)
                 for (int ty = 0; ty < 4; ty++)
-                    for (int tx = 0; tx < 64; tx++) rtm::tile_psi(*L, S, V, x0, z0, cls, tx, ty);
+                    for (int tx = 0; tx < 64; tx++) rtm::tile_psi(*L, G[ty * 64 + tx], S, x0, z0, cls, tx, ty);
             for (int ty = 0; ty < 4; ty++)
-                for (int tx = 0; tx < 64; tx++) rtm::tile_points(*L, S, V, x0, z0, cls, tx, ty);
+                for (int tx = 0; tx < 64; tx++) rtm::tile_points(*L, G[ty * 64 + tx], S, V, x0, z0, cls, tx, ty);
         }
+    delete[] G;
     delete L;
 }
 

@@ -87,6 +87,41 @@ def test_tilings_and_graph_blocks_bitexact(gpu_available, rz, block):
     assert np.array_equal(I, I_orc) and np.array_equal(Is, Is_orc)
 
 
+@pytest.mark.parametrize("rz", [2, 8])
+def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
+    """A batch of realizations advanced in lockstep: each image bit-identical
+    to the oracle's for that realization (alone or in the batch), and the
+    FP64 Welford mean/variance equal to numpy's over the same images."""
+    say(f"[gpu] batch ensemble rz={rz}")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    sx, sz = common.shot_positions(cfg)[0]
+    d = g["seismograms"][0]
+    rng = np.random.default_rng(12)
+    models = [g["models"][0] * np.float32(1 + 0.05 * k) + rng.uniform(-5, 5, g["models"][0].shape).astype(np.float32)
+              for k in range(5)]
+    vpads = [O.extend(op, m) for m in models]
+    refs = [O.rtm_shot(op, sx, sz, v, g["wavelet"], d) for v in vpads]
+    with R.Batch(p, len(models)) as b:
+        b.set_tiling(rz, 100)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(g["wavelet"])
+        b.set_seismogram(d)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k in range(len(models)):
+            I, Is = b.image(k)
+            assert np.array_equal(I, refs[k][0]), k
+            assert np.array_equal(Is, refs[k][1]), k
+        mean, var = b.welford()
+    stack = np.stack([r[0].astype(np.float64) for r in refs])
+    assert np.allclose(mean, stack.mean(0), rtol=1e-13, atol=1e-13 * np.abs(stack).max())
+    assert np.allclose(var, stack.var(0, ddof=1), rtol=1e-10, atol=1e-12 * stack.var(0).max())
+
+
 def test_injection_outside_stepped_region(gpu_available):
     """Source and receivers on row 2 of the padded grid (outside the stepped
     region 4 <= k < Z-4, inside the top CPML): the per-point path with the
