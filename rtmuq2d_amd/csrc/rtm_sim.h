@@ -46,6 +46,21 @@ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // that each XCD (workgroups are dealt round-robin to the 8 XCDs) walks a
 // contiguous run of neighbouring tiles and their shared halo rows stay in its
 // L2.  `s` is the step within the current block; sim i is at n0[i] + s.
+template <typename R, int RZ, int CLS>
+__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, const SimDesc<R> &S,
+                                         const rtm::StepView<R> &V, int x0, int z0, int tx, int ty)
+{
+    rtm::TileRegs<R, RZ> G;
+    rtm::tile_fetch<R, RZ, CLS>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_stage<R, RZ, CLS>(L, G, tx, ty);
+    __syncthreads();
+    if (CLS) {
+        rtm::tile_psi<R, RZ, CLS>(L, G, S, x0, z0, tx, ty);
+        __syncthreads();
+    }
+    rtm::tile_points<R, RZ, CLS>(L, G, S, V, x0, z0, tx, ty);
+}
+
 template <typename R, int RZ>
 __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                const int *__restrict__ n0,
@@ -59,20 +74,19 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const int t = (b & 7) * per + (b >> 3);
     if ((b >> 3) >= per || t >= ntiles) return;
     const int4 tile = tiles[t];
+    // read through the restrict pointer: uniform fields become scalar loads
+    // that no store of the step can invalidate
     const SimDesc<R> &S = sims[tile.x];
     const rtm::StepView<R> V = rtm::step_view(S, n0[tile.x] + s);
     if (!V.active) return;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    rtm::TileRegs<R, RZ> G;
-    rtm::tile_fetch(G, S, V, tile.y, tile.z, tile.w, tx, ty);
-    rtm::tile_stage(L, G, tx, ty);
-    __syncthreads();
-    if (tile.w) {
-        rtm::tile_psi(L, G, S, tile.y, tile.z, tile.w, tx, ty);
-        __syncthreads();
+    switch (tile.w) {
+    case 0: run_tile<R, RZ, 0>(L, S, V, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1>(L, S, V, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2>(L, S, V, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3>(L, S, V, tile.y, tile.z, tx, ty); break;
     }
-    rtm::tile_points(L, G, S, V, tile.y, tile.z, tile.w, tx, ty);
 }
 
 // n0[i] += block for every simulation (the last node of a captured block)

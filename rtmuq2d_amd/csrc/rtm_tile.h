@@ -31,6 +31,38 @@
 
 namespace rtm {
 
+/* Global-memory access.  Device pointers reach the kernel through SimDesc
+ * (not as kernel arguments), so the compiler cannot infer their address
+ * space and would emit flat loads, whose completion counter is shared with
+ * LDS; casting to the global address space keeps HBM loads and LDS traffic
+ * independent.  Offsets are 32-bit (one field is < 2^31 elements). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RTM_GAS __attribute__((address_space(1)))
+#else
+#define RTM_GAS
+#endif
+
+/* The byte offset is a 32-bit unsigned value (every index here is >= 0 and
+ * every array is < 4 GiB; snapshot slices are offset per step on the host
+ * side of the view), which lets the compiler use the scalar-base + 32-bit
+ * vector-offset addressing form instead of 64-bit address arithmetic per
+ * access. */
+template <typename T>
+RTM_HD T ldg(const T *p, int i)
+{
+    const unsigned off = (unsigned)i * (unsigned)sizeof(T);
+    return *(const RTM_GAS T *)((const RTM_GAS char *)p + off);
+}
+
+template <typename T>
+RTM_HD void stg(T *p, int i, T v)
+{
+    const unsigned off = (unsigned)i * (unsigned)sizeof(T);
+    *(RTM_GAS T *)((RTM_GAS char *)p + off) = v;
+}
+
+RTM_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 enum SimPhase : int { PH_IDLE = 0, PH_FORWARD = 1, PH_BACKWARD = 2 };
 enum SimFlags : int {
     F_KAHAN = 1,    /* compensated image accumulation          */
@@ -104,7 +136,7 @@ RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
     const size_t nint = (size_t)S.nx * S.nz;
     if (v.forward) {
         v.src = n < S.nw && S.src_j >= 0;
-        v.src_w = v.src ? R(S.wav[n]) : R(0);
+        v.src_w = v.src ? R(ldg(S.wav, n)) : R(0);
         v.rec_row_out = (S.flags & F_RECORD) ? S.seis_rec + (size_t)n * S.nrec : nullptr;
         v.snap = ((S.flags & F_SNAP) && n % S.TS == 0) ? S.snaps + (size_t)(n / S.TS) * nint : nullptr;
     } else {
@@ -139,153 +171,170 @@ RTM_HD int tile_class(const SimDesc<R> &S, int x0, int z0, int TZ)
     return cls;
 }
 
-/* Per-thread values fetched from HBM before the first barrier, so that the
- * whole step costs one memory round trip: the thread's share of the u tile,
- * and the C, U_old, csi, psi and profile values its later phases need. */
+
+/* Per-thread values fetched from HBM before the first barrier, so that a
+ * step costs one memory round trip: the thread's share of the u tile and the
+ * C, U_old, csi, psi, profile, imaging and trace values its later phases
+ * need.  Every load uses a clamped (always valid) address and its value is
+ * selected afterwards: no load is predicated, which keeps the scalar unit
+ * free of exec-mask bookkeeping. */
 template <typename R, int RZ>
 struct TileRegs {
-    R u0[RZ + 4], u1[RZ + 4];   /* LDS rows ty+4t, columns tx and 64+tx */
+    R u0[RZ + 4], u1[RZ + 4];   /* u rows of the tile+halo, 2 column blocks */
     R C[RZ], Uold[RZ];          /* own points: rows ty*RZ+rr, column tx */
     R csx[RZ], csz[RZ];
     R px0[RZ], px1[RZ];         /* psiX^n at the psi-phase points       */
     R pz[RZ + 2];               /* psiZ^n at the psi-phase points       */
-    R ax0, bx0, ax1, bx1;       /* x profiles at psi columns tx-4, tx+60 */
+    R ax0, bx0, ax1, bx1;       /* x profiles at psi columns tx-4, 64+(tx&7)-4 */
     R axo, bxo;                 /* x profiles at the own column         */
-    R sv[RZ], Iv[RZ], Isv[RZ];  /* imaging: snapshot and image accumulators */
+    R sv[RZ], Iv[RZ], Isv[RZ];  /* imaging: snapshot and accumulators   */
     R Icv[RZ], Iscv[RZ];        /* Kahan compensations                  */
-    int recr;                   /* receiver at the own column (rec_row), or -1 */
+    int recr;                   /* receiver at the own column, or -1    */
     R recv;                     /* its trace sample (backward)          */
 };
 
-/* Phase 1a: issue every global load of the step.  tx in [0,64), ty in [0,4). */
-template <typename R, int RZ>
+/* Phase 1a: every global load of the step.  tx in [0,64), ty in [0,4) (the
+ * wavefront index: uniform within a wavefront).  CLS is the tile class. */
+template <typename R, int RZ, int CLS>
 RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
-                       int cls, int tx, int ty)
+                       int tx, int ty)
 {
     constexpr int TZ = 4 * RZ;
-    const int hx = (cls & T_XSTRIP) ? 8 : 4, hz = (cls & T_ZSTRIP) ? 8 : 4;
-    const int i0 = x0 - 8 + tx, i1 = x0 + 56 + tx;          /* LDS columns tx and 64+tx */
-    const bool c0 = tx >= 8 - hx && i0 >= 0 && i0 < S.X;
-    const bool c1 = tx < 8 + hx && i1 < S.X;
+    constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4, HZ = (CLS & T_ZSTRIP) ? 8 : 4;
+    constexpr int NR = RZ + HZ / 2;                  /* (TZ + 2 HZ) / 4 rows per thread */
+    /* column blocks: 64 columns, then the 2*HX remaining ones (lanes repeat) */
+    const int ia = x0 - HX + tx, ib = x0 - HX + 64 + (tx % (2 * HX));
+    const bool oka = ia >= 0 && ia < S.X, okb = ib < S.X;
+    const int ca = clampi(ia, 0, S.X - 1), cb = clampi(ib, 0, S.X - 1);
 #pragma unroll
-    for (int t = 0; t < RZ + 4; t++) {
-        const int r = ty + 4 * t, k = z0 - 8 + r;
-        const bool rok = r >= 8 - hz && r < TZ + 8 + hz && k >= 0 && k < S.Z;
-        const ptrdiff_t base = (ptrdiff_t)k * S.P;
-        G.u0[t] = (rok && c0) ? V.u[base + i0] : R(0);
-        G.u1[t] = (rok && c1) ? V.u[base + i1] : R(0);
+    for (int t = 0; t < NR; t++) {
+        const int k = z0 - HZ + ty + 4 * t;
+        const bool okk = k >= 0 && k < S.Z;
+        const int base = clampi(k, 0, S.Z - 1) * S.P;
+        const R a = ldg(V.u, base + ca), b = ldg(V.u, base + cb);
+        G.u0[t] = (okk && oka) ? a : R(0);
+        G.u1[t] = (okk && okb) ? b : R(0);
     }
     const int i = x0 + tx;
-    const bool col_ok = i < S.X - kHalf;
-    const bool inX = col_ok && (i < S.TL - kHalf || i >= S.X - S.TR + kHalf);
-    G.axo = inX ? S.ax[i] : R(0);
-    G.bxo = inX ? S.bx[i] : R(0);
+    const int ic = clampi(i, 0, S.X - 1);
+    const bool inX = i < S.TL - kHalf || (i >= S.X - S.TR + kHalf && i < S.X - kHalf);
 #pragma unroll
     for (int rr = 0; rr < RZ; rr++) {
-        const int k = z0 + ty * RZ + rr;
-        const bool ok = col_ok && k < S.Z - kHalf;
-        const ptrdiff_t j = (ptrdiff_t)k * S.P + i;
-        G.C[rr] = ok ? S.C[j] : R(0);
-        G.Uold[rr] = ok ? V.U[j] : R(0);
-        G.csx[rr] = (ok && inX) ? S.csiX[j] : R(0);
-        const bool inZ = ok && (k < S.TU - kHalf || k >= S.Z - S.TD + kHalf);
-        G.csz[rr] = inZ ? S.csiZ[j] : R(0);
-        /* imaging operands (backward steps that image) */
-        const int ii = i - S.TL, kk = k - S.TU;
-        const bool img = ok && V.snap && !V.forward && ii >= 0 && ii < S.nx && kk >= 0 && kk < S.nz;
-        const ptrdiff_t q = (ptrdiff_t)kk * S.nx + ii;
-        G.sv[rr] = img ? V.snap[q] : R(0);
-        G.Iv[rr] = img ? S.I[q] : R(0);
-        G.Isv[rr] = img ? S.Is[q] : R(0);
-        const bool kah = img && (S.flags & F_KAHAN);
-        G.Icv[rr] = kah ? S.Ic[q] : R(0);
-        G.Iscv[rr] = kah ? S.Isc[q] : R(0);
+        const int k = clampi(z0 + ty * RZ + rr, 0, S.Z - 1);
+        const int j = k * S.P + ic;
+        G.C[rr] = ldg(S.C, j);
+        G.Uold[rr] = ldg(V.U, j);
+        if (CLS & T_XSTRIP) G.csx[rr] = ldg(S.csiX, j);
+        if (CLS & T_ZSTRIP) G.csz[rr] = ldg(S.csiZ, j);
     }
-    /* receiver on the own column, if the receiver row is one of the own rows */
-    const int rrow = S.rec_row - (z0 + ty * RZ);
-    const bool has_rec = col_ok && rrow >= 0 && rrow < RZ && S.rec_row < S.Z - kHalf &&
-                         (!V.forward || V.rec_row_out);
-    G.recr = has_rec ? S.recmap[i] : -1;
-    G.recv = (!V.forward && G.recr >= 0) ? R(V.seis_row[G.recr]) : R(0);
-    if (cls & T_XSTRIP) {
-        const int ia = x0 - 4 + tx, ib = x0 + 60 + tx;     /* psi columns c = tx, 64+tx */
-        const bool oka = x_strip(S, ia), okb = tx < 8 && x_strip(S, ib);
-        G.ax0 = oka ? S.ax[ia] : R(0);
-        G.bx0 = oka ? S.bx[ia] : R(0);
-        G.ax1 = okb ? S.ax[ib] : R(0);
-        G.bx1 = okb ? S.bx[ib] : R(0);
+    if (CLS & T_XSTRIP) {
+        G.axo = inX ? ldg(S.ax, ic) : R(0);
+        G.bxo = inX ? ldg(S.bx, ic) : R(0);
+        const int pa = x0 - 4 + tx, pb = x0 + 60 + (tx & 7);   /* psi columns c = tx, 64 + (tx & 7) */
+        const bool sa = x_strip(S, pa), sb = x_strip(S, pb);
+        const int qa = clampi(pa, 0, S.X - 1), qb = clampi(pb, 0, S.X - 1);
+        const R a0 = ldg(S.ax, qa), b0 = ldg(S.bx, qa), a1 = ldg(S.ax, qb), b1 = ldg(S.bx, qb);
+        G.ax0 = sa ? a0 : R(0);
+        G.bx0 = sa ? b0 : R(0);
+        G.ax1 = sb ? a1 : R(0);
+        G.bx1 = sb ? b1 : R(0);
 #pragma unroll
         for (int t = 0; t < RZ; t++) {
             const int k = z0 + ty + 4 * t;
             const bool kok = k < S.Z - kHalf;
-            G.px0[t] = (kok && oka) ? V.psiXi[(ptrdiff_t)k * S.P + ia] : R(0);
-            G.px1[t] = (kok && okb) ? V.psiXi[(ptrdiff_t)k * S.P + ib] : R(0);
+            const int base = clampi(k, 0, S.Z - 1) * S.P;
+            const R va = ldg(V.psiXi, base + qa), vb = ldg(V.psiXi, base + qb);
+            G.px0[t] = (kok && sa) ? va : R(0);
+            G.px1[t] = (kok && sb) ? vb : R(0);
         }
     }
-    if (cls & T_ZSTRIP) {
+    if (CLS & T_ZSTRIP) {
+        const bool iok = i < S.X - kHalf;
 #pragma unroll
         for (int t = 0; t < RZ + 2; t++) {
             const int k = z0 - 4 + ty + 4 * t;
-            G.pz[t] = (col_ok && z_strip(S, k)) ? V.psiZi[(ptrdiff_t)k * S.P + i] : R(0);
+            const R v = ldg(V.psiZi, clampi(k, 0, S.Z - 1) * S.P + ic);
+            G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
+        }
+    }
+    if (!V.forward && V.snap) {
+        const int ii = clampi(i - S.TL, 0, S.nx - 1);
+#pragma unroll
+        for (int rr = 0; rr < RZ; rr++) {
+            const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
+            const int q = kk * S.nx + ii;
+            G.sv[rr] = ldg(V.snap, q);
+            G.Iv[rr] = ldg(S.I, q);
+            G.Isv[rr] = ldg(S.Is, q);
+            if (S.flags & F_KAHAN) {
+                G.Icv[rr] = ldg(S.Ic, q);
+                G.Iscv[rr] = ldg(S.Isc, q);
+            }
+        }
+    }
+    G.recr = -1;
+    G.recv = R(0);
+    const int rrow = S.rec_row - (z0 + ty * RZ);
+    if (rrow >= 0 && rrow < RZ && (!V.forward || V.rec_row_out)) {   /* uniform */
+        G.recr = i < S.X - kHalf ? ldg(S.recmap, ic) : -1;
+        if (!V.forward) {
+            const R v = R(ldg(V.seis_row, G.recr > 0 ? G.recr : 0));
+            G.recv = G.recr >= 0 ? v : R(0);
         }
     }
 }
 
-/* Phase 1b: the fetched u into LDS (rows z0-8 .., columns x0-8 ..). */
-template <typename R, int RZ>
+/* Phase 1b: the fetched u into LDS (row z0-8+r, column x0-8+c at [r][c]). */
+template <typename R, int RZ, int CLS>
 RTM_HD void tile_stage(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, int tx, int ty)
 {
+    constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4, HZ = (CLS & T_ZSTRIP) ? 8 : 4;
+    constexpr int NR = RZ + HZ / 2;
+    const int cb = 64 + (tx % (2 * HX)) + 8 - HX;   /* lanes beyond 2*HX repeat a column: same value */
 #pragma unroll
-    for (int t = 0; t < RZ + 4; t++) {
-        L.u[ty + 4 * t][tx] = G.u0[t];
-        if (tx < 16) L.u[ty + 4 * t][64 + tx] = G.u1[t];
+    for (int t = 0; t < NR; t++) {
+        const int r = ty + 4 * t + 8 - HZ;
+        L.u[r][tx + 8 - HX] = G.u0[t];
+        L.u[r][cb] = G.u1[t];
     }
 }
 
-/* Phase 2 (strip tiles): psi^{n+1} over the tile plus a radius-4 halo. */
-template <typename R, int RZ>
+/* Phase 2 (strip classes): psi^{n+1} over the tile plus a radius-4 halo. */
+template <typename R, int RZ, int CLS>
 RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S, int x0, int z0,
-                     int cls, int tx, int ty)
+                     int tx, int ty)
 {
-    if (cls & T_XSTRIP) {
-        const bool oka = x_strip(S, x0 - 4 + tx), okb = tx < 8 && x_strip(S, x0 + 60 + tx);
+    if (CLS & T_XSTRIP) {
+        const int cb = 64 + (tx & 7);
+        const bool sa = x_strip(S, x0 - 4 + tx), sb = x_strip(S, x0 - 4 + cb);
 #pragma unroll
         for (int t = 0; t < RZ; t++) {
             const int r = ty + 4 * t;
             const bool kok = z0 + r < S.Z - kHalf;
-            {
-                const R *lu = &L.u[r + 8][tx + 4];
-                R v = R(0);
-                if (kok && oka) {
-                    const R d1 = d1_val(lu[1], lu[2], lu[3], lu[4], lu[-1], lu[-2], lu[-3], lu[-4]);
-                    v = psi_val(G.ax0, G.bx0, G.px0[t], d1, S.h1);
-                }
-                L.px[r][tx] = v;
-            }
-            if (tx < 8) {
-                const R *lu = &L.u[r + 8][tx + 68];
-                R v = R(0);
-                if (kok && okb) {
-                    const R d1 = d1_val(lu[1], lu[2], lu[3], lu[4], lu[-1], lu[-2], lu[-3], lu[-4]);
-                    v = psi_val(G.ax1, G.bx1, G.px1[t], d1, S.h1);
-                }
-                L.px[r][64 + tx] = v;
-            }
+            const R *la = &L.u[r + 8][tx + 4];
+            const R *lb = &L.u[r + 8][cb + 4];
+            const R da = d1_val(la[1], la[2], la[3], la[4], la[-1], la[-2], la[-3], la[-4]);
+            const R db = d1_val(lb[1], lb[2], lb[3], lb[4], lb[-1], lb[-2], lb[-3], lb[-4]);
+            const R va = psi_val(G.ax0, G.bx0, G.px0[t], da, S.h1);
+            const R vb = psi_val(G.ax1, G.bx1, G.px1[t], db, S.h1);
+            L.px[r][tx] = (kok && sa) ? va : R(0);
+            L.px[r][cb] = (kok && sb) ? vb : R(0);
         }
     }
-    if (cls & T_ZSTRIP) {
-        const int i = x0 + tx;
+    if (CLS & T_ZSTRIP) {
+        const bool iok = x0 + tx < S.X - kHalf;
         const int lc = tx + 8;
 #pragma unroll
         for (int t = 0; t < RZ + 2; t++) {
             const int r = ty + 4 * t;
             const int k = z0 - 4 + r;
             R v = R(0);
-            if (i < S.X - kHalf && z_strip(S, k)) {
+            if (z_strip(S, k)) {   /* uniform: k is the same for the whole wavefront */
                 const int lr = r + 4;
                 const R d1 = d1_val(L.u[lr + 1][lc], L.u[lr + 2][lc], L.u[lr + 3][lc], L.u[lr + 4][lc],
                                     L.u[lr - 1][lc], L.u[lr - 2][lc], L.u[lr - 3][lc], L.u[lr - 4][lc]);
-                v = psi_val(S.az[k], S.bz[k], G.pz[t], d1, S.h1);
+                const R pv = psi_val(ldg(S.az, k), ldg(S.bz, k), G.pz[t], d1, S.h1);
+                v = iok ? pv : R(0);
             }
             L.pz[r][tx] = v;
         }
@@ -293,24 +342,27 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-template <typename R, int RZ>
+template <typename R, int RZ, int CLS>
 RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
-                        const StepView<R> &V, int x0, int z0, int cls, int tx, int ty)
+                        const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
+    constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4;
+    (void)HX;
     const int i = x0 + tx;
-    if (i >= S.X - kHalf) return;
+    const bool iok = i < S.X - kHalf;
     const int lc = tx + 8;
-    /* this column's u from row ty*RZ-4 to ty*RZ+RZ+3 (tile-relative) */
+    /* this column's u from tile row ty*RZ-4 to ty*RZ+RZ+3 */
     R col[RZ + 8];
 #pragma unroll
     for (int t = 0; t < RZ + 8; t++) col[t] = L.u[ty * RZ + 4 + t][lc];
     const bool inL = i < S.TL - kHalf, inR = i >= S.X - S.TR + kHalf;
+    const bool img_col = i - S.TL >= 0 && i - S.TL < S.nx;
 #pragma unroll
     for (int rr = 0; rr < RZ; rr++) {
-        const int kl = ty * RZ + rr;         /* tile-relative row */
+        const int kl = ty * RZ + rr;         /* tile-relative row (uniform) */
         const int k = z0 + kl;
         if (k >= S.Z - kHalf) break;
-        const ptrdiff_t j = (ptrdiff_t)k * S.P + i;
+        const int j = k * S.P + i;
         const R *lu = &L.u[kl + 8][lc];
         const R uc = col[rr + 4];
         const R xm1 = lu[-1], xp1 = lu[1], xm2 = lu[-2], xp2 = lu[2];
@@ -320,66 +372,71 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const
         const R Cj = G.C[rr];
         R U = stencil_val(Cj, G.Uold[rr], uc, xm1, xp1, zm1, zp1, xm2, xp2, zm2, zp2, xm3, xp3, zm3, zp3,
                           xm4, xp4, zm4, zp4);
-        if (cls) {
-            const bool inT = k < S.TU - kHalf, inB = k >= S.Z - S.TD + kHalf;
-            R corrX = R(0), corrZ = R(0);
-            if (inL || inR) {
-                const R *p = &L.px[kl][tx + 4];
-                V.psiXo[j] = p[0];
-                const R dpsi = d1_val(p[1], p[2], p[3], p[4], p[-1], p[-2], p[-3], p[-4]) * S.h1;
-                const R d2 = d2_val(uc, xp1, xp2, xp3, xp4, xm1, xm2, xm3, xm4);
-                const R cs = csi_val(G.axo, G.bxo, G.csx[rr], d2, dpsi, S.h3);
-                S.csiX[j] = cs;
-                corrX = corr_val(Cj, S.h2, dpsi, cs);
-            }
-            if (inT || inB) {
-                const int pr = kl + 4;
-                V.psiZo[j] = L.pz[pr][tx];
-                const R dpsi = d1_val(L.pz[pr + 1][tx], L.pz[pr + 2][tx], L.pz[pr + 3][tx], L.pz[pr + 4][tx],
-                                      L.pz[pr - 1][tx], L.pz[pr - 2][tx], L.pz[pr - 3][tx], L.pz[pr - 4][tx])
-                               * S.h1;
-                const R d2 = d2_val(uc, zp1, zp2, zp3, zp4, zm1, zm2, zm3, zm4);
-                const R cs = csi_val(S.az[k], S.bz[k], G.csz[rr], d2, dpsi, S.h3);
-                S.csiZ[j] = cs;
-                corrZ = corr_val(Cj, S.h2, dpsi, cs);
-            }
-            /* additions in the reference's strip order: L, T, R, B */
-            if (inL) U = U + corrX;
-            if (inT) U = U + corrZ;
-            if (inR) U = U + corrX;
-            if (inB) U = U + corrZ;
+        R corrX = R(0), corrZ = R(0), csx = R(0), csz = R(0), pxo = R(0), pzo = R(0);
+        const bool inT = k < S.TU - kHalf, inB = k >= S.Z - S.TD + kHalf;   /* uniform */
+        if (CLS & T_XSTRIP) {
+            const R *p = &L.px[kl][tx + 4];
+            pxo = p[0];
+            const R dpsi = d1_val(p[1], p[2], p[3], p[4], p[-1], p[-2], p[-3], p[-4]) * S.h1;
+            const R d2 = d2_val(uc, xp1, xp2, xp3, xp4, xm1, xm2, xm3, xm4);
+            csx = csi_val(G.axo, G.bxo, G.csx[rr], d2, dpsi, S.h3);
+            corrX = corr_val(Cj, S.h2, dpsi, csx);
         }
-        const bool rec_here = k == S.rec_row && G.recr >= 0;
+        if ((CLS & T_ZSTRIP) && (inT || inB)) {
+            const int pr = kl + 4;
+            pzo = L.pz[pr][tx];
+            const R dpsi = d1_val(L.pz[pr + 1][tx], L.pz[pr + 2][tx], L.pz[pr + 3][tx], L.pz[pr + 4][tx],
+                                  L.pz[pr - 1][tx], L.pz[pr - 2][tx], L.pz[pr - 3][tx], L.pz[pr - 4][tx])
+                           * S.h1;
+            const R d2 = d2_val(uc, zp1, zp2, zp3, zp4, zm1, zm2, zm3, zm4);
+            csz = csi_val(ldg(S.az, k), ldg(S.bz, k), G.csz[rr], d2, dpsi, S.h3);
+            corrZ = corr_val(Cj, S.h2, dpsi, csz);
+        }
+        /* additions in the reference's strip order: L, T, R, B */
+        if (CLS & T_XSTRIP) U = inL ? U + corrX : U;
+        if ((CLS & T_ZSTRIP) && inT) U = U + corrZ;
+        if (CLS & T_XSTRIP) U = inR ? U + corrX : U;
+        if ((CLS & T_ZSTRIP) && inB) U = U + corrZ;
+        const bool rec_row = k == S.rec_row && G.recr >= 0;
         if (V.forward) {
             if (V.src && j == S.src_j) U = U + V.src_w * Cj;
-        } else if (rec_here) {
-            U = G.recv;
+        } else {
+            U = rec_row ? G.recv : U;
         }
-        V.U[j] = U;
-        if (V.forward && rec_here) V.rec_row_out[G.recr] = float(U);
-        if (V.snap) {
-            const int ii = i - S.TL, kk = k - S.TU;
-            if (ii >= 0 && ii < S.nx && kk >= 0 && kk < S.nz) {
-                const ptrdiff_t q = (ptrdiff_t)kk * S.nx + ii;
+        if (iok) {
+            stg(V.U, j, U);
+            if ((CLS & T_XSTRIP) && (inL || inR)) {
+                stg(V.psiXo, j, pxo);
+                stg(S.csiX, j, csx);
+            }
+            if ((CLS & T_ZSTRIP) && (inT || inB)) {
+                stg(V.psiZo, j, pzo);
+                stg(S.csiZ, j, csz);
+            }
+            if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
+            const int kk = k - S.TU;
+            if (V.snap && img_col && kk >= 0 && kk < S.nz) {
+                const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
-                    V.snap[q] = U;
+                    stg(V.snap, q, U);
+                } else if (S.flags & F_KAHAN) {
+                    R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
+                    const R s = G.sv[rr];
+                    kahan_add(I, Ic, s * U);
+                    kahan_add(Is, Isc, s * s);
+                    stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
                 } else {
                     const R s = G.sv[rr];
-                    if (S.flags & F_KAHAN) {
-                        R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
-                        kahan_add(I, Ic, s * U);
-                        kahan_add(Is, Isc, s * s);
-                        S.I[q] = I; S.Ic[q] = Ic; S.Is[q] = Is; S.Isc[q] = Isc;
-                    } else {
-                        S.I[q] = G.Iv[rr] + s * U;
-                        S.Is[q] = G.Isv[rr] + s * s;
-                    }
+                    stg(S.I, q, G.Iv[rr] + s * U);
+                    stg(S.Is, q, G.Isv[rr] + s * s);
                 }
             }
         }
     }
 }
 
+/* The whole tile for a class known at compile time (the kernel and the CPU
+ * harness dispatch on the tile's class). */
 }  // namespace rtm
 
 #endif /* RTM_TILE_H */

@@ -32,6 +32,22 @@ static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const 
 
 // Emulation of k_tiles for one simulation at step n: every tile, phase by
 // phase, all 256 threads of a phase before the next (the barriers).
+template <typename R, int RZ, int CLS>
+static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm::SimDesc<R> &S,
+                     const rtm::StepView<R> &V, int x0, int z0)
+{
+    for (int ty = 0; ty < 4; ty++)
+        for (int tx = 0; tx < 64; tx++) {
+            rtm::tile_fetch<R, RZ, CLS>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            rtm::tile_stage<R, RZ, CLS>(L, G[ty * 64 + tx], tx, ty);
+        }
+    if (CLS)
+        for (int ty = 0; ty < 4; ty++)
+            for (int tx = 0; tx < 64; tx++) rtm::tile_psi<R, RZ, CLS>(L, G[ty * 64 + tx], S, x0, z0, tx, ty);
+    for (int ty = 0; ty < 4; ty++)
+        for (int tx = 0; tx < 64; tx++) rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+}
+
 template <typename R, int RZ>
 static void run_tiles(const rtm::SimDesc<R> &S, int n)
 {
@@ -42,18 +58,13 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n)
     const int TZ = 4 * RZ;
     for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64)
         for (int z0 = rtm::kHalf; z0 < S.Z - rtm::kHalf; z0 += TZ) {
-            const int cls = rtm::tile_class(S, x0, z0, TZ);
             // the LDS holds stale data from the previous tile, as on the GPU
-            for (int ty = 0; ty < 4; ty++)
-                for (int tx = 0; tx < 64; tx++) {
-                    rtm::tile_fetch(G[ty * 64 + tx], S, V, x0, z0, cls, tx, ty);
-                    rtm::tile_stage(*L, G[ty * 64 + tx], tx, ty);
-                }
-            if (cls)
-                for (int ty = 0; ty < 4; ty++)
-                    for (int tx = 0; tx < 64; tx++) rtm::tile_psi(*L, G[ty * 64 + tx], S, x0, z0, cls, tx, ty);
-            for (int ty = 0; ty < 4; ty++)
-                for (int tx = 0; tx < 64; tx++) rtm::tile_points(*L, G[ty * 64 + tx], S, V, x0, z0, cls, tx, ty);
+            switch (rtm::tile_class(S, x0, z0, TZ)) {
+            case 0: run_tile<R, RZ, 0>(*L, G, S, V, x0, z0); break;
+            case 1: run_tile<R, RZ, 1>(*L, G, S, V, x0, z0); break;
+            case 2: run_tile<R, RZ, 2>(*L, G, S, V, x0, z0); break;
+            default: run_tile<R, RZ, 3>(*L, G, S, V, x0, z0); break;
+            }
         }
     delete[] G;
     delete L;
