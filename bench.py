@@ -63,11 +63,11 @@ def c1_inputs(p, seed=2024):
 
 def cpu_baseline(p, v_mig, w, seis):
     """The reference's own rtm_routine (oracle/_ref, -O3 -march=native
-    OpenMP) on a bounded sample of the workload: the same 1801x451 padded
-    grid over the first 1000 time steps, on this host's cores."""
+    OpenMP, its disk snapshots every 20 steps) on one full c1 image of the
+    same inputs, on this host's cores (OMP_NUM_THREADS)."""
     ref = os.path.join(REPO, "oracle", "_ref", "c1s_native", "ref_main")
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    nt_s = 1000
+    nt_s = p.number_of_time_steps
     X, Z = p.X, p.Z
     import rtmuq2d_amd as R
     if not os.path.exists(ref):
@@ -95,8 +95,8 @@ def cpu_baseline(p, v_mig, w, seis):
         "kind": "reference",
         "gupdates_per_s": round(upd / secs / 1e9, 4),
         "sample": (f"reference rtm_routine (gcc -O3 -march=native, OpenMP, disk snapshots every "
-                   f"20 steps) on the {X}x{Z} padded c1 grid for {nt_s} of {p.number_of_time_steps} "
-                   f"steps ({secs:.2f} s), scaled to a full image"),
+                   f"20 steps) on the {X}x{Z} padded c1 grid, {nt_s} of {p.number_of_time_steps} "
+                   f"steps ({secs:.2f} s wall)"),
     }
 
 
@@ -108,6 +108,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--batch", type=int, default=64, help="realizations in the batched run (0: skip)")
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed workload for counter profiles (see run_profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the RTM path shards by "
@@ -118,6 +120,9 @@ def main() -> None:
     if R.lib().rtm_device_count() < 1:
         log("no HIP device")
         sys.exit(1)
+    if args.profile:
+        run_profile(R)
+        return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
     log(f"[bench] c1 plain RTM {p.nx}x{p.nz} padded {X}x{Z} nt={nt} TS=1 FP32; "
@@ -143,21 +148,22 @@ def main() -> None:
         plan.zero_image()
         plan.migrate(p.first_location_shot, p.depth_shot)
 
-    # tile height x graph block sweep (one image each); the fastest is timed
+    # kernel-shape sweep (one image each); the fastest is timed
     best = None
     tuning = []
-    for rz in (1, 2, 4, 8):
-        for block in (1, 100):
-            plan.set_tiling(rz, block)
-            one_image()
-            f_ms, b_ms, _ = plan.last_timing()
-            tuning.append({"rows_per_thread": rz, "graph_block": block,
-                           "fwd_us": round(f_ms * 1e3, 3), "bwd_us": round(b_ms * 1e3, 3)})
-            log(f"[bench] tiling rz={rz} block={block}: fwd {f_ms * 1e3:.2f} us/step, bwd {b_ms * 1e3:.2f} us/step")
-            if best is None or f_ms + b_ms < best[0]:
-                best = (f_ms + b_ms, rz, block)
-    plan.set_tiling(best[1], best[2])
-    log(f"[bench] using rz={best[1]} graph_block={best[2]}")
+    for rz, block, march in [(1, 100, 0), (2, 100, 0), (4, 100, 0), (2, 1, 0), (1, 100, 32), (2, 100, 32),
+                             (2, 100, 64), (4, 100, 64), (4, 100, 128)]:
+        plan.set_tiling(rz, block, march)
+        one_image()
+        f_ms, b_ms, _ = plan.last_timing()
+        tuning.append({"rows_per_thread": rz, "graph_block": block, "march_rows": march,
+                       "fwd_us": round(f_ms * 1e3, 3), "bwd_us": round(b_ms * 1e3, 3)})
+        log(f"[bench] tiling rz={rz} block={block} march={march}: fwd {f_ms * 1e3:.2f} us/step, "
+            f"bwd {b_ms * 1e3:.2f} us/step")
+        if best is None or f_ms + b_ms < best[0]:
+            best = (f_ms + b_ms, rz, block, march)
+    plan.set_tiling(best[1], best[2], best[3])
+    log(f"[bench] using rz={best[1]} graph_block={best[2]} march_rows={best[3]}")
 
     for i in range(args.warmup):
         one_image()
@@ -197,7 +203,7 @@ def main() -> None:
             "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
                         "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
             "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
-            "tiling": {"rows_per_thread": best[1], "graph_block": best[2]},
+            "tiling": {"rows_per_thread": best[1], "graph_block": best[2], "march_rows": best[3]},
         },
         "tiling_sweep": tuning,
         "gupdates_per_s": round(updates / secs / 1e9, 3),
@@ -219,6 +225,47 @@ def main() -> None:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
     print(json.dumps(out), flush=True)
+
+
+def run_profile(R, nt=400):
+    """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
+    pass per counter group over the same dispatches):
+      1. a 1 GiB device-to-device copy (torch), the calibration of the
+         FETCH_SIZE / WRITE_SIZE counters (1 GiB read, 1 GiB written);
+      2. the c1 plain workload for `nt` steps, tiles of 64x8, graph blocks;
+      3. 16 realizations of c1 for `nt` steps (TS=20), tiles of 64x16 only,
+         then strip tiles + marched interior."""
+    import torch
+    x = torch.rand(1 << 28, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    p = c1_params().update(number_of_time_steps=nt, sampling_wavelet=nt)
+    v_true, v_mig, w = c1_inputs(p)
+    seis = np.zeros((p.number_of_receivers, nt), np.float32)
+    with R.Plan(p) as plan:
+        plan.set_tiling(2, 100, 0)
+        plan.set_velocity(R.extend(p, v_mig))
+        plan.set_wavelet(w)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(p.first_location_shot, p.depth_shot)
+        log("[profile] plain", plan.last_timing())
+    pb = p.copy().update(time_slices=20)
+    with R.Batch(pb, 16) as b:
+        vpad = R.extend(pb, v_mig)
+        for k in range(16):
+            b.set_velocity(k, vpad)
+        b.set_wavelet(w)
+        b.set_seismogram(seis)
+        for rz, march in ((4, 0), (4, 1000)):
+            b.set_tiling(rz, 100, march)
+            b.zero_images()
+            b.migrate(pb.first_location_shot, pb.depth_shot)
+            log(f"[profile] batch16 rz={rz} march={march}", b.last_timing())
+    print(json.dumps({"profile": "done", "nt": nt}), flush=True)
 
 
 def rel_l2(a, b):
@@ -249,14 +296,14 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     plan.set_wavelet(w)
     plan.set_seismogram(seis)
     best = None
-    for rz in (1, 2, 4):
-        plan.set_tiling(rz, 100)
+    for rz, march in ((1, 0), (2, 0), (4, 0), (2, 32), (2, 64)):
+        plan.set_tiling(rz, 100, march)
         plan.zero_image()
         plan.migrate(sx, sz)
         f_ms, b_ms, _ = plan.last_timing()
         if best is None or f_ms + b_ms < best[0]:
-            best = (f_ms + b_ms, rz)
-    plan.set_tiling(best[1], 100)
+            best = (f_ms + b_ms, rz, march)
+    plan.set_tiling(best[1], 100, best[2])
     reps = 3
     plan.synchronize()
     t0 = time.perf_counter()
@@ -267,7 +314,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     plain_secs = (time.perf_counter() - t0) / reps
     I_plain, _ = plan.image()
     plan.close()
-    log(f"[bench] plain (1 sim, TS={ts}) rz={best[1]}: {plain_secs * 1e3:.1f} ms/image")
+    log(f"[bench] plain (1 sim, TS={ts}) rz={best[1]} march={best[2]}: {plain_secs * 1e3:.1f} ms/image")
 
     # FP64 oracle run of realization 0
     with R.Plan(p, R.REAL_F64) as p64:
@@ -285,19 +332,19 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     b.set_seismogram(seis)
     sweep = []
     bbest = None
-    for rz in (2, 4, 8):
-        b.set_tiling(rz, 100)
+    for rz, march in ((4, 0), (2, 1000), (4, 1000), (8, 1000)):
+        b.set_tiling(rz, 100, march)
         b.zero_images()
         t0 = time.perf_counter()
         b.migrate(sx, sz)
         dt = time.perf_counter() - t0
         f_ms, b_ms, _ = b.last_timing()
-        sweep.append({"rows_per_thread": rz, "seconds": round(dt, 4), "fwd_us": round(f_ms * 1e3, 2),
-                      "bwd_us": round(b_ms * 1e3, 2)})
-        log(f"[bench] batch rz={rz}: {dt:.3f} s ({f_ms * 1e3:.1f} / {b_ms * 1e3:.1f} us per step)")
+        sweep.append({"rows_per_thread": rz, "march_rows": march, "seconds": round(dt, 4),
+                      "fwd_us": round(f_ms * 1e3, 2), "bwd_us": round(b_ms * 1e3, 2)})
+        log(f"[bench] batch rz={rz} march={march}: {dt:.3f} s ({f_ms * 1e3:.1f} / {b_ms * 1e3:.1f} us per step)")
         if bbest is None or dt < bbest[0]:
-            bbest = (dt, rz)
-    b.set_tiling(bbest[1], 100)
+            bbest = (dt, rz, march)
+    b.set_tiling(bbest[1], 100, bbest[2])
     b.zero_images()
     t0 = time.perf_counter()
     b.migrate(sx, sz)
@@ -322,7 +369,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "plain_images_per_hour_same_ts": round(plain_iph, 1),
         "batched_over_plain": round(batch_iph / plain_iph, 3),
         "step_kernel_us": {"forward": round(f_ms * 1e3, 2), "backward": round(b_ms * 1e3, 2)},
-        "tiling": {"rows_per_thread": bbest[1], "graph_block": 100},
+        "tiling": {"rows_per_thread": bbest[1], "graph_block": 100, "march_rows": bbest[2]},
         "tiling_sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4)},

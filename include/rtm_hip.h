@@ -142,10 +142,13 @@ int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out);
 int rtm_plan_destroy(rtm_plan *plan);
 size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan allocates */
 
-/* Tile height of the step kernel (rows_per_thread in {1,2,4,8}: tiles of
- * 64 x 4*rows_per_thread points) and the number of time steps per captured
- * hipGraph block (<= 1: one eager launch per step).  Defaults: 4, 100. */
-int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block);
+/* Kernel shape: rows_per_thread in {1,2,4,8} gives tiles of 64 x
+ * 4*rows_per_thread points; graph_block = time steps per captured hipGraph
+ * block (<= 1: one eager launch per step); march_rows > 0 streams the
+ * interior (points outside every CPML strip) with the marching kernel in
+ * segments of about march_rows rows, leaving tiles only where a strip is
+ * touched (0: tiles everywhere).  Defaults: 4, 100, 0. */
+int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int march_rows);
 
 int rtm_plan_set_velocity(rtm_plan *plan, const float *vpad);     /* padded [X][Z] */
 int rtm_plan_set_wavelet(rtm_plan *plan, const float *w, int n);
@@ -183,7 +186,7 @@ int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_batch **out);
 int rtm_batch_destroy(rtm_batch *batch);
 size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real);
 int rtm_batch_size(rtm_batch *batch);
-int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block);
+int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block, int march_rows);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
 int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
 int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */

@@ -79,11 +79,11 @@ REF_CONFIGS = {
                sampligWavelet=2000, firstLocationShot=300, depthShot=2, spacingShot=1,
                numberOfShots=1, firstLocationReceiver=0, depthReceiver=2, spacingReceiver=2,
                numberOfReceivers=600, numberOfSimulations=2, zfp_tolerance=1e-4),
-    # config c1 grid (1701x351, CPML 50, dt=0.8 ms) over a bounded 1000-step
-    # record: the CPU-baseline sample
+    # config c1 (1701x351, CPML 50, dt=0.8 ms, nt=4000): one full image,
+    # the CPU-baseline sample
     "c1s": dict(Nx=1701, Nz=351, thicknessLeft=50, thicknessRight=50, thicknessUp=50,
-                thicknessDown=50, spacingGrid=10.0, sampling=0.0008, numberOfTimeStep=1000,
-                sampligWavelet=1000, firstLocationShot=850, depthShot=2, spacingShot=1,
+                thicknessDown=50, spacingGrid=10.0, sampling=0.0008, numberOfTimeStep=4000,
+                sampligWavelet=4000, firstLocationShot=850, depthShot=2, spacingShot=1,
                 numberOfShots=1, firstLocationReceiver=0, depthReceiver=2, spacingReceiver=2,
                 numberOfReceivers=1701, numberOfSimulations=1, zfp_tolerance=1e-4),
 }

@@ -52,6 +52,7 @@ struct rtm_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int rz = 4;                     // tile rows per thread
     int graph_block = 100;          // steps per captured graph block (<= 1: eager)
+    int march_rows = 0;             // interior marched in segments of this many rows (0: tiles only)
     int last_nt = 0;                // steps of the last pass
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
@@ -72,15 +73,16 @@ static void plan_free(rtm_plan *q)
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
-static int plan_set_tiling(rtm_plan *q, int rz, int graph_block)
+static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int march_rows)
 {
-    std::vector<int4> tiles;
-    append_tiles(tiles, 0, q->b.p, rz);
+    std::vector<int4> tiles, march;
+    append_work(tiles, march, 0, q->b.p, rz, march_rows);
     q->ex.release();
-    if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz)) return -1;
-    if (q->ex.set_tiles(tiles, q->stream)) return -1;
+    if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)march.size())) return -1;
+    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_march(march, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = graph_block;
+    q->march_rows = march_rows;
     return 0;
 }
 
@@ -108,7 +110,7 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
     if (q->b.alloc(*p, real)) return fail();
     const char *env = getenv("RTM_TILE_RZ");
     const int rz = env && *env ? atoi(env) : 4;
-    if (plan_set_tiling(q, rz, 100)) return fail();
+    if (plan_set_tiling(q, rz, 100, 0)) return fail();
     *out = q;
     return 0;
 }
@@ -123,12 +125,12 @@ extern "C" int rtm_plan_destroy(rtm_plan *q)
     return 0;
 }
 
-extern "C" int rtm_plan_set_tiling(rtm_plan *q, int rows_per_thread, int graph_block)
+extern "C" int rtm_plan_set_tiling(rtm_plan *q, int rows_per_thread, int graph_block, int march_rows)
 {
     if (!q) { rtm_set_error("rtm_plan_set_tiling: null plan"); return -1; }
     HIP_TRY(hipSetDevice(q->b.p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
-    return plan_set_tiling(q, rows_per_thread, graph_block);
+    return plan_set_tiling(q, rows_per_thread, graph_block, march_rows);
 }
 
 extern "C" int rtm_plan_set_velocity(rtm_plan *q, const float *vpad)

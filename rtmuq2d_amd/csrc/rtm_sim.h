@@ -16,6 +16,7 @@
 #include "rtm_internal.h"
 #include "rtm_step.h"
 #include "rtm_tile.h"
+#include "rtm_march.h"
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
@@ -86,6 +87,49 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     case 1: run_tile<R, RZ, 1>(L, S, V, tile.y, tile.z, tx, ty); break;
     case 2: run_tile<R, RZ, 2>(L, S, V, tile.y, tile.z, tx, ty); break;
     default: run_tile<R, RZ, 3>(L, S, V, tile.y, tile.z, tx, ty); break;
+    }
+}
+
+// The streaming interior kernel: one workgroup marches down a 64-column
+// interior strip {sim, x0, zA, zB} (rtm_march.h), prefetching the next
+// chunk while computing the current one.
+template <typename R, int RZ>
+__global__ void __launch_bounds__(256) k_march(const SimDesc<R> *__restrict__ sims,
+                                               const int *__restrict__ n0,
+                                               const int4 *__restrict__ items,
+                                               const int *__restrict__ nitems_p, int s)
+{
+    constexpr int TZ = 4 * RZ;
+    __shared__ rtm::MarchLds<R, RZ> L;
+    const int nitems = *nitems_p;
+    const int per = (nitems + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= nitems) return;
+    const int4 it = items[t];
+    const SimDesc<R> &S = sims[it.x];
+    const rtm::StepView<R> V = rtm::step_view(S, n0[it.x] + s);
+    if (!V.active) return;
+    const int tx = threadIdx.x & 63;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x0 = it.y, zA = it.z, zB = it.w;
+    rtm::MarchRegs<R, RZ> cur, nxt;
+    rtm::march_fetch_rows<R, RZ, RZ + 2>(cur, S, V, x0, zA - 4, tx, ty);
+    rtm::march_fetch_points(cur, S, V, x0, zA, tx, ty);
+    rtm::march_stage_rows<R, RZ, RZ + 2>(L, cur, zA - 4, tx, ty);
+    __syncthreads();
+    for (int zc = zA; zc < zB; zc += TZ) {
+        const bool more = zc + TZ < zB;
+        if (more) {
+            rtm::march_fetch_rows<R, RZ, RZ>(nxt, S, V, x0, zc + TZ + 4, tx, ty);
+            rtm::march_fetch_points(nxt, S, V, x0, zc + TZ, tx, ty);
+        }
+        rtm::march_points(L, cur, S, V, x0, zc, tx, ty);
+        if (more) {
+            rtm::march_stage_rows<R, RZ, RZ>(L, nxt, zc + TZ + 4, tx, ty);
+            rtm::march_copy(cur, nxt);
+        }
+        __syncthreads();
     }
 }
 
@@ -267,8 +311,17 @@ struct SimBuffers {
 // plan memory in bytes for params p (the admission footprint)
 size_t sim_bytes(const rtm_params &p, size_t esz);
 
-// tiles {sim, x0, z0, class} covering the stepped region of a grid
-void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz);
+// Work lists covering the stepped region of a grid.  With march_rows == 0:
+// tiles {sim, x0, z0, class} everywhere.  With march_rows > 0: tiles only
+// where a tile touches a CPML strip, and the remaining interior as march
+// items {sim, x0, zA, zB} of at most march_rows rows (whole tiles).
+void append_work(std::vector<int4> &tiles, std::vector<int4> &march, int sim, const rtm_params &p,
+                 int rz, int march_rows);
+inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
+{
+    std::vector<int4> none;
+    append_work(out, none, sim, p, rz, 0);
+}
 
 // ---------------------------------------------------------------------------
 // executor: one launch per time step over a tile list, hipGraph blocks
@@ -277,20 +330,23 @@ void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz);
 struct Exec {
     int real = RTM_REAL_F32;
     int rz = 4;                         // rows per thread (tile height 4*rz)
-    int cap_sims = 0, cap_tiles = 0;
-    int nsims = 0, ntiles = 0;
+    int cap_sims = 0, cap_tiles = 0, cap_march = 0;
+    int nsims = 0, ntiles = 0, nmarch = 0;
     void *d_desc = nullptr;             // SimDesc<R>[cap_sims]
     int *d_n0 = nullptr;                // [cap_sims]
     int4 *d_tiles = nullptr;            // [cap_tiles]
     int *d_ntiles = nullptr;
+    int4 *d_march = nullptr;            // [cap_march]
+    int *d_nmarch = nullptr;
     std::vector<int> h_n0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int gblock = 0;
 
-    int init(int real_type, int max_sims, int max_tiles, int rows_per_thread);
+    int init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_march = 0);
     void release();
     int set_tiles(const std::vector<int4> &tiles, hipStream_t s);
+    int set_march(const std::vector<int4> &items, hipStream_t s);
     int upload_descs(const void *descs, int n, hipStream_t s);   // SimDesc<R>[n]
     int set_n0(const std::vector<int> &n0, hipStream_t s);
     // `steps` launches from the current n0 (eager, or in graph blocks of

@@ -289,7 +289,8 @@ SimDesc<R> SimBuffers::desc() const
 template SimDesc<float> SimBuffers::desc<float>() const;
 template SimDesc<double> SimBuffers::desc<double>() const;
 
-void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
+void append_work(std::vector<int4> &tiles, std::vector<int4> &march, int sim, const rtm_params &p, int rz,
+                 int march_rows)
 {
     const int X = p.thickness_left + p.nx + p.thickness_right;
     const int Z = p.thickness_up + p.nz + p.thickness_down;
@@ -297,18 +298,34 @@ void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
     SimDesc<float> d{};
     d.X = X; d.Z = Z;
     d.TL = p.thickness_left; d.TR = p.thickness_right; d.TU = p.thickness_up; d.TD = p.thickness_down;
+    const int seg = march_rows > 0 ? (march_rows + TZ - 1) / TZ * TZ : 0;
     // z fastest: vertically adjacent tiles (which share halo rows) are
     // neighbours in the list and so land on the same XCD
-    for (int x0 = rtm::kHalf; x0 < X - rtm::kHalf; x0 += 64)
-        for (int z0 = rtm::kHalf; z0 < Z - rtm::kHalf; z0 += TZ)
-            out.push_back(make_int4(sim, x0, z0, rtm::tile_class(d, x0, z0, TZ)));
+    for (int x0 = rtm::kHalf; x0 < X - rtm::kHalf; x0 += 64) {
+        int zA = -1, zB = -1;   // run of interior tiles in this column
+        for (int z0 = rtm::kHalf; z0 < Z - rtm::kHalf; z0 += TZ) {
+            const int cls = rtm::tile_class(d, x0, z0, TZ);
+            if (seg && cls == 0) {
+                // an interior tile: all 64 x TZ points lie in the stepped
+                // region (the strips are at least 9 points thick)
+                if (zA < 0) zA = z0;
+                zB = z0 + TZ;
+            } else {
+                tiles.push_back(make_int4(sim, x0, z0, cls));
+            }
+        }
+        for (int z = zA; zA >= 0 && z < zB; z += seg) {
+            const int e = z + seg < zB ? z + seg : zB;
+            march.push_back(make_int4(sim, x0, z, e));
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // executor
 // ---------------------------------------------------------------------------
 
-int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread)
+int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_march)
 {
     real = real_type;
     rz = rows_per_thread;
@@ -326,6 +343,11 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread)
     HIP_TRY(hipMemset(d_desc, 0, dsz * cap_sims));
     HIP_TRY(hipMemset(d_n0, 0, sizeof(int) * cap_sims));
     HIP_TRY(hipMemset(d_ntiles, 0, sizeof(int)));
+    cap_march = max_march > 0 ? max_march : 1;
+    HIP_TRY(hipMalloc((void **)&d_march, sizeof(int4) * cap_march));
+    HIP_TRY(hipMalloc((void **)&d_nmarch, sizeof(int)));
+    HIP_TRY(hipMemset(d_nmarch, 0, sizeof(int)));
+    nmarch = 0;
     h_n0.assign(cap_sims, 0);
     return 0;
 }
@@ -343,10 +365,26 @@ int Exec::drop_graph()
 void Exec::release()
 {
     drop_graph();
-    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles};
+    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles, d_march, d_nmarch};
     for (void *p : all)
         if (p) (void)hipFree(p);
     d_desc = nullptr; d_n0 = nullptr; d_tiles = nullptr; d_ntiles = nullptr;
+    d_march = nullptr; d_nmarch = nullptr;
+    nmarch = ntiles = 0;
+}
+
+int Exec::set_march(const std::vector<int4> &items, hipStream_t s)
+{
+    if ((int)items.size() > cap_march) {
+        rtm_set_error("march list of %zu exceeds the executor capacity %d", items.size(), cap_march);
+        return -1;
+    }
+    nmarch = (int)items.size();
+    if (nmarch) HIP_TRY(hipMemcpyAsync(d_march, items.data(), sizeof(int4) * nmarch, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_nmarch, &nmarch, sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    drop_graph();
+    return 0;
 }
 
 int Exec::set_tiles(const std::vector<int4> &tiles, hipStream_t s)
@@ -359,6 +397,7 @@ int Exec::set_tiles(const std::vector<int4> &tiles, hipStream_t s)
     if (ntiles) HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * ntiles, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
+    drop_graph();
     return 0;
 }
 
@@ -384,8 +423,18 @@ int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
 template <typename R>
 static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
 {
-    const int grid = 8 * ((e.cap_tiles + 7) / 8);
     const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
+    if (e.nmarch) {
+        const int mgrid = 8 * ((e.cap_march + 7) / 8);
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_march<R, 1>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_march<R, 2>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_march<R, 8>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
+        default: hipLaunchKernelGGL((k_march<R, 4>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
+        }
+    }
+    if (!e.ntiles) return;
+    const int grid = 8 * ((e.cap_tiles + 7) / 8);
     switch (e.rz) {
     case 1: hipLaunchKernelGGL((k_tiles<R, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
     case 2: hipLaunchKernelGGL((k_tiles<R, 2>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;

@@ -6,6 +6,7 @@
 
 #include "rtm_step.h"
 #include "rtm_tile.h"
+#include "rtm_march.h"
 
 template <typename R>
 static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const R *ax, const R *bx,
@@ -48,24 +49,71 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
         for (int tx = 0; tx < 64; tx++) rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
 }
 
+// Emulation of k_march for one work item {x0, zA, zB}.
 template <typename R, int RZ>
-static void run_tiles(const rtm::SimDesc<R> &S, int n)
+static void run_march(rtm::MarchLds<R, RZ> &L, rtm::MarchRegs<R, RZ> *cur, rtm::MarchRegs<R, RZ> *nxt,
+                      const rtm::SimDesc<R> &S, const rtm::StepView<R> &V, int x0, int zA, int zB)
+{
+    const int TZ = 4 * RZ;
+    for (int th = 0; th < 256; th++) {
+        rtm::march_fetch_rows<R, RZ, RZ + 2>(cur[th], S, V, x0, zA - 4, th & 63, th >> 6);
+        rtm::march_fetch_points(cur[th], S, V, x0, zA, th & 63, th >> 6);
+    }
+    for (int th = 0; th < 256; th++) rtm::march_stage_rows<R, RZ, RZ + 2>(L, cur[th], zA - 4, th & 63, th >> 6);
+    for (int zc = zA; zc < zB; zc += TZ) {
+        const bool more = zc + TZ < zB;
+        if (more)
+            for (int th = 0; th < 256; th++) {
+                rtm::march_fetch_rows<R, RZ, RZ>(nxt[th], S, V, x0, zc + TZ + 4, th & 63, th >> 6);
+                rtm::march_fetch_points(nxt[th], S, V, x0, zc + TZ, th & 63, th >> 6);
+            }
+        for (int th = 0; th < 256; th++) rtm::march_points(L, cur[th], S, V, x0, zc, th & 63, th >> 6);
+        if (more)
+            for (int th = 0; th < 256; th++) {
+                rtm::march_stage_rows<R, RZ, RZ>(L, nxt[th], zc + TZ + 4, th & 63, th >> 6);
+                rtm::march_copy(cur[th], nxt[th]);
+            }
+    }
+}
+
+// One step of one simulation: tiles where a CPML strip is touched (all tiles
+// when march_rows == 0) and marched interior segments otherwise, split the
+// way the library's work-list builder splits them.
+template <typename R, int RZ>
+static void run_tiles(const rtm::SimDesc<R> &S, int n, int march_rows)
 {
     const rtm::StepView<R> V = rtm::step_view(S, n);
     if (!V.active) return;
     auto *L = new rtm::TileLds<R, RZ>();
     auto *G = new rtm::TileRegs<R, RZ>[256];
+    auto *M = new rtm::MarchLds<R, RZ>();
+    auto *cur = new rtm::MarchRegs<R, RZ>[256];
+    auto *nxt = new rtm::MarchRegs<R, RZ>[256];
     const int TZ = 4 * RZ;
-    for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64)
+    const int seg = march_rows > 0 ? (march_rows + TZ - 1) / TZ * TZ : 0;
+    for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64) {
+        int zA = -1, zB = -1;
         for (int z0 = rtm::kHalf; z0 < S.Z - rtm::kHalf; z0 += TZ) {
+            const int cls = rtm::tile_class(S, x0, z0, TZ);
+            if (seg && cls == 0) {
+                if (zA < 0) zA = z0;
+                zB = z0 + TZ;
+                continue;
+            }
             // the LDS holds stale data from the previous tile, as on the GPU
-            switch (rtm::tile_class(S, x0, z0, TZ)) {
+            switch (cls) {
             case 0: run_tile<R, RZ, 0>(*L, G, S, V, x0, z0); break;
             case 1: run_tile<R, RZ, 1>(*L, G, S, V, x0, z0); break;
             case 2: run_tile<R, RZ, 2>(*L, G, S, V, x0, z0); break;
             default: run_tile<R, RZ, 3>(*L, G, S, V, x0, z0); break;
             }
         }
+        for (int z = zA; zA >= 0 && z < zB; z += seg)
+            run_march<R, RZ>(*M, cur, nxt, S, V, x0, z, z + seg < zB ? z + seg : zB);
+    }
+    delete[] nxt;
+    delete[] cur;
+    delete M;
     delete[] G;
     delete L;
 }
@@ -79,7 +127,7 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.TL = g[3]; S.TR = g[4]; S.TU = g[5]; S.TD = g[6];
     S.nx = g[7]; S.nz = g[8]; S.nt = g[9]; S.TS = g[10]; S.nw = g[11]; S.nrec = g[12];
     S.phase = g[13]; S.flags = g[14]; S.src_j = g[15]; S.rec_row = g[16];
-    const int n = g[17], rz = g[18];
+    const int n = g[17], rz = g[18], march_rows = g[19];
     S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
     S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
     S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];
@@ -88,10 +136,10 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.snaps = ptr[13]; S.I = ptr[14]; S.Is = ptr[15]; S.Ic = ptr[16]; S.Isc = ptr[17];
     S.recmap = recmap; S.wav = wav; S.seis_t = seis_t; S.seis_rec = seis_rec;
     switch (rz) {
-    case 1: run_tiles<R, 1>(S, n); break;
-    case 2: run_tiles<R, 2>(S, n); break;
-    case 8: run_tiles<R, 8>(S, n); break;
-    default: run_tiles<R, 4>(S, n); break;
+    case 1: run_tiles<R, 1>(S, n, march_rows); break;
+    case 2: run_tiles<R, 2>(S, n, march_rows); break;
+    case 8: run_tiles<R, 8>(S, n, march_rows); break;
+    default: run_tiles<R, 4>(S, n, march_rows); break;
     }
 }
 

@@ -131,7 +131,8 @@ class DevSim:
 
     # ---- the tiled kernel (rtm_tile.h), emulated phase by phase ----
 
-    def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False):
+    def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False,
+                    march=0):
         op = self.op
         self.reset()
         nint = op.nx * op.nz
@@ -152,19 +153,19 @@ class DevSim:
         for n in range(op.nt):
             g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx,
                           op.nz, op.nt, TS, op.n_wavelet, op.n_receivers, phase, flags, src_j,
-                          self.rec_row, n, rz], np.int32)
+                          self.rec_row, n, rz, march], np.int32)
             fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
         self.cur = op.nt & 1
         return snaps, I, Is, seis_rec
 
-    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False):
+    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, march=0):
         op = self.op
         src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
         flags = 4 | (2 if record else 0)
-        snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record)
+        snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record, march=march)
         return snaps, np.ascontiguousarray(rec.T)
 
-    def backward_tiled(self, seis, snaps, rz=4, kahan=False):
+    def backward_tiled(self, seis, snaps, rz=4, kahan=False, march=0):
         flags = 8 | (1 if kahan else 0)
-        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps)
+        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, march=march)
         return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)

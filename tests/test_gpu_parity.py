@@ -87,6 +87,77 @@ def test_tilings_and_graph_blocks_bitexact(gpu_available, rz, block):
     assert np.array_equal(I, I_orc) and np.array_equal(Is, Is_orc)
 
 
+def _wide_case():
+    """A grid whose interior spans several tile columns and marched chunks,
+    with the source and the receivers in the interior."""
+    cfg = dict(Nx=230, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
+               spacingGrid=10.0, sampling=0.001, numberOfTimeStep=120, sampligWavelet=120,
+               firstLocationShot=100, depthShot=30, spacingShot=1, numberOfShots=1,
+               firstLocationReceiver=3, depthReceiver=35, spacingReceiver=3, numberOfReceivers=100,
+               numberOfSimulations=1, zfp_tolerance=1e-4)
+    rng = np.random.default_rng(21)
+    model = rng.uniform(1800.0, 3200.0, (cfg["Nx"], cfg["Nz"])).astype(np.float32)
+    from rtmuq2d_amd import synth
+    wav = synth.ricker(25.0, 0.001, cfg["numberOfTimeStep"])
+    seis = rng.standard_normal((cfg["numberOfReceivers"], cfg["numberOfTimeStep"])).astype(np.float32)
+    return cfg, model, wav, seis
+
+
+@pytest.mark.parametrize("rz,march", [(1, 8), (2, 16), (4, 48), (4, 1000), (8, 64)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_marched_interior_bitexact(gpu_available, rz, march, dtype):
+    """Strip tiles plus the streaming interior kernel: fields, recorded
+    traces and images bit-identical to the oracle."""
+    say(f"[gpu] march rz={rz} rows={march} {np.dtype(dtype).name}")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=10)
+    p = common.rtm_params(cfg, time_slices=10)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    S, d_orc, lu, lU = O.forward(op, sx, sz, vpad, wav, dtype=dtype, record=True)
+    I, Is, lu2, lU2 = O.backward(op, vpad, seis, S)
+    f64 = dtype == np.float64
+    with R.Plan(p, R.REAL_F64 if f64 else R.REAL_F32) as plan:
+        plan.set_tiling(rz, 100, march)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.forward(sx, sz, record=True)
+        a, b = plan.fields(f64=f64)
+        assert np.array_equal(a, lu) and np.array_equal(b, lU)
+        if not f64:
+            assert np.array_equal(plan.seismogram(), d_orc)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.backward()
+        a, b = plan.fields(f64=f64)
+        assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
+        I2, Is2 = plan.image(f64=f64)
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
+def test_batch_marched_bitexact(gpu_available):
+    """A batch with the streaming interior kernel: every realization's image
+    equals the oracle's, in the batch as alone."""
+    say("[gpu] batch with marched interior")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=10)
+    p = common.rtm_params(cfg, time_slices=10)
+    sx, sz = common.shot_positions(cfg)[0]
+    vpads = [O.extend(op, model * np.float32(1 + 0.03 * k)) for k in range(3)]
+    refs = [O.rtm_shot(op, sx, sz, v, wav, seis) for v in vpads]
+    with R.Batch(p, len(vpads)) as b:
+        b.set_tiling(4, 100, 32)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(wav)
+        b.set_seismogram(seis)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k in range(len(vpads)):
+            I, Is = b.image(k)
+            assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
+
+
 @pytest.mark.parametrize("rz", [2, 8])
 def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
     """A batch of realizations advanced in lockstep: each image bit-identical
