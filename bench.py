@@ -37,6 +37,19 @@ METRIC = "RTM images/hour + Gpt-updates/s, batched vs plain, 1 GPU, image relL2<
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
 
+# kernel shapes swept: (tile rows per thread, graph block, interior kernel
+# 0 tiles / 1 march / 2 columns, rows per interior segment)
+PLAIN_SHAPES = [(1, 100, 0, 0), (2, 100, 0, 0), (4, 100, 0, 0), (2, 1, 0, 0), (2, 100, 1, 32),
+                (2, 100, 2, 0), (2, 100, 2, 32), (2, 100, 2, 16), (4, 100, 2, 16), (1, 100, 2, 8)]
+BATCH_SHAPES = [(4, 100, 0, 0), (4, 100, 1, 1000), (4, 100, 2, 0), (4, 100, 2, 64), (4, 100, 2, 32),
+                (2, 100, 2, 32), (8, 100, 2, 32)]
+
+
+def tiling_dict(shape):
+    rz, block, interior, seg = shape
+    return {"rows_per_thread": rz, "graph_block": block,
+            "interior": ("tiles", "march", "columns")[interior], "seg_rows": seg}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -110,6 +123,8 @@ def main() -> None:
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed workload for counter profiles (see run_profile)")
+    ap.add_argument("--profile-sims", type=int, default=16, help="realizations in the profile batch")
+    ap.add_argument("--profile-nt", type=int, default=400, help="time steps of the profile workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the RTM path shards by "
@@ -121,7 +136,7 @@ def main() -> None:
         log("no HIP device")
         sys.exit(1)
     if args.profile:
-        run_profile(R)
+        run_profile(R, args.profile_nt, args.profile_sims)
         return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
@@ -151,19 +166,16 @@ def main() -> None:
     # kernel-shape sweep (one image each); the fastest is timed
     best = None
     tuning = []
-    for rz, block, march in [(1, 100, 0), (2, 100, 0), (4, 100, 0), (2, 1, 0), (1, 100, 32), (2, 100, 32),
-                             (2, 100, 64), (4, 100, 64), (4, 100, 128)]:
-        plan.set_tiling(rz, block, march)
+    for shape in PLAIN_SHAPES:
+        plan.set_tiling(*shape)
         one_image()
         f_ms, b_ms, _ = plan.last_timing()
-        tuning.append({"rows_per_thread": rz, "graph_block": block, "march_rows": march,
-                       "fwd_us": round(f_ms * 1e3, 3), "bwd_us": round(b_ms * 1e3, 3)})
-        log(f"[bench] tiling rz={rz} block={block} march={march}: fwd {f_ms * 1e3:.2f} us/step, "
-            f"bwd {b_ms * 1e3:.2f} us/step")
+        tuning.append(dict(tiling_dict(shape), fwd_us=round(f_ms * 1e3, 3), bwd_us=round(b_ms * 1e3, 3)))
+        log(f"[bench] tiling {tiling_dict(shape)}: fwd {f_ms * 1e3:.2f} us/step, bwd {b_ms * 1e3:.2f} us/step")
         if best is None or f_ms + b_ms < best[0]:
-            best = (f_ms + b_ms, rz, block, march)
-    plan.set_tiling(best[1], best[2], best[3])
-    log(f"[bench] using rz={best[1]} graph_block={best[2]} march_rows={best[3]}")
+            best = (f_ms + b_ms, shape)
+    plan.set_tiling(*best[1])
+    log(f"[bench] using {tiling_dict(best[1])}")
 
     for i in range(args.warmup):
         one_image()
@@ -203,7 +215,7 @@ def main() -> None:
             "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
                         "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
             "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
-            "tiling": {"rows_per_thread": best[1], "graph_block": best[2], "march_rows": best[3]},
+            "tiling": tiling_dict(best[1]),
         },
         "tiling_sweep": tuning,
         "gupdates_per_s": round(updates / secs / 1e9, 3),
@@ -227,7 +239,7 @@ def main() -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_profile(R, nt=400):
+def run_profile(R, nt=400, nsims=16):
     """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
     pass per counter group over the same dispatches):
       1. a 1 GiB device-to-device copy (torch), the calibration of the
@@ -246,7 +258,7 @@ def run_profile(R, nt=400):
     v_true, v_mig, w = c1_inputs(p)
     seis = np.zeros((p.number_of_receivers, nt), np.float32)
     with R.Plan(p) as plan:
-        plan.set_tiling(2, 100, 0)
+        plan.set_tiling(2, 100, 0, 0)
         plan.set_velocity(R.extend(p, v_mig))
         plan.set_wavelet(w)
         plan.set_seismogram(seis)
@@ -254,17 +266,17 @@ def run_profile(R, nt=400):
         plan.migrate(p.first_location_shot, p.depth_shot)
         log("[profile] plain", plan.last_timing())
     pb = p.copy().update(time_slices=20)
-    with R.Batch(pb, 16) as b:
+    with R.Batch(pb, nsims) as b:
         vpad = R.extend(pb, v_mig)
-        for k in range(16):
+        for k in range(nsims):
             b.set_velocity(k, vpad)
         b.set_wavelet(w)
         b.set_seismogram(seis)
-        for rz, march in ((4, 0), (4, 1000)):
-            b.set_tiling(rz, 100, march)
+        for shape in ((4, 100, 0, 0), (4, 100, 2, 32)):
+            b.set_tiling(*shape)
             b.zero_images()
             b.migrate(pb.first_location_shot, pb.depth_shot)
-            log(f"[profile] batch16 rz={rz} march={march}", b.last_timing())
+            log(f"[profile] batch{nsims} {tiling_dict(shape)}", b.last_timing())
     print(json.dumps({"profile": "done", "nt": nt}), flush=True)
 
 
@@ -296,14 +308,14 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     plan.set_wavelet(w)
     plan.set_seismogram(seis)
     best = None
-    for rz, march in ((1, 0), (2, 0), (4, 0), (2, 32), (2, 64)):
-        plan.set_tiling(rz, 100, march)
+    for shape in PLAIN_SHAPES:
+        plan.set_tiling(*shape)
         plan.zero_image()
         plan.migrate(sx, sz)
         f_ms, b_ms, _ = plan.last_timing()
         if best is None or f_ms + b_ms < best[0]:
-            best = (f_ms + b_ms, rz, march)
-    plan.set_tiling(best[1], 100, best[2])
+            best = (f_ms + b_ms, shape)
+    plan.set_tiling(*best[1])
     reps = 3
     plan.synchronize()
     t0 = time.perf_counter()
@@ -314,7 +326,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     plain_secs = (time.perf_counter() - t0) / reps
     I_plain, _ = plan.image()
     plan.close()
-    log(f"[bench] plain (1 sim, TS={ts}) rz={best[1]} march={best[2]}: {plain_secs * 1e3:.1f} ms/image")
+    log(f"[bench] plain (1 sim, TS={ts}) {tiling_dict(best[1])}: {plain_secs * 1e3:.1f} ms/image")
 
     # FP64 oracle run of realization 0
     with R.Plan(p, R.REAL_F64) as p64:
@@ -332,19 +344,19 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     b.set_seismogram(seis)
     sweep = []
     bbest = None
-    for rz, march in ((4, 0), (2, 1000), (4, 1000), (8, 1000)):
-        b.set_tiling(rz, 100, march)
+    for shape in BATCH_SHAPES:
+        b.set_tiling(*shape)
         b.zero_images()
         t0 = time.perf_counter()
         b.migrate(sx, sz)
         dt = time.perf_counter() - t0
         f_ms, b_ms, _ = b.last_timing()
-        sweep.append({"rows_per_thread": rz, "march_rows": march, "seconds": round(dt, 4),
-                      "fwd_us": round(f_ms * 1e3, 2), "bwd_us": round(b_ms * 1e3, 2)})
-        log(f"[bench] batch rz={rz} march={march}: {dt:.3f} s ({f_ms * 1e3:.1f} / {b_ms * 1e3:.1f} us per step)")
+        sweep.append(dict(tiling_dict(shape), seconds=round(dt, 4), fwd_us=round(f_ms * 1e3, 2),
+                          bwd_us=round(b_ms * 1e3, 2)))
+        log(f"[bench] batch {tiling_dict(shape)}: {dt:.3f} s ({f_ms * 1e3:.1f} / {b_ms * 1e3:.1f} us per step)")
         if bbest is None or dt < bbest[0]:
-            bbest = (dt, rz, march)
-    b.set_tiling(bbest[1], 100, bbest[2])
+            bbest = (dt, shape)
+    b.set_tiling(*bbest[1])
     b.zero_images()
     t0 = time.perf_counter()
     b.migrate(sx, sz)
@@ -369,7 +381,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "plain_images_per_hour_same_ts": round(plain_iph, 1),
         "batched_over_plain": round(batch_iph / plain_iph, 3),
         "step_kernel_us": {"forward": round(f_ms * 1e3, 2), "backward": round(b_ms * 1e3, 2)},
-        "tiling": {"rows_per_thread": bbest[1], "graph_block": 100, "march_rows": bbest[2]},
+        "tiling": tiling_dict(bbest[1]),
         "tiling_sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4)},

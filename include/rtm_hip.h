@@ -142,13 +142,19 @@ int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out);
 int rtm_plan_destroy(rtm_plan *plan);
 size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan allocates */
 
+/* Which kernel steps the interior (the points outside every CPML strip):
+ * TILES   the LDS tile kernel, as everywhere else;
+ * MARCH   workgroups march down 64-column strips with an LDS row ring;
+ * COLUMNS wavefronts march down 128-column strips with a register window
+ *         and lane shifts (no LDS, no barrier). */
+enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_MARCH = 1, RTM_INTERIOR_COLUMNS = 2 };
+
 /* Kernel shape: rows_per_thread in {1,2,4,8} gives tiles of 64 x
  * 4*rows_per_thread points; graph_block = time steps per captured hipGraph
- * block (<= 1: one eager launch per step); march_rows > 0 streams the
- * interior (points outside every CPML strip) with the marching kernel in
- * segments of about march_rows rows, leaving tiles only where a strip is
- * touched (0: tiles everywhere).  Defaults: 4, 100, 0. */
-int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int march_rows);
+ * block (<= 1: one eager launch per step); interior = enum rtm_interior;
+ * seg_rows = rows per marched segment (MARCH, COLUMNS; <= 0: whole strips).
+ * Tiles stay wherever a CPML strip is touched.  Defaults: 4, 100, TILES, 0. */
+int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int interior, int seg_rows);
 
 int rtm_plan_set_velocity(rtm_plan *plan, const float *vpad);     /* padded [X][Z] */
 int rtm_plan_set_wavelet(rtm_plan *plan, const float *w, int n);
@@ -186,7 +192,7 @@ int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_batch **out);
 int rtm_batch_destroy(rtm_batch *batch);
 size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real);
 int rtm_batch_size(rtm_batch *batch);
-int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block, int march_rows);
+int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block, int interior, int seg_rows);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
 int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
 int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */

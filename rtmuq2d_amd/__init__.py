@@ -22,6 +22,7 @@ PATH_LEN = 512
 
 SNAP_FP32, SNAP_FP16, SNAP_BF16, SNAP_BOUNDARY = 0, 1, 2, 3
 REAL_F32, REAL_F64 = 0, 1
+INTERIOR_TILES, INTERIOR_MARCH, INTERIOR_COLUMNS = 0, 1, 2
 
 
 class RtmParams(C.Structure):
@@ -112,7 +113,7 @@ def lib() -> C.CDLL:
         "rtm_plan_create": (C.c_int, [P(RtmParams), C.c_int, P(plan_p)]),
         "rtm_plan_destroy": (C.c_int, [plan_p]),
         "rtm_plan_footprint": (C.c_size_t, [P(RtmParams), C.c_int]),
-        "rtm_plan_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int]),
+        "rtm_plan_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int, C.c_int]),
         "rtm_plan_set_velocity": (C.c_int, [plan_p, f32p]),
         "rtm_plan_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
         "rtm_plan_set_seismogram": (C.c_int, [plan_p, f32p]),
@@ -131,7 +132,7 @@ def lib() -> C.CDLL:
         "rtm_batch_destroy": (C.c_int, [plan_p]),
         "rtm_batch_footprint": (C.c_size_t, [P(RtmParams), C.c_int, C.c_int]),
         "rtm_batch_size": (C.c_int, [plan_p]),
-        "rtm_batch_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int]),
+        "rtm_batch_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int, C.c_int]),
         "rtm_batch_set_velocity": (C.c_int, [plan_p, C.c_int, f32p]),
         "rtm_batch_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
         "rtm_batch_set_seismogram": (C.c_int, [plan_p, f32p]),
@@ -221,9 +222,13 @@ class Plan:
     def __exit__(self, *a):
         self.close()
 
-    def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, march_rows: int = 0) -> None:
-        _check(lib().rtm_plan_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(march_rows)),
-               "set_tiling")
+    def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, interior: int = 0,
+                   seg_rows: int = 0) -> None:
+        """Kernel shape (rtm_plan_set_tiling): tile rows per thread, steps per
+        graph block, interior kernel (INTERIOR_TILES / _MARCH / _COLUMNS) and
+        rows per marched segment."""
+        _check(lib().rtm_plan_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
+                                         int(seg_rows)), "set_tiling")
 
     def set_velocity(self, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)
@@ -346,9 +351,10 @@ class Batch:
     def __exit__(self, *a):
         self.close()
 
-    def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, march_rows: int = 0) -> None:
-        _check(lib().rtm_batch_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(march_rows)),
-               "set_tiling")
+    def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, interior: int = 0,
+                   seg_rows: int = 0) -> None:
+        _check(lib().rtm_batch_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
+                                          int(seg_rows)), "set_tiling")
 
     def set_velocity(self, sim: int, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)

@@ -43,7 +43,7 @@ struct rtm_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int real = RTM_REAL_F32;
-    int rz = 4, graph_block = 100, march_rows = 0;
+    int rz = 4, graph_block = 100, interior = rtm::IN_TILES, seg_rows = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
 };
@@ -58,16 +58,18 @@ static void batch_free(rtm_batch *q)
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
-static int batch_set_tiling(rtm_batch *q, int rz, int block, int march_rows)
+static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
 {
-    std::vector<int4> tiles, march;
-    for (int s = 0; s < (int)q->sims.size(); s++) append_work(tiles, march, s, q->sims[s].p, rz, march_rows);
+    std::vector<int4> tiles, inner;
+    for (int s = 0; s < (int)q->sims.size(); s++)
+        append_work(tiles, inner, s, q->sims[s].p, rz, interior, seg_rows);
     q->ex.release();
-    if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz, (int)march.size())) return -1;
-    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_march(march, q->stream)) return -1;
+    if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = block;
-    q->march_rows = march_rows;
+    q->interior = interior;
+    q->seg_rows = seg_rows;
     return 0;
 }
 
@@ -103,7 +105,7 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     for (int s = 0; s < nsims; s++)
         if (q->sims[s].alloc(*p, real, s ? &q->sims[0] : nullptr)) return fail();
     const char *env = getenv("RTM_TILE_RZ");
-    if (batch_set_tiling(q, env && *env ? atoi(env) : 4, 100, 0)) return fail();
+    if (batch_set_tiling(q, env && *env ? atoi(env) : 4, 100, rtm::IN_TILES, 0)) return fail();
     *out = q;
     return 0;
 }
@@ -118,11 +120,11 @@ extern "C" int rtm_batch_destroy(rtm_batch *q)
     return 0;
 }
 
-extern "C" int rtm_batch_set_tiling(rtm_batch *q, int rows_per_thread, int graph_block, int march_rows)
+extern "C" int rtm_batch_set_tiling(rtm_batch *q, int rows_per_thread, int graph_block, int interior, int seg_rows)
 {
     if (!q) { rtm_set_error("rtm_batch_set_tiling: null batch"); return -1; }
     HIP_TRY(hipStreamSynchronize(q->stream));
-    return batch_set_tiling(q, rows_per_thread, graph_block, march_rows);
+    return batch_set_tiling(q, rows_per_thread, graph_block, interior, seg_rows);
 }
 
 extern "C" int rtm_batch_size(rtm_batch *q) { return q ? (int)q->sims.size() : 0; }

@@ -1,0 +1,119 @@
+/*
+ * rtm_column.h -- the interior column kernel body: one wavefront marches
+ * down a strip of 128 (or 64) columns of the region that no CPML strip
+ * touches, with no LDS and no barrier.
+ *
+ * Lane l owns the two adjacent columns x0+2l and x0+2l+1 of every row.
+ * z-neighbours come from a register window of rows k-4 .. k+7 that slides
+ * down four rows at a time while the next four rows are already in flight;
+ * x-neighbours come from lanes l-1, l-2 (x-4 .. x-1) and l+1, l+2 (x+2 ..
+ * x+5) through DPP wave shifts.  The halo columns x0-4 .. x0-1 (lane 0) and
+ * x0+128 .. x0+131 (lane 63) arrive as the "old" operand of the shift, from
+ * one 4-wide load per row.  In a 64-column strip lanes 32 .. 63 load the
+ * next 64 columns (in bounds: the strip ends at least 4 points before the
+ * right CPML) and only supply halo values.
+ *
+ * Per point the work is the stencil of rtm::stencil_val (identical
+ * association order, so identical bits) plus, on the rows where they apply,
+ * the source add, receiver replacement or recording, snapshot store and
+ * imaging of rtm_tile.h's tile_points.  The CPML strips themselves are
+ * stepped by the tile kernel in the same launch sequence.
+ */
+#ifndef RTM_COLUMN_H
+#define RTM_COLUMN_H
+
+#include "rtm_tile.h"
+
+namespace rtm {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) Pair {
+    R a, b;
+};
+
+template <typename R>
+struct alignas(4 * sizeof(R)) Quad {
+    R x, y, z, w;
+};
+
+/* Column-strip work item {sim | half << 24, x0, zA, zB}: columns x0 ..
+ * x0+127 (x0+63 when `half`), rows zA .. zB-1 (a multiple of 4 rows). */
+RTM_HD int col_sim(int packed) { return packed & 0xffffff; }
+RTM_HD int col_half(int packed) { return packed >> 24; }
+RTM_HD int col_pack(int sim, int half) { return sim | (half << 24); }
+
+/* The two updated points of one lane at row k.  w[0..8] are rows k-4 ..
+ * k+4 of the lane's columns (x, x+1); L1 = (x-2, x-1), L2 = (x-4, x-3),
+ * R1 = (x+2, x+3), R2 = (x+4, x+5). */
+template <typename R>
+RTM_HD Pair<R> col_update(const Pair<R> *w, Pair<R> L1, Pair<R> L2, Pair<R> R1, Pair<R> R2, Pair<R> Cp,
+                          Pair<R> Op)
+{
+    const Pair<R> c = w[4];
+    Pair<R> U;
+    U.a = stencil_val(Cp.a, Op.a, c.a, L1.b, c.b, w[3].a, w[5].a, L1.a, R1.a, w[2].a, w[6].a,
+                      L2.b, R1.b, w[1].a, w[7].a, L2.a, R2.a, w[0].a, w[8].a);
+    U.b = stencil_val(Cp.b, Op.b, c.b, c.a, R1.a, w[3].b, w[5].b, L1.b, R1.b, w[2].b, w[6].b,
+                      L1.a, R2.a, w[1].b, w[7].b, L2.b, R2.b, w[0].b, w[8].b);
+    return U;
+}
+
+/* Source add, receiver replacement or recording, snapshot store and
+ * imaging for the lane's two points at row k (in the order of
+ * tile_points); the row tests are uniform across the wavefront. */
+template <typename R>
+RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl, bool ok, int src_row, Pair<R> &U,
+                       Pair<R> Cp)
+{
+    if (V.forward) {
+        if (k == src_row) {
+            const int j = k * S.P + xl;
+            if (j == S.src_j) U.a = U.a + V.src_w * Cp.a;
+            if (j + 1 == S.src_j) U.b = U.b + V.src_w * Cp.b;
+        }
+    }
+    if (k == S.rec_row && (!V.forward || V.rec_row_out)) {
+        const int ra = ok ? ldg(S.recmap, xl) : -1;
+        const int rb = ok ? ldg(S.recmap, xl + 1) : -1;
+        if (V.forward) {
+            if (ra >= 0) stg(V.rec_row_out, ra, float(U.a));
+            if (rb >= 0) stg(V.rec_row_out, rb, float(U.b));
+        } else {
+            if (ra >= 0) U.a = R(ldg(V.seis_row, ra));
+            if (rb >= 0) U.b = R(ldg(V.seis_row, rb));
+        }
+    }
+    const int kk = k - S.TU;
+    if (!V.snap || kk < 0 || kk >= S.nz || !ok) return;
+    R vals[2] = {U.a, U.b};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const int ii = xl + e - S.TL;
+        if (ii < 0 || ii >= S.nx) continue;
+        const int q = kk * S.nx + ii;
+        if (V.forward) {
+            stg(V.snap, q, vals[e]);
+        } else if (S.flags & F_KAHAN) {
+            R I = ldg(S.I, q), Ic = ldg(S.Ic, q), Is = ldg(S.Is, q), Isc = ldg(S.Isc, q);
+            const R s = ldg(V.snap, q);
+            kahan_add(I, Ic, s * vals[e]);
+            kahan_add(Is, Isc, s * s);
+            stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
+        } else {
+            const R s = ldg(V.snap, q);
+            stg(S.I, q, ldg(S.I, q) + s * vals[e]);
+            stg(S.Is, q, ldg(S.Is, q) + s * s);
+        }
+    }
+}
+
+/* Source row of a sim this step (-1 when no source is added). */
+template <typename R>
+RTM_HD int col_src_row(const SimDesc<R> &S, const StepView<R> &V)
+{
+    return (V.forward && V.src) ? S.src_j / S.P : -1;
+}
+
+}  // namespace rtm
+
+#endif /* RTM_COLUMN_H */

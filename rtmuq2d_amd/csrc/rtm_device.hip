@@ -52,7 +52,8 @@ struct rtm_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int rz = 4;                     // tile rows per thread
     int graph_block = 100;          // steps per captured graph block (<= 1: eager)
-    int march_rows = 0;             // interior marched in segments of this many rows (0: tiles only)
+    int interior = rtm::IN_TILES;   // interior kernel (rtm::Interior)
+    int seg_rows = 0;               // rows per marched interior segment
     int last_nt = 0;                // steps of the last pass
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
@@ -73,16 +74,17 @@ static void plan_free(rtm_plan *q)
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
-static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int march_rows)
+static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, int seg_rows)
 {
-    std::vector<int4> tiles, march;
-    append_work(tiles, march, 0, q->b.p, rz, march_rows);
+    std::vector<int4> tiles, inner;
+    append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
     q->ex.release();
-    if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)march.size())) return -1;
-    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_march(march, q->stream)) return -1;
+    if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = graph_block;
-    q->march_rows = march_rows;
+    q->interior = interior;
+    q->seg_rows = seg_rows;
     return 0;
 }
 
@@ -110,7 +112,7 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
     if (q->b.alloc(*p, real)) return fail();
     const char *env = getenv("RTM_TILE_RZ");
     const int rz = env && *env ? atoi(env) : 4;
-    if (plan_set_tiling(q, rz, 100, 0)) return fail();
+    if (plan_set_tiling(q, rz, 100, rtm::IN_TILES, 0)) return fail();
     *out = q;
     return 0;
 }
@@ -125,12 +127,12 @@ extern "C" int rtm_plan_destroy(rtm_plan *q)
     return 0;
 }
 
-extern "C" int rtm_plan_set_tiling(rtm_plan *q, int rows_per_thread, int graph_block, int march_rows)
+extern "C" int rtm_plan_set_tiling(rtm_plan *q, int rows_per_thread, int graph_block, int interior, int seg_rows)
 {
     if (!q) { rtm_set_error("rtm_plan_set_tiling: null plan"); return -1; }
     HIP_TRY(hipSetDevice(q->b.p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
-    return plan_set_tiling(q, rows_per_thread, graph_block, march_rows);
+    return plan_set_tiling(q, rows_per_thread, graph_block, interior, seg_rows);
 }
 
 extern "C" int rtm_plan_set_velocity(rtm_plan *q, const float *vpad)

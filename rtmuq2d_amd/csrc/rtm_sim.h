@@ -17,6 +17,8 @@
 #include "rtm_step.h"
 #include "rtm_tile.h"
 #include "rtm_march.h"
+#include "rtm_column.h"
+#include "rtm_work.h"
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
@@ -130,6 +132,179 @@ __global__ void __launch_bounds__(256) k_march(const SimDesc<R> *__restrict__ si
             rtm::march_copy(cur, nxt);
         }
         __syncthreads();
+    }
+}
+
+// Wavefront shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
+// lane l-1's (l+1's) value; lane 0 (63) keeps `old`.
+__device__ __forceinline__ int dpp_shr1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_shl1_i(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_shr1(float v, float old)
+{
+    return __builtin_bit_cast(float, dpp_shr1_i(__builtin_bit_cast(int, v), __builtin_bit_cast(int, old)));
+}
+__device__ __forceinline__ float lane_shl1(float v, float old)
+{
+    return __builtin_bit_cast(float, dpp_shl1_i(__builtin_bit_cast(int, v), __builtin_bit_cast(int, old)));
+}
+__device__ __forceinline__ double lane_shr1(double v, double old)
+{
+    const int2 a = __builtin_bit_cast(int2, v), o = __builtin_bit_cast(int2, old);
+    return __builtin_bit_cast(double, make_int2(dpp_shr1_i(a.x, o.x), dpp_shr1_i(a.y, o.y)));
+}
+__device__ __forceinline__ double lane_shl1(double v, double old)
+{
+    const int2 a = __builtin_bit_cast(int2, v), o = __builtin_bit_cast(int2, old);
+    return __builtin_bit_cast(double, make_int2(dpp_shl1_i(a.x, o.x), dpp_shl1_i(a.y, o.y)));
+}
+
+// One lane's per-row operands for four rows: C, U_old and the halo quad.
+template <typename R>
+struct ColSet {
+    rtm::Pair<R> C[4], O[4];
+    rtm::Quad<R> H[4];
+};
+
+// Per-wavefront constants of a column item.
+template <typename R>
+struct ColCtx {
+    const rtm::Pair<R> *u2, *C2;
+    rtm::Pair<R> *U2;
+    const rtm::Quad<R> *u4;
+    int Ph, Pq;     // row pitch in pairs and in quads
+    int xh, hq;     // the lane's pair index and halo quad index within a row
+    int xl;         // the lane's first column
+    bool ok;        // the lane's columns belong to the item
+    int srow;       // source row this step (-1: none)
+};
+
+// Loads of chunk rows k0 .. k0+3: u rows k0+4 .. k0+7 into window slots
+// (WS+rr) & 15, and C, U_old, halo of the rows themselves into `st`.  Rows
+// past the item (the prefetch after its last chunk) are clamped to its last
+// row: the loads are always issued, so every chunk waits for exactly the
+// loads of the previous prefetch and never for the one just issued.
+template <typename R, int WS>
+__device__ __forceinline__ void col_fetch(rtm::Pair<R> (&w)[16], ColSet<R> &st, const ColCtx<R> &X, int k0,
+                                          int zB)
+{
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int k = k0 + rr < zB ? k0 + rr : zB - 1;
+        w[(WS + rr) & 15] = rtm::ldg(X.u2 + (k + 4) * X.Ph, X.xh);
+        st.C[rr] = rtm::ldg(X.C2 + k * X.Ph, X.xh);
+        st.O[rr] = rtm::ldg((const rtm::Pair<R> *)X.U2 + k * X.Ph, X.xh);
+        st.H[rr] = rtm::ldg(X.u4 + k * X.Pq, X.hq);
+    }
+}
+
+// Rows k0 .. k0+3; row k0+rr's window rows k-4 .. k+4 are slots
+// (B + rr + j) & 15, j = 0..8.  EXTRAS: some row of the chunk has a source,
+// receiver, snapshot or imaging term.
+template <typename R, int B, bool EXTRAS>
+__device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const ColSet<R> &st, const ColCtx<R> &X,
+                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0)
+{
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int k = k0 + rr;
+        rtm::Pair<R> win[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) win[j] = w[(B + rr + j) & 15];
+        const rtm::Pair<R> c = win[4];
+        const rtm::Quad<R> H = st.H[rr];
+        rtm::Pair<R> L1, L2, R1, R2;
+        L1.a = lane_shr1(c.a, H.z);
+        L1.b = lane_shr1(c.b, H.w);
+        L2.a = lane_shr1(L1.a, H.x);
+        L2.b = lane_shr1(L1.b, H.y);
+        R1.a = lane_shl1(c.a, H.x);
+        R1.b = lane_shl1(c.b, H.y);
+        R2.a = lane_shl1(R1.a, H.z);
+        R2.b = lane_shl1(R1.b, H.w);
+        rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, st.C[rr], st.O[rr]);
+        if (EXTRAS) rtm::col_extras(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr]);
+        if (X.ok) rtm::stg(X.U2 + k * X.Ph, X.xh, U);
+    }
+}
+
+// Chunk c of the four-chunk cycle (rows k0 .. k0+3): prefetch the next
+// chunk into the other set, then compute this one.
+template <typename R, int CI>
+__device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, ColSet<R> &nxt, const ColCtx<R> &X,
+                                         const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, int zB,
+                                         int ex_lo, int ex_hi)
+{
+    col_fetch<R, (4 * CI + 12) & 15>(w, nxt, X, k0 + 4, zB);
+    // rows [ex_lo, ex_hi) carry snapshot/imaging terms; single rows the
+    // source or receiver terms
+    const bool extras = (k0 < ex_hi && k0 + 4 > ex_lo) || (X.srow >= k0 && X.srow < k0 + 4)
+                        || (S.rec_row >= k0 && S.rec_row < k0 + 4 && (!V.forward || V.rec_row_out));
+    if (extras) col_chunk<R, 4 * CI, true>(w, cur, X, S, V, k0);
+    else col_chunk<R, 4 * CI, false>(w, cur, X, S, V, k0);
+}
+
+// The interior column kernel (rtm_column.h): four independent wavefronts
+// per workgroup, each marching one item {sim(+half), x0, zA, zB} with the
+// next four rows in flight.  The register window holds 16 rows (12 in use,
+// 4 arriving) and its slots cycle every four chunks, so the loop is
+// unrolled by four chunks and nothing is copied between registers.  Blocks
+// are remapped so that each XCD walks a contiguous run of items
+// (consecutive segments of a column share rows).
+template <typename R>
+__global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ sims,
+                                                 const int *__restrict__ n0,
+                                                 const int4 *__restrict__ items,
+                                                 const int *__restrict__ nitems_p, int s)
+{
+    const int nitems = *nitems_p;
+    const int nblk = (nitems + 3) >> 2;
+    const int per = (nblk + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= nblk) return;
+    const int item = t * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int4 it = items[item];
+    const int sim = rtm::col_sim(it.x);
+    const SimDesc<R> &S = sims[sim];
+    const rtm::StepView<R> V = rtm::step_view(S, n0[sim] + s);
+    if (!V.active) return;
+    const int lane = threadIdx.x & 63;
+    const int x0 = it.y, zA = it.z, zB = it.w;
+    const int P = S.P;
+    ColCtx<R> X;
+    X.u2 = (const rtm::Pair<R> *)V.u;
+    X.C2 = (const rtm::Pair<R> *)S.C;
+    X.U2 = (rtm::Pair<R> *)V.U;
+    X.u4 = (const rtm::Quad<R> *)V.u;
+    X.Ph = P >> 1;
+    X.Pq = P >> 2;
+    X.xl = x0 + 2 * lane;
+    X.xh = (X.xl < P - 2 ? X.xl : P - 2) >> 1;
+    X.hq = rtm::clampi(lane == 63 ? x0 + 128 : x0 - 4, 0, P - 4) >> 2;
+    X.ok = lane < (rtm::col_half(it.x) ? 32 : 64);
+    X.srow = rtm::col_src_row(S, V);
+    const int ex_lo = V.snap ? S.TU : 0, ex_hi = V.snap ? S.TU + S.nz : 0;
+    rtm::Pair<R> w[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) w[r] = rtm::ldg(X.u2 + (zA - 4 + r) * X.Ph, X.xh);
+    ColSet<R> A, Bs;
+    col_fetch<R, 8>(w, A, X, zA, zB);
+    for (int k0 = zA;; k0 += 16) {
+        col_step<R, 0>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi);
+        if (k0 + 4 >= zB) break;
+        col_step<R, 1>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi);
+        if (k0 + 8 >= zB) break;
+        col_step<R, 2>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi);
+        if (k0 + 12 >= zB) break;
+        col_step<R, 3>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi);
+        if (k0 + 16 >= zB) break;
     }
 }
 
@@ -311,16 +486,15 @@ struct SimBuffers {
 // plan memory in bytes for params p (the admission footprint)
 size_t sim_bytes(const rtm_params &p, size_t esz);
 
-// Work lists covering the stepped region of a grid.  With march_rows == 0:
-// tiles {sim, x0, z0, class} everywhere.  With march_rows > 0: tiles only
-// where a tile touches a CPML strip, and the remaining interior as march
-// items {sim, x0, zA, zB} of at most march_rows rows (whole tiles).
-void append_work(std::vector<int4> &tiles, std::vector<int4> &march, int sim, const rtm_params &p,
-                 int rz, int march_rows);
+// Work lists covering the stepped region of a grid (rtm::build_work): tiles
+// {sim, x0, z0, class}, and interior items for the kernel `interior`
+// (rtm::Interior) in segments of at most seg_rows rows.
+void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, const rtm_params &p,
+                 int rz, int interior, int seg_rows);
 inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
 {
     std::vector<int4> none;
-    append_work(out, none, sim, p, rz, 0);
+    append_work(out, none, sim, p, rz, rtm::IN_TILES, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -330,23 +504,29 @@ inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, i
 struct Exec {
     int real = RTM_REAL_F32;
     int rz = 4;                         // rows per thread (tile height 4*rz)
-    int cap_sims = 0, cap_tiles = 0, cap_march = 0;
-    int nsims = 0, ntiles = 0, nmarch = 0;
+    int interior = rtm::IN_TILES;       // kernel of the interior items
+    int cap_sims = 0, cap_tiles = 0, cap_inner = 0;
+    int nsims = 0, ntiles = 0, ninner = 0;
     void *d_desc = nullptr;             // SimDesc<R>[cap_sims]
     int *d_n0 = nullptr;                // [cap_sims]
     int4 *d_tiles = nullptr;            // [cap_tiles]
     int *d_ntiles = nullptr;
-    int4 *d_march = nullptr;            // [cap_march]
-    int *d_nmarch = nullptr;
+    int4 *d_inner = nullptr;            // [cap_inner] march or column items
+    int *d_ninner = nullptr;
     std::vector<int> h_n0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int gblock = 0;
+    // the strip tiles run on `side` beside the interior kernel (they write
+    // disjoint points); fork/join through two events, also under capture
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
-    int init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_march = 0);
+    int init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_inner = 0,
+             int interior_kernel = rtm::IN_TILES);
     void release();
     int set_tiles(const std::vector<int4> &tiles, hipStream_t s);
-    int set_march(const std::vector<int4> &items, hipStream_t s);
+    int set_inner(const std::vector<int4> &items, hipStream_t s);
     int upload_descs(const void *descs, int n, hipStream_t s);   // SimDesc<R>[n]
     int set_n0(const std::vector<int> &n0, hipStream_t s);
     // `steps` launches from the current n0 (eager, or in graph blocks of

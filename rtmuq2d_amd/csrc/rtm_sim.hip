@@ -289,48 +289,32 @@ SimDesc<R> SimBuffers::desc() const
 template SimDesc<float> SimBuffers::desc<float>() const;
 template SimDesc<double> SimBuffers::desc<double>() const;
 
-void append_work(std::vector<int4> &tiles, std::vector<int4> &march, int sim, const rtm_params &p, int rz,
-                 int march_rows)
+void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, const rtm_params &p, int rz,
+                 int interior, int seg_rows)
 {
-    const int X = p.thickness_left + p.nx + p.thickness_right;
-    const int Z = p.thickness_up + p.nz + p.thickness_down;
-    const int TZ = 4 * rz;
-    SimDesc<float> d{};
-    d.X = X; d.Z = Z;
-    d.TL = p.thickness_left; d.TR = p.thickness_right; d.TU = p.thickness_up; d.TD = p.thickness_down;
-    const int seg = march_rows > 0 ? (march_rows + TZ - 1) / TZ * TZ : 0;
-    // z fastest: vertically adjacent tiles (which share halo rows) are
-    // neighbours in the list and so land on the same XCD
-    for (int x0 = rtm::kHalf; x0 < X - rtm::kHalf; x0 += 64) {
-        int zA = -1, zB = -1;   // run of interior tiles in this column
-        for (int z0 = rtm::kHalf; z0 < Z - rtm::kHalf; z0 += TZ) {
-            const int cls = rtm::tile_class(d, x0, z0, TZ);
-            if (seg && cls == 0) {
-                // an interior tile: all 64 x TZ points lie in the stepped
-                // region (the strips are at least 9 points thick)
-                if (zA < 0) zA = z0;
-                zB = z0 + TZ;
-            } else {
-                tiles.push_back(make_int4(sim, x0, z0, cls));
-            }
-        }
-        for (int z = zA; zA >= 0 && z < zB; z += seg) {
-            const int e = z + seg < zB ? z + seg : zB;
-            march.push_back(make_int4(sim, x0, z, e));
-        }
-    }
+    std::vector<rtm::WorkItem> t, in;
+    rtm::build_work(t, in, sim, p.thickness_left + p.nx + p.thickness_right,
+                    p.thickness_up + p.nz + p.thickness_down, p.thickness_left, p.thickness_right, p.thickness_up,
+                    p.thickness_down, rz, interior, seg_rows);
+    for (const auto &w : t) tiles.push_back(make_int4(w.a, w.b, w.c, w.d));
+    for (const auto &w : in) inner.push_back(make_int4(w.a, w.b, w.c, w.d));
 }
 
 // ---------------------------------------------------------------------------
 // executor
 // ---------------------------------------------------------------------------
 
-int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_march)
+int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_inner, int interior_kernel)
 {
     real = real_type;
     rz = rows_per_thread;
+    interior = interior_kernel;
     if (rz != 1 && rz != 2 && rz != 4 && rz != 8) {
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
+        return -1;
+    }
+    if (interior != rtm::IN_TILES && interior != rtm::IN_MARCH && interior != rtm::IN_COLUMNS) {
+        rtm_set_error("interior kernel must be 0 (tiles), 1 (march) or 2 (columns) (got %d)", interior);
         return -1;
     }
     cap_sims = max_sims > 0 ? max_sims : 1;
@@ -343,12 +327,15 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
     HIP_TRY(hipMemset(d_desc, 0, dsz * cap_sims));
     HIP_TRY(hipMemset(d_n0, 0, sizeof(int) * cap_sims));
     HIP_TRY(hipMemset(d_ntiles, 0, sizeof(int)));
-    cap_march = max_march > 0 ? max_march : 1;
-    HIP_TRY(hipMalloc((void **)&d_march, sizeof(int4) * cap_march));
-    HIP_TRY(hipMalloc((void **)&d_nmarch, sizeof(int)));
-    HIP_TRY(hipMemset(d_nmarch, 0, sizeof(int)));
-    nmarch = 0;
+    cap_inner = max_inner > 0 ? max_inner : 1;
+    HIP_TRY(hipMalloc((void **)&d_inner, sizeof(int4) * cap_inner));
+    HIP_TRY(hipMalloc((void **)&d_ninner, sizeof(int)));
+    HIP_TRY(hipMemset(d_ninner, 0, sizeof(int)));
+    ninner = 0;
     h_n0.assign(cap_sims, 0);
+    HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     return 0;
 }
 
@@ -365,23 +352,31 @@ int Exec::drop_graph()
 void Exec::release()
 {
     drop_graph();
-    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles, d_march, d_nmarch};
+    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles, d_inner, d_ninner};
     for (void *p : all)
         if (p) (void)hipFree(p);
     d_desc = nullptr; d_n0 = nullptr; d_tiles = nullptr; d_ntiles = nullptr;
-    d_march = nullptr; d_nmarch = nullptr;
-    nmarch = ntiles = 0;
+    d_inner = nullptr; d_ninner = nullptr;
+    ninner = ntiles = 0;
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    side = nullptr;
+    ev_fork = ev_join = nullptr;
 }
 
-int Exec::set_march(const std::vector<int4> &items, hipStream_t s)
+int Exec::set_inner(const std::vector<int4> &items, hipStream_t s)
 {
-    if ((int)items.size() > cap_march) {
-        rtm_set_error("march list of %zu exceeds the executor capacity %d", items.size(), cap_march);
+    if ((int)items.size() > cap_inner) {
+        rtm_set_error("interior list of %zu exceeds the executor capacity %d", items.size(), cap_inner);
         return -1;
     }
-    nmarch = (int)items.size();
-    if (nmarch) HIP_TRY(hipMemcpyAsync(d_march, items.data(), sizeof(int4) * nmarch, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_nmarch, &nmarch, sizeof(int), hipMemcpyHostToDevice, s));
+    ninner = (int)items.size();
+    if (ninner) HIP_TRY(hipMemcpyAsync(d_inner, items.data(), sizeof(int4) * ninner, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_ninner, &ninner, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     drop_graph();
     return 0;
@@ -424,22 +419,39 @@ template <typename R>
 static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
 {
     const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
-    if (e.nmarch) {
-        const int mgrid = 8 * ((e.cap_march + 7) / 8);
+    // interior and strip tiles write disjoint points and read only u^n and
+    // their own points: the tiles go to the side stream, joined at the end
+    const bool fork = e.ninner && e.ntiles && e.side;
+    hipStream_t ts = s;
+    if (fork) {
+        (void)hipEventRecord(e.ev_fork, s);
+        (void)hipStreamWaitEvent(e.side, e.ev_fork, 0);
+        ts = e.side;
+    }
+    if (e.ninner && e.interior == rtm::IN_COLUMNS) {
+        // four items per workgroup
+        const int cgrid = 8 * (((e.cap_inner + 3) / 4 + 7) / 8);
+        hipLaunchKernelGGL((k_columns<R>), dim3(cgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block);
+    } else if (e.ninner) {
+        const int mgrid = 8 * ((e.cap_inner + 7) / 8);
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_march<R, 1>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_march<R, 2>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_march<R, 8>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
-        default: hipLaunchKernelGGL((k_march<R, 4>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_march, e.d_nmarch, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_march<R, 1>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_march<R, 2>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_march<R, 8>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
+        default: hipLaunchKernelGGL((k_march<R, 4>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
         }
     }
     if (!e.ntiles) return;
     const int grid = 8 * ((e.cap_tiles + 7) / 8);
     switch (e.rz) {
-    case 1: hipLaunchKernelGGL((k_tiles<R, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    case 2: hipLaunchKernelGGL((k_tiles<R, 2>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    case 8: hipLaunchKernelGGL((k_tiles<R, 8>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    default: hipLaunchKernelGGL((k_tiles<R, 4>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    case 1: hipLaunchKernelGGL((k_tiles<R, 1>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    case 2: hipLaunchKernelGGL((k_tiles<R, 2>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    case 8: hipLaunchKernelGGL((k_tiles<R, 8>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    default: hipLaunchKernelGGL((k_tiles<R, 4>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    }
+    if (fork) {
+        (void)hipEventRecord(e.ev_join, e.side);
+        (void)hipStreamWaitEvent(s, e.ev_join, 0);
     }
 }
 

@@ -3,10 +3,12 @@
 // without a GPU.  Built by tests/test_cpu_harness.py with
 // g++ -O2 -ffp-contract=off; test infrastructure only.
 #include <cstddef>
+#include <vector>
 
 #include "rtm_step.h"
 #include "rtm_tile.h"
 #include "rtm_march.h"
+#include "rtm_work.h"
 
 template <typename R>
 static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const R *ax, const R *bx,
@@ -76,11 +78,51 @@ static void run_march(rtm::MarchLds<R, RZ> &L, rtm::MarchRegs<R, RZ> *cur, rtm::
     }
 }
 
-// One step of one simulation: tiles where a CPML strip is touched (all tiles
-// when march_rows == 0) and marched interior segments otherwise, split the
-// way the library's work-list builder splits them.
+// Emulation of k_columns for one item: every row, all 64 lanes, with the
+// DPP lane shifts done as array shifts (lane 0 / 63 take the halo quad).
+template <typename R>
+static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V, int packed, int x0, int zA, int zB)
+{
+    using P2 = rtm::Pair<R>;
+    const int P = S.P;
+    const int half = rtm::col_half(packed);
+    const int srow = rtm::col_src_row(S, V);
+    auto at = [&](const R *f, int k, int x) { return f[(size_t)k * P + x]; };
+    for (int k = zA; k < zB; k++) {
+        P2 c[64], L1[64], L2[64], R1[64], R2[64];
+        rtm::Quad<R> H[64];
+        int xl[64], xc[64];
+        for (int l = 0; l < 64; l++) {
+            xl[l] = x0 + 2 * l;
+            xc[l] = xl[l] < P - 2 ? xl[l] : P - 2;
+            c[l] = {at(V.u, k, xc[l]), at(V.u, k, xc[l] + 1)};
+            const int hx = rtm::clampi(l == 63 ? x0 + 128 : x0 - 4, 0, P - 4);
+            H[l] = {at(V.u, k, hx), at(V.u, k, hx + 1), at(V.u, k, hx + 2), at(V.u, k, hx + 3)};
+        }
+        for (int l = 0; l < 64; l++) L1[l] = l > 0 ? c[l - 1] : P2{H[0].z, H[0].w};
+        for (int l = 0; l < 64; l++) L2[l] = l > 0 ? L1[l - 1] : P2{H[0].x, H[0].y};
+        for (int l = 0; l < 64; l++) R1[l] = l < 63 ? c[l + 1] : P2{H[63].x, H[63].y};
+        for (int l = 0; l < 64; l++) R2[l] = l < 63 ? R1[l + 1] : P2{H[63].z, H[63].w};
+        for (int l = 0; l < 64; l++) {
+            P2 w[9];
+            for (int r = 0; r < 9; r++) w[r] = {at(V.u, k - 4 + r, xc[l]), at(V.u, k - 4 + r, xc[l] + 1)};
+            const P2 Cp = {at(S.C, k, xc[l]), at(S.C, k, xc[l] + 1)};
+            const P2 Op = {at(V.U, k, xc[l]), at(V.U, k, xc[l] + 1)};
+            P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, Op);
+            const bool ok = l < (half ? 32 : 64);
+            rtm::col_extras(S, V, k, xl[l], ok, srow, U, Cp);
+            if (ok) {
+                V.U[(size_t)k * P + xl[l]] = U.a;
+                V.U[(size_t)k * P + xl[l] + 1] = U.b;
+            }
+        }
+    }
+}
+
+// One step of one simulation over the work items of the library's builder
+// (rtm::build_work): tiles, plus marched or column interior items.
 template <typename R, int RZ>
-static void run_tiles(const rtm::SimDesc<R> &S, int n, int march_rows)
+static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
 {
     const rtm::StepView<R> V = rtm::step_view(S, n);
     if (!V.active) return;
@@ -89,27 +131,20 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n, int march_rows)
     auto *M = new rtm::MarchLds<R, RZ>();
     auto *cur = new rtm::MarchRegs<R, RZ>[256];
     auto *nxt = new rtm::MarchRegs<R, RZ>[256];
-    const int TZ = 4 * RZ;
-    const int seg = march_rows > 0 ? (march_rows + TZ - 1) / TZ * TZ : 0;
-    for (int x0 = rtm::kHalf; x0 < S.X - rtm::kHalf; x0 += 64) {
-        int zA = -1, zB = -1;
-        for (int z0 = rtm::kHalf; z0 < S.Z - rtm::kHalf; z0 += TZ) {
-            const int cls = rtm::tile_class(S, x0, z0, TZ);
-            if (seg && cls == 0) {
-                if (zA < 0) zA = z0;
-                zB = z0 + TZ;
-                continue;
-            }
-            // the LDS holds stale data from the previous tile, as on the GPU
-            switch (cls) {
-            case 0: run_tile<R, RZ, 0>(*L, G, S, V, x0, z0); break;
-            case 1: run_tile<R, RZ, 1>(*L, G, S, V, x0, z0); break;
-            case 2: run_tile<R, RZ, 2>(*L, G, S, V, x0, z0); break;
-            default: run_tile<R, RZ, 3>(*L, G, S, V, x0, z0); break;
-            }
+    std::vector<rtm::WorkItem> tiles, inner;
+    rtm::build_work(tiles, inner, 0, S.X, S.Z, S.TL, S.TR, S.TU, S.TD, RZ, interior, seg_rows);
+    for (const auto &t : tiles) {
+        // the LDS holds stale data from the previous tile, as on the GPU
+        switch (t.d) {
+        case 0: run_tile<R, RZ, 0>(*L, G, S, V, t.b, t.c); break;
+        case 1: run_tile<R, RZ, 1>(*L, G, S, V, t.b, t.c); break;
+        case 2: run_tile<R, RZ, 2>(*L, G, S, V, t.b, t.c); break;
+        default: run_tile<R, RZ, 3>(*L, G, S, V, t.b, t.c); break;
         }
-        for (int z = zA; zA >= 0 && z < zB; z += seg)
-            run_march<R, RZ>(*M, cur, nxt, S, V, x0, z, z + seg < zB ? z + seg : zB);
+    }
+    for (const auto &it : inner) {
+        if (interior == rtm::IN_MARCH) run_march<R, RZ>(*M, cur, nxt, S, V, it.b, it.c, it.d);
+        else run_column_item<R>(S, V, it.a, it.b, it.c, it.d);
     }
     delete[] nxt;
     delete[] cur;
@@ -127,7 +162,7 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.TL = g[3]; S.TR = g[4]; S.TU = g[5]; S.TD = g[6];
     S.nx = g[7]; S.nz = g[8]; S.nt = g[9]; S.TS = g[10]; S.nw = g[11]; S.nrec = g[12];
     S.phase = g[13]; S.flags = g[14]; S.src_j = g[15]; S.rec_row = g[16];
-    const int n = g[17], rz = g[18], march_rows = g[19];
+    const int n = g[17], rz = g[18], seg_rows = g[19], interior = g[20];
     S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
     S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
     S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];
@@ -136,10 +171,10 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.snaps = ptr[13]; S.I = ptr[14]; S.Is = ptr[15]; S.Ic = ptr[16]; S.Isc = ptr[17];
     S.recmap = recmap; S.wav = wav; S.seis_t = seis_t; S.seis_rec = seis_rec;
     switch (rz) {
-    case 1: run_tiles<R, 1>(S, n, march_rows); break;
-    case 2: run_tiles<R, 2>(S, n, march_rows); break;
-    case 8: run_tiles<R, 8>(S, n, march_rows); break;
-    default: run_tiles<R, 4>(S, n, march_rows); break;
+    case 1: run_tiles<R, 1>(S, n, interior, seg_rows); break;
+    case 2: run_tiles<R, 2>(S, n, interior, seg_rows); break;
+    case 8: run_tiles<R, 8>(S, n, interior, seg_rows); break;
+    default: run_tiles<R, 4>(S, n, interior, seg_rows); break;
     }
 }
 

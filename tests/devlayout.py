@@ -132,7 +132,7 @@ class DevSim:
     # ---- the tiled kernel (rtm_tile.h), emulated phase by phase ----
 
     def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False,
-                    march=0):
+                    interior=0, seg=0):
         op = self.op
         self.reset()
         nint = op.nx * op.nz
@@ -153,19 +153,20 @@ class DevSim:
         for n in range(op.nt):
             g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx,
                           op.nz, op.nt, TS, op.n_wavelet, op.n_receivers, phase, flags, src_j,
-                          self.rec_row, n, rz, march], np.int32)
+                          self.rec_row, n, rz, seg, interior], np.int32)
             fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
         self.cur = op.nt & 1
         return snaps, I, Is, seis_rec
 
-    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, march=0):
+    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0):
         op = self.op
         src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
         flags = 4 | (2 if record else 0)
-        snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record, march=march)
+        snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record,
+                                            interior=interior, seg=seg)
         return snaps, np.ascontiguousarray(rec.T)
 
-    def backward_tiled(self, seis, snaps, rz=4, kahan=False, march=0):
+    def backward_tiled(self, seis, snaps, rz=4, kahan=False, interior=0, seg=0):
         flags = 8 | (1 if kahan else 0)
-        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, march=march)
+        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, interior=interior, seg=seg)
         return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)

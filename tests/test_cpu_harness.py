@@ -43,25 +43,30 @@ def test_forward_backward_bitexact(hx, name, dtype):
 
 @pytest.mark.parametrize("name", ["tiny", "asym", "wide"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("rz,march", [(1, 0), (2, 0), (4, 0), (8, 0), (1, 8), (2, 16), (4, 48), (8, 1000)])
-def test_tiled_kernel_bitexact(hx, name, dtype, rz, march):
-    """The LDS-tiled kernel body (rtm_tile.h), its phases run in sequence for
-    all threads of each tile, against the oracle: fields, snapshots,
-    recorded traces and images."""
+@pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (1, 1, 8), (2, 1, 16),
+                                             (4, 1, 48), (8, 1, 1000), (4, 2, 0), (2, 2, 12), (1, 2, 4),
+                                             (8, 2, 32)])
+def test_tiled_kernel_bitexact(hx, name, dtype, rz, interior, seg):
+    """The step kernels' bodies run on the CPU against the oracle: the
+    LDS-tiled kernel (rtm_tile.h, its phases in sequence for all threads of
+    each tile) everywhere or on the CPML strips only, with the interior
+    marched (rtm_march.h) or stepped by lane-shift columns (rtm_column.h,
+    lane exchanges as array shifts).  Fields, snapshots, recorded traces and
+    images."""
     cfg, g = _case(name)
     op = common.orc_params(cfg)
     vpad = O.extend(op, g["models"][0])
     sx, sz = common.shot_positions(cfg)[0]
     S, drec, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=dtype, record=True)
     ds = DL.DevSim(hx, op, vpad, dtype)
-    S2, rec = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, record=True, march=march)
+    S2, rec = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, record=True, interior=interior, seg=seg)
     a, b = ds.last_fields()
     assert np.array_equal(a, lu) and np.array_equal(b, lU)
     assert np.array_equal(np.ascontiguousarray(S2.transpose(0, 2, 1)), S)
     assert np.array_equal(rec, drec.astype(np.float32))
     d = g["seismograms"][0]
     I, Is, lu2, lU2 = O.backward(op, vpad, d, S)
-    I2, Is2 = ds.backward_tiled(d, S2, rz=rz, march=march)
+    I2, Is2 = ds.backward_tiled(d, S2, rz=rz, interior=interior, seg=seg)
     a, b = ds.last_fields()
     assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
@@ -69,11 +74,12 @@ def test_tiled_kernel_bitexact(hx, name, dtype, rz, march):
 
 def _case(name):
     """A golden configuration, or "wide": a grid large enough that the
-    interior spans several tile columns and marched chunks (receivers on an
-    interior row, source inside, 3 imaging steps)."""
+    interior spans three tile columns (a 128-column item and a 64-column
+    one) and several marched chunks (receivers on an interior row, source
+    inside, 3 imaging steps)."""
     if name != "wide":
         return common.REF_CONFIGS[name], np.load(os.path.join(common.GOLDEN, f"{name}.npz"))
-    cfg = dict(Nx=230, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
+    cfg = dict(Nx=250, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
                spacingGrid=10.0, sampling=0.001, numberOfTimeStep=60, sampligWavelet=60,
                firstLocationShot=100, depthShot=30, spacingShot=1, numberOfShots=1,
                firstLocationReceiver=3, depthReceiver=35, spacingReceiver=3, numberOfReceivers=100,

@@ -88,9 +88,10 @@ def test_tilings_and_graph_blocks_bitexact(gpu_available, rz, block):
 
 
 def _wide_case():
-    """A grid whose interior spans several tile columns and marched chunks,
-    with the source and the receivers in the interior."""
-    cfg = dict(Nx=230, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
+    """A grid whose interior spans three tile columns (one 128-column and one
+    64-column item) and several marched chunks, with the source and the
+    receivers in the interior."""
+    cfg = dict(Nx=250, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
                spacingGrid=10.0, sampling=0.001, numberOfTimeStep=120, sampligWavelet=120,
                firstLocationShot=100, depthShot=30, spacingShot=1, numberOfShots=1,
                firstLocationReceiver=3, depthReceiver=35, spacingReceiver=3, numberOfReceivers=100,
@@ -103,12 +104,14 @@ def _wide_case():
     return cfg, model, wav, seis
 
 
-@pytest.mark.parametrize("rz,march", [(1, 8), (2, 16), (4, 48), (4, 1000), (8, 64)])
+@pytest.mark.parametrize("rz,interior,seg", [(1, 1, 8), (2, 1, 16), (4, 1, 48), (4, 1, 1000), (8, 1, 64),
+                                             (4, 2, 0), (2, 2, 12), (1, 2, 4), (8, 2, 32), (4, 2, 16)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_marched_interior_bitexact(gpu_available, rz, march, dtype):
-    """Strip tiles plus the streaming interior kernel: fields, recorded
-    traces and images bit-identical to the oracle."""
-    say(f"[gpu] march rz={rz} rows={march} {np.dtype(dtype).name}")
+def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
+    """Strip tiles plus an interior kernel (LDS-ring march or lane-shift
+    columns, including a 64-column item): fields, recorded traces and images
+    bit-identical to the oracle."""
+    say(f"[gpu] interior={interior} rz={rz} rows={seg} {np.dtype(dtype).name}")
     cfg, model, wav, seis = _wide_case()
     op = common.orc_params(cfg, time_slices=10)
     p = common.rtm_params(cfg, time_slices=10)
@@ -118,7 +121,7 @@ def test_marched_interior_bitexact(gpu_available, rz, march, dtype):
     I, Is, lu2, lU2 = O.backward(op, vpad, seis, S)
     f64 = dtype == np.float64
     with R.Plan(p, R.REAL_F64 if f64 else R.REAL_F32) as plan:
-        plan.set_tiling(rz, 100, march)
+        plan.set_tiling(rz, 100, interior, seg)
         plan.set_velocity(vpad)
         plan.set_wavelet(wav)
         plan.forward(sx, sz, record=True)
@@ -135,10 +138,11 @@ def test_marched_interior_bitexact(gpu_available, rz, march, dtype):
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
-def test_batch_marched_bitexact(gpu_available):
-    """A batch with the streaming interior kernel: every realization's image
+@pytest.mark.parametrize("interior,seg", [(1, 32), (2, 16)])
+def test_batch_marched_bitexact(gpu_available, interior, seg):
+    """A batch with a streaming interior kernel: every realization's image
     equals the oracle's, in the batch as alone."""
-    say("[gpu] batch with marched interior")
+    say(f"[gpu] batch with interior kernel {interior}")
     cfg, model, wav, seis = _wide_case()
     op = common.orc_params(cfg, time_slices=10)
     p = common.rtm_params(cfg, time_slices=10)
@@ -146,7 +150,7 @@ def test_batch_marched_bitexact(gpu_available):
     vpads = [O.extend(op, model * np.float32(1 + 0.03 * k)) for k in range(3)]
     refs = [O.rtm_shot(op, sx, sz, v, wav, seis) for v in vpads]
     with R.Batch(p, len(vpads)) as b:
-        b.set_tiling(4, 100, 32)
+        b.set_tiling(4, 100, interior, seg)
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
         b.set_wavelet(wav)
