@@ -45,6 +45,30 @@ BATCH_SHAPES = [(4, 100, 0, 0), (4, 100, 1, 1000), (4, 100, 2, 0), (4, 100, 2, 6
                 (2, 100, 2, 32), (8, 100, 2, 32)]
 
 
+# L2-to-fabric bytes per launch of the c1 plain step kernel (tiles 64x8),
+# mean of a forward and a backward step at TS=1: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes over `bench.py --profile`, each scaled by what the
+# counter reads for a 1 GiB copy in the same run (x2.0 and x1.0).
+C1_TRAFFIC_BYTES = 24601475
+
+
+def full_step_bytes(p, ts):
+    """Algorithmic bytes of one step, mean of forward and backward: 16 B per
+    padded point, +16 B per active CPML point (psi, csi read and written),
+    +4 B per interior point stored (forward) or +20 B per interior point
+    imaged (snapshot read, I and Is read-modify-write), every ts steps, and
+    +4 B per receiver trace sample (backward)."""
+    X, Z = p.X, p.Z
+    TL, TR, TU, TD = p.thickness_left, p.thickness_right, p.thickness_up, p.thickness_down
+    xs = (TL - 8) + (TR - 8)
+    zs = (TU - 8) + (TD - 8)
+    strip = xs * (Z - 8) + zs * (X - 8)        # corner points carry both x and z variables
+    nint = p.nx * p.nz
+    fwd = 16 * X * Z + 16 * strip + 4 * nint / ts
+    bwd = 16 * X * Z + 16 * strip + 20 * nint / ts + 4 * p.number_of_receivers
+    return int((fwd + bwd) / 2)
+
+
 def tiling_dict(shape):
     rz, block, interior, seg = shape
     return {"rows_per_thread": rz, "graph_block": block,
@@ -197,6 +221,7 @@ def main() -> None:
     updates = 2.0 * nt * X * Z
     step_ms = (np.mean(fwd_ms) + np.mean(bwd_ms)) / 2.0
     bytes_per_launch = BYTES_PER_UPDATE * X * Z
+    full_step = full_step_bytes(p, ts=1)
     achieved = bytes_per_launch / (step_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -227,7 +252,10 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": C1_TRAFFIC_BYTES,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "full_step_bytes_per_launch": full_step,
+            "full_step_achieved": round(full_step / (step_ms * 1e-3) / 1e9, 1),
         },
         "image_norm": float(np.linalg.norm(I.astype(np.float64))),
     }
