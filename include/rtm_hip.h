@@ -144,15 +144,14 @@ size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan 
 
 /* Which kernel steps the interior (the points outside every CPML strip):
  * TILES   the LDS tile kernel, as everywhere else;
- * MARCH   workgroups march down 64-column strips with an LDS row ring;
  * COLUMNS wavefronts march down 128-column strips with a register window
- *         and lane shifts (no LDS, no barrier). */
-enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_MARCH = 1, RTM_INTERIOR_COLUMNS = 2 };
+ *         and lane shifts (no LDS, no barrier), beside the strip tiles. */
+enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1 };
 
 /* Kernel shape: rows_per_thread in {1,2,4,8} gives tiles of 64 x
  * 4*rows_per_thread points; graph_block = time steps per captured hipGraph
  * block (<= 1: one eager launch per step); interior = enum rtm_interior;
- * seg_rows = rows per marched segment (MARCH, COLUMNS; <= 0: whole strips).
+ * seg_rows = rows per column segment (COLUMNS; <= 0: whole strips).
  * Tiles stay wherever a CPML strip is touched.  Defaults: 4, 100, TILES, 0. */
 int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int interior, int seg_rows);
 

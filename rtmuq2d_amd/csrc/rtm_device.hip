@@ -53,7 +53,7 @@ struct rtm_plan {
     int rz = 4;                     // tile rows per thread
     int graph_block = 100;          // steps per captured graph block (<= 1: eager)
     int interior = rtm::IN_TILES;   // interior kernel (rtm::Interior)
-    int seg_rows = 0;               // rows per marched interior segment
+    int seg_rows = 0;               // rows per interior column segment
     int last_nt = 0;                // steps of the last pass
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;

@@ -16,7 +16,6 @@
 #include "rtm_internal.h"
 #include "rtm_step.h"
 #include "rtm_tile.h"
-#include "rtm_march.h"
 #include "rtm_column.h"
 #include "rtm_work.h"
 
@@ -89,49 +88,6 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     case 1: run_tile<R, RZ, 1>(L, S, V, tile.y, tile.z, tx, ty); break;
     case 2: run_tile<R, RZ, 2>(L, S, V, tile.y, tile.z, tx, ty); break;
     default: run_tile<R, RZ, 3>(L, S, V, tile.y, tile.z, tx, ty); break;
-    }
-}
-
-// The streaming interior kernel: one workgroup marches down a 64-column
-// interior strip {sim, x0, zA, zB} (rtm_march.h), prefetching the next
-// chunk while computing the current one.
-template <typename R, int RZ>
-__global__ void __launch_bounds__(256) k_march(const SimDesc<R> *__restrict__ sims,
-                                               const int *__restrict__ n0,
-                                               const int4 *__restrict__ items,
-                                               const int *__restrict__ nitems_p, int s)
-{
-    constexpr int TZ = 4 * RZ;
-    __shared__ rtm::MarchLds<R, RZ> L;
-    const int nitems = *nitems_p;
-    const int per = (nitems + 7) >> 3;
-    const int b = blockIdx.x;
-    const int t = (b & 7) * per + (b >> 3);
-    if ((b >> 3) >= per || t >= nitems) return;
-    const int4 it = items[t];
-    const SimDesc<R> &S = sims[it.x];
-    const rtm::StepView<R> V = rtm::step_view(S, n0[it.x] + s);
-    if (!V.active) return;
-    const int tx = threadIdx.x & 63;
-    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int x0 = it.y, zA = it.z, zB = it.w;
-    rtm::MarchRegs<R, RZ> cur, nxt;
-    rtm::march_fetch_rows<R, RZ, RZ + 2>(cur, S, V, x0, zA - 4, tx, ty);
-    rtm::march_fetch_points(cur, S, V, x0, zA, tx, ty);
-    rtm::march_stage_rows<R, RZ, RZ + 2>(L, cur, zA - 4, tx, ty);
-    __syncthreads();
-    for (int zc = zA; zc < zB; zc += TZ) {
-        const bool more = zc + TZ < zB;
-        if (more) {
-            rtm::march_fetch_rows<R, RZ, RZ>(nxt, S, V, x0, zc + TZ + 4, tx, ty);
-            rtm::march_fetch_points(nxt, S, V, x0, zc + TZ, tx, ty);
-        }
-        rtm::march_points(L, cur, S, V, x0, zc, tx, ty);
-        if (more) {
-            rtm::march_stage_rows<R, RZ, RZ>(L, nxt, zc + TZ + 4, tx, ty);
-            rtm::march_copy(cur, nxt);
-        }
-        __syncthreads();
     }
 }
 
@@ -511,7 +467,7 @@ struct Exec {
     int *d_n0 = nullptr;                // [cap_sims]
     int4 *d_tiles = nullptr;            // [cap_tiles]
     int *d_ntiles = nullptr;
-    int4 *d_inner = nullptr;            // [cap_inner] march or column items
+    int4 *d_inner = nullptr;            // [cap_inner] column items
     int *d_ninner = nullptr;
     std::vector<int> h_n0;
     hipGraphExec_t gexec = nullptr;

@@ -313,8 +313,8 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
         return -1;
     }
-    if (interior != rtm::IN_TILES && interior != rtm::IN_MARCH && interior != rtm::IN_COLUMNS) {
-        rtm_set_error("interior kernel must be 0 (tiles), 1 (march) or 2 (columns) (got %d)", interior);
+    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS) {
+        rtm_set_error("interior kernel must be 0 (tiles) or 1 (columns) (got %d)", interior);
         return -1;
     }
     cap_sims = max_sims > 0 ? max_sims : 1;
@@ -432,14 +432,6 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
         // four items per workgroup
         const int cgrid = 8 * (((e.cap_inner + 3) / 4 + 7) / 8);
         hipLaunchKernelGGL((k_columns<R>), dim3(cgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block);
-    } else if (e.ninner) {
-        const int mgrid = 8 * ((e.cap_inner + 7) / 8);
-        switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_march<R, 1>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_march<R, 2>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_march<R, 8>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
-        default: hipLaunchKernelGGL((k_march<R, 4>), dim3(mgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block); break;
-        }
     }
     if (!e.ntiles) return;
     const int grid = 8 * ((e.cap_tiles + 7) / 8);

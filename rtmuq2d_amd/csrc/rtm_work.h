@@ -3,14 +3,10 @@
  * 4 <= k < Z-4 is split into work items for the step kernels.  Shared by
  * the library and the CPU test harness so both walk the same items.
  *
- * Tiles are 64 x TZ (TZ = 4*rz) from (4, 4), z fastest within a tile column
- * so vertically adjacent tiles, which share halo rows, are neighbours in the
- * list (and land on the same XCD).  Each tile has a class: which CPML strips
- * its points touch.  The interior (class-0 tiles, whose points need no CPML
- * term) is handled by one of three kernels:
+ * Tiles are 64 x TZ (TZ = 4*rz) from (4, 4).  Each tile has a class: which
+ * CPML strips its points touch.  The interior (class-0 tiles, whose points
+ * need no CPML term) is handled by one of two kernels:
  *   IN_TILES    every tile goes to the tile kernel;
- *   IN_MARCH    class-0 runs of a tile column become march items
- *               {sim, x0, zA, zB} of at most seg_rows rows (rtm_march.h);
  *   IN_COLUMNS  class-0 runs of two adjacent tile columns with the same
  *               rows become one 128-column item, a lone column a 64-column
  *               item, in segments of at most seg_rows rows (rtm_column.h).
@@ -24,7 +20,7 @@
 
 namespace rtm {
 
-enum Interior : int { IN_TILES = 0, IN_MARCH = 1, IN_COLUMNS = 2 };
+enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1 };
 
 struct WorkItem {
     int a, b, c, d;   /* tile {sim, x0, z0, class}; interior {sim(+half), x0, zA, zB} */
@@ -37,7 +33,6 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
     SimDesc<float> d{};
     d.X = X; d.Z = Z; d.TL = TL; d.TR = TR; d.TU = TU; d.TD = TD;
     int seg = 0;
-    if (interior == IN_MARCH && seg_rows > 0) seg = (seg_rows + TZ - 1) / TZ * TZ;
     if (interior == IN_COLUMNS) seg = seg_rows > 0 ? (seg_rows + 3) / 4 * 4 : 1 << 30;
     /* Items are listed x fastest (a row of tiles, then the next row), so the
      * workgroups an XCD runs at the same time cover whole grid rows: every
@@ -70,12 +65,12 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
     for (size_t t = 0; t < runs.size(); t++) {
         const Run &r = runs[t];
         if (r.zA < 0) continue;
-        const bool pair = interior == IN_COLUMNS && t + 1 < runs.size() && runs[t + 1].zA == r.zA
+        const bool pair = t + 1 < runs.size() && runs[t + 1].zA == r.zA
                           && runs[t + 1].zB == r.zB;
         cols.push_back({r.x0, r.zA, r.zB, pair ? 0 : 1});
         if (pair) t++;
     }
-    if (interior != IN_MARCH && interior != IN_COLUMNS) return;
+    if (interior != IN_COLUMNS) return;
     int zmax = 0;
     for (const Col &c : cols) zmax = c.zB - c.zA > zmax ? c.zB - c.zA : zmax;
     for (int off = 0; off < zmax; off += seg)      /* segment-major, x fastest */
@@ -83,7 +78,7 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
             const int z = c.zA + off;
             if (z >= c.zB) continue;
             const int e = z + seg < c.zB ? z + seg : c.zB;
-            inner.push_back({interior == IN_COLUMNS ? col_pack(sim, c.half) : sim, c.x0, z, e});
+            inner.push_back({col_pack(sim, c.half), c.x0, z, e});
         }
 }
 

@@ -7,7 +7,6 @@
 
 #include "rtm_step.h"
 #include "rtm_tile.h"
-#include "rtm_march.h"
 #include "rtm_work.h"
 
 template <typename R>
@@ -49,33 +48,6 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
             for (int tx = 0; tx < 64; tx++) rtm::tile_psi<R, RZ, CLS>(L, G[ty * 64 + tx], S, x0, z0, tx, ty);
     for (int ty = 0; ty < 4; ty++)
         for (int tx = 0; tx < 64; tx++) rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
-}
-
-// Emulation of k_march for one work item {x0, zA, zB}.
-template <typename R, int RZ>
-static void run_march(rtm::MarchLds<R, RZ> &L, rtm::MarchRegs<R, RZ> *cur, rtm::MarchRegs<R, RZ> *nxt,
-                      const rtm::SimDesc<R> &S, const rtm::StepView<R> &V, int x0, int zA, int zB)
-{
-    const int TZ = 4 * RZ;
-    for (int th = 0; th < 256; th++) {
-        rtm::march_fetch_rows<R, RZ, RZ + 2>(cur[th], S, V, x0, zA - 4, th & 63, th >> 6);
-        rtm::march_fetch_points(cur[th], S, V, x0, zA, th & 63, th >> 6);
-    }
-    for (int th = 0; th < 256; th++) rtm::march_stage_rows<R, RZ, RZ + 2>(L, cur[th], zA - 4, th & 63, th >> 6);
-    for (int zc = zA; zc < zB; zc += TZ) {
-        const bool more = zc + TZ < zB;
-        if (more)
-            for (int th = 0; th < 256; th++) {
-                rtm::march_fetch_rows<R, RZ, RZ>(nxt[th], S, V, x0, zc + TZ + 4, th & 63, th >> 6);
-                rtm::march_fetch_points(nxt[th], S, V, x0, zc + TZ, th & 63, th >> 6);
-            }
-        for (int th = 0; th < 256; th++) rtm::march_points(L, cur[th], S, V, x0, zc, th & 63, th >> 6);
-        if (more)
-            for (int th = 0; th < 256; th++) {
-                rtm::march_stage_rows<R, RZ, RZ>(L, nxt[th], zc + TZ + 4, th & 63, th >> 6);
-                rtm::march_copy(cur[th], nxt[th]);
-            }
-    }
 }
 
 // Emulation of k_columns for one item: every row, all 64 lanes, with the
@@ -120,7 +92,7 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
 }
 
 // One step of one simulation over the work items of the library's builder
-// (rtm::build_work): tiles, plus marched or column interior items.
+// (rtm::build_work): tiles, plus column interior items.
 template <typename R, int RZ>
 static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
 {
@@ -128,9 +100,6 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_row
     if (!V.active) return;
     auto *L = new rtm::TileLds<R, RZ>();
     auto *G = new rtm::TileRegs<R, RZ>[256];
-    auto *M = new rtm::MarchLds<R, RZ>();
-    auto *cur = new rtm::MarchRegs<R, RZ>[256];
-    auto *nxt = new rtm::MarchRegs<R, RZ>[256];
     std::vector<rtm::WorkItem> tiles, inner;
     rtm::build_work(tiles, inner, 0, S.X, S.Z, S.TL, S.TR, S.TU, S.TD, RZ, interior, seg_rows);
     for (const auto &t : tiles) {
@@ -142,13 +111,7 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_row
         default: run_tile<R, RZ, 3>(*L, G, S, V, t.b, t.c); break;
         }
     }
-    for (const auto &it : inner) {
-        if (interior == rtm::IN_MARCH) run_march<R, RZ>(*M, cur, nxt, S, V, it.b, it.c, it.d);
-        else run_column_item<R>(S, V, it.a, it.b, it.c, it.d);
-    }
-    delete[] nxt;
-    delete[] cur;
-    delete M;
+    for (const auto &it : inner) run_column_item<R>(S, V, it.a, it.b, it.c, it.d);
     delete[] G;
     delete L;
 }

@@ -43,16 +43,14 @@ def test_forward_backward_bitexact(hx, name, dtype):
 
 @pytest.mark.parametrize("name", ["tiny", "asym", "wide"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (1, 1, 8), (2, 1, 16),
-                                             (4, 1, 48), (8, 1, 1000), (4, 2, 0), (2, 2, 12), (1, 2, 4),
-                                             (8, 2, 32)])
+@pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (4, 1, 0), (2, 1, 12),
+                                             (1, 1, 4), (8, 1, 32)])
 def test_tiled_kernel_bitexact(hx, name, dtype, rz, interior, seg):
     """The step kernels' bodies run on the CPU against the oracle: the
     LDS-tiled kernel (rtm_tile.h, its phases in sequence for all threads of
     each tile) everywhere or on the CPML strips only, with the interior
-    marched (rtm_march.h) or stepped by lane-shift columns (rtm_column.h,
-    lane exchanges as array shifts).  Fields, snapshots, recorded traces and
-    images."""
+    stepped by lane-shift columns (rtm_column.h, lane exchanges as array
+    shifts).  Fields, snapshots, recorded traces and images."""
     cfg, g = _case(name)
     op = common.orc_params(cfg)
     vpad = O.extend(op, g["models"][0])
@@ -75,7 +73,7 @@ def test_tiled_kernel_bitexact(hx, name, dtype, rz, interior, seg):
 def _case(name):
     """A golden configuration, or "wide": a grid large enough that the
     interior spans three tile columns (a 128-column item and a 64-column
-    one) and several marched chunks (receivers on an interior row, source
+    one) and several column segments (receivers on an interior row, source
     inside, 3 imaging steps)."""
     if name != "wide":
         return common.REF_CONFIGS[name], np.load(os.path.join(common.GOLDEN, f"{name}.npz"))

@@ -89,7 +89,7 @@ def test_tilings_and_graph_blocks_bitexact(gpu_available, rz, block):
 
 def _wide_case():
     """A grid whose interior spans three tile columns (one 128-column and one
-    64-column item) and several marched chunks, with the source and the
+    64-column item) and several column segments, with the source and the
     receivers in the interior."""
     cfg = dict(Nx=250, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
                spacingGrid=10.0, sampling=0.001, numberOfTimeStep=120, sampligWavelet=120,
@@ -104,12 +104,11 @@ def _wide_case():
     return cfg, model, wav, seis
 
 
-@pytest.mark.parametrize("rz,interior,seg", [(1, 1, 8), (2, 1, 16), (4, 1, 48), (4, 1, 1000), (8, 1, 64),
-                                             (4, 2, 0), (2, 2, 12), (1, 2, 4), (8, 2, 32), (4, 2, 16)])
+@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 0), (2, 1, 12), (1, 1, 4), (8, 1, 32), (4, 1, 16)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
-    """Strip tiles plus an interior kernel (LDS-ring march or lane-shift
-    columns, including a 64-column item): fields, recorded traces and images
+    """Strip tiles plus the lane-shift column kernel on the interior
+    (including a 64-column item): fields, recorded traces and images
     bit-identical to the oracle."""
     say(f"[gpu] interior={interior} rz={rz} rows={seg} {np.dtype(dtype).name}")
     cfg, model, wav, seis = _wide_case()
@@ -138,7 +137,7 @@ def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
-@pytest.mark.parametrize("interior,seg", [(1, 32), (2, 16)])
+@pytest.mark.parametrize("interior,seg", [(0, 0), (1, 16)])
 def test_batch_marched_bitexact(gpu_available, interior, seg):
     """A batch with a streaming interior kernel: every realization's image
     equals the oracle's, in the batch as alone."""
