@@ -142,6 +142,7 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-precision", action="store_true", help="skip the snapshot-precision variants")
     ap.add_argument("--batch", type=int, default=64, help="realizations in the batched run (0: skip)")
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
     ap.add_argument("--profile", action="store_true",
@@ -260,6 +261,8 @@ def main() -> None:
     }
     if args.batch > 0:
         out["batched"] = run_batch(R, p, v_true, w, seis, args.batch, args.batch_ts)
+    if not args.no_precision:
+        out["precision"] = run_precision(R, p, v_mig, w, seis, best[1])
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
@@ -305,6 +308,54 @@ def run_profile(R, nt=400, nsims=16):
             b.migrate(pb.first_location_shot, pb.depth_shot)
             log(f"[profile] batch{nsims} {tiling_dict(shape)}", b.last_timing())
     print(json.dumps({"profile": "done", "nt": nt}), flush=True)
+
+
+def run_precision(R, p1, v_mig, w, seis, shape):
+    """Config c3: the c1 workload with the source snapshot stored every 4
+    steps in FP32, in FP16 or in BF16 (per-snapshot max-abs power-of-two
+    scale), stepping FP32, image accumulated in FP32 with Kahan compensation
+    (and, for reference, FP32 without it); each image against the FP64 GPU
+    run of the same scheme (FP64 stepping, FP64 snapshots, FP64 image)."""
+    p = p1.copy().update(time_slices=4)
+    sx, sz = p.first_location_shot, p.depth_shot
+    vpad = R.extend(p, v_mig)
+
+    def image(q, real=R.REAL_F32):
+        with R.Plan(q, real) as plan:
+            if real == R.REAL_F32:
+                plan.set_tiling(*shape)
+            plan.set_velocity(vpad)
+            plan.set_wavelet(w)
+            plan.set_seismogram(seis)
+            plan.zero_image()
+            plan.synchronize()
+            t0 = time.perf_counter()
+            plan.migrate(sx, sz)
+            plan.synchronize()
+            dt = time.perf_counter() - t0
+            I, _ = plan.image(f64=real == R.REAL_F64)
+        return I, dt
+
+    I64, t64 = image(p, R.REAL_F64)
+    nrm = float(np.max(np.abs(I64)))
+    out = {"workload": "c3: c1 workload, snapshots every 4 steps (1000 stored), FP32 stepping, "
+                       "errors against the FP64 GPU run of the same scheme",
+           "fp64_seconds": round(t64, 4), "variants": []}
+    nint = p.nx * p.nz
+    for name, mode, kahan in (("fp32", R.SNAP_FP32, 1), ("fp16_scaled", R.SNAP_FP16, 1),
+                              ("bf16_scaled", R.SNAP_BF16, 1), ("fp32_no_kahan", R.SNAP_FP32, 0)):
+        q = p.copy().update(snapshot_mode=mode, image_kahan=kahan)
+        I, dt = image(q)
+        err = np.abs(I.astype(np.float64) - I64)
+        v = {"variant": name, "kahan_image": bool(kahan), "seconds_per_image": round(dt, 4),
+             "snapshot_store_GB": round(1000 * nint * (2 if mode in (R.SNAP_FP16, R.SNAP_BF16) else 4) / 1e9, 3),
+             "relL2_vs_fp64": rel_l2(I, I64), "max_abs_err": float(err.max()),
+             "max_abs_err_rel_to_max": float(err.max() / nrm)}
+        out["variants"].append(v)
+        log(f"[bench] c3 {name}: relL2 {v['relL2_vs_fp64']:.3e} max-abs {v['max_abs_err']:.3e} "
+            f"({v['max_abs_err_rel_to_max']:.2e} of max) {dt * 1e3:.1f} ms/image")
+    out["boundary_saving"] = "not built"
+    return out
 
 
 def rel_l2(a, b):

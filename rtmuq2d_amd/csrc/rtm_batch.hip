@@ -88,7 +88,7 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     *out = nullptr;
     if (rtm_params_check(p)) return -1;
     if (real != RTM_REAL_F32 && real != RTM_REAL_F64) { rtm_set_error("rtm_batch_create: bad real type"); return -1; }
-    if (p->snapshot_mode != RTM_SNAP_FP32) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BF16) {
         rtm_set_error("rtm_batch_create: snapshot mode %d not available", p->snapshot_mode);
         return -1;
     }
@@ -173,6 +173,7 @@ static int batch_pass(rtm_batch *q, int phase, int src_j)
     for (int s = 0; s < N; s++) {
         SimBuffers &b = q->sims[s];
         b.reset_state(q->stream);
+        if (phase == rtm::PH_FORWARD) b.reset_snap_scales(q->stream);
         d[s] = b.desc<R>();
         d[s].phase = phase;
         if (phase == rtm::PH_FORWARD) {
@@ -187,6 +188,9 @@ static int batch_pass(rtm_batch *q, int phase, int src_j)
     const int nt = q->sims[0].p.number_of_time_steps;
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+    if (phase == rtm::PH_FORWARD)
+        for (int s = 0; s < N; s++)
+            if (q->sims[s].flush_last_snapshot(q->stream)) return -1;
     HIP_TRY(hipEventRecord(q->ev1, q->stream));
     HIP_TRY(hipEventSynchronize(q->ev1));
     float ms = 0.f;

@@ -58,12 +58,13 @@ RTM_HD Pair<R> col_update(const Pair<R> *w, Pair<R> L1, Pair<R> L2, Pair<R> R1, 
     return U;
 }
 
-/* Source add, receiver replacement or recording, snapshot store and
- * imaging for the lane's two points at row k (in the order of
- * tile_points); the row tests are uniform across the wavefront. */
+/* Source add, receiver replacement or recording, snapshot store (FP32, or
+ * the 16-bit store's max |U| and quantization of u^n) and imaging for the
+ * lane's two points at row k (in the order of tile_points); the row tests
+ * are uniform across the wavefront.  uc = u^n at the two points. */
 template <typename R>
 RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl, bool ok, int src_row, Pair<R> &U,
-                       Pair<R> Cp)
+                       Pair<R> Cp, Pair<R> uc, float &umax)
 {
     if (V.forward) {
         if (k == src_row) {
@@ -84,25 +85,30 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         }
     }
     const int kk = k - S.TU;
-    if (!V.snap || kk < 0 || kk >= S.nz || !ok) return;
-    R vals[2] = {U.a, U.b};
+    if (!(V.snap || V.qsnap || V.smax) || kk < 0 || kk >= S.nz) return;
+    if (!ok) return;
+    const R vals[2] = {U.a, U.b};
+    const R us[2] = {uc.a, uc.b};
 #pragma unroll
     for (int e = 0; e < 2; e++) {
         const int ii = xl + e - S.TL;
         if (ii < 0 || ii >= S.nx) continue;
         const int q = kk * S.nx + ii;
         if (V.forward) {
-            stg(V.snap, q, vals[e]);
-        } else if (S.flags & F_KAHAN) {
-            R I = ldg(S.I, q), Ic = ldg(S.Ic, q), Is = ldg(S.Is, q), Isc = ldg(S.Isc, q);
-            const R s = ldg(V.snap, q);
-            kahan_add(I, Ic, s * vals[e]);
-            kahan_add(Is, Isc, s * s);
-            stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
+            if (V.snap) stg(V.snap, q, vals[e]);
+            if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
+            if (V.qsnap) stg(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
         } else {
-            const R s = ldg(V.snap, q);
-            stg(S.I, q, ldg(S.I, q) + s * vals[e]);
-            stg(S.Is, q, ldg(S.Is, q) + s * s);
+            const R s = V.snap ? ldg(V.snap, q) : R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            if (S.flags & F_KAHAN) {
+                R I = ldg(S.I, q), Ic = ldg(S.Ic, q), Is = ldg(S.Is, q), Isc = ldg(S.Isc, q);
+                kahan_add(I, Ic, s * vals[e]);
+                kahan_add(Is, Isc, s * s);
+                stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
+            } else {
+                stg(S.I, q, ldg(S.I, q) + s * vals[e]);
+                stg(S.Is, q, ldg(S.Is, q) + s * s);
+            }
         }
     }
 }

@@ -97,8 +97,8 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
         rtm_set_error("rtm_plan_create: unknown real type %d", real);
         return -1;
     }
-    if (p->snapshot_mode != RTM_SNAP_FP32) {
-        rtm_set_error("rtm_plan_create: snapshot mode %d is served by the batch API", p->snapshot_mode);
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BF16) {
+        rtm_set_error("rtm_plan_create: snapshot mode %d is not available", p->snapshot_mode);
         return -1;
     }
     HIP_TRY(hipSetDevice(p->device));
@@ -280,7 +280,12 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
     const int src_j = zs * b.P + xs;
     const int nt = p.number_of_time_steps;
     b.reset_state(q->stream);
+    b.reset_snap_scales(q->stream);
     const bool tiled = src_inside;   // the tiled kernel covers only the stepped region
+    if (!tiled && b.qmode != rtm::Q_NONE) {
+        rtm_set_error("16-bit snapshot stores need the source inside the stepped region");
+        return -1;
+    }
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (tiled) {
         SimDesc<R> d = b.desc<R>();
@@ -299,6 +304,7 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
         } else if (q->ex.run(nt, q->graph_block, q->stream)) {
             return -1;
         }
+        if (b.flush_last_snapshot(q->stream)) return -1;
     } else {
         v1_forward<R>(q, src_j, src_inside, record);
     }
@@ -330,6 +336,9 @@ static int backward_t(rtm_plan *q)
         d.flags = rtm::F_IMAGE | (p.image_kahan ? rtm::F_KAHAN : 0);
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+    } else if (b.qmode != rtm::Q_NONE) {
+        rtm_set_error("16-bit snapshot stores need every receiver inside the stepped region");
+        return -1;
     } else {
         v1_backward<R>(q);
     }

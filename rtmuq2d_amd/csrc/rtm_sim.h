@@ -48,8 +48,37 @@ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // that each XCD (workgroups are dealt round-robin to the 8 XCDs) walks a
 // contiguous run of neighbouring tiles and their shared halo rows stay in its
 // L2.  `s` is the step within the current block; sim i is at n0[i] + s.
+// A snapshot's max |U| for the 16-bit store: wavefront max by butterfly
+// shuffles, then a (vector) atomic max on the float bits, whose unsigned
+// order is the order of non-negative floats, into one of the snapshot's
+// slots (rtm::kSnapSlots, picked by `who`).
+__device__ __forceinline__ float wave_max(float m)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    return m;
+}
+
+__device__ __forceinline__ void commit_snap_max(unsigned *slots, float m, int who)
+{
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(slots + (who % rtm::kSnapSlots) * rtm::kSlotStride, __float_as_uint(m));
+}
+
+// the same for a 256-thread workgroup: one atomic per workgroup
+__device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, float *red)
+{
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float t = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(slots + (blockIdx.x % rtm::kSnapSlots) * rtm::kSlotStride, __float_as_uint(t));
+    }
+}
+
 template <typename R, int RZ, int CLS>
-__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, const SimDesc<R> &S,
+__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S,
                                          const rtm::StepView<R> &V, int x0, int z0, int tx, int ty)
 {
     rtm::TileRegs<R, RZ> G;
@@ -60,7 +89,8 @@ __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, const SimDesc<R
         rtm::tile_psi<R, RZ, CLS>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    rtm::tile_points<R, RZ, CLS>(L, G, S, V, x0, z0, tx, ty);
+    const float m = rtm::tile_points<R, RZ, CLS>(L, G, S, V, x0, z0, tx, ty);
+    if (V.smax) commit_snap_max_wg(V.smax, m, red);
 }
 
 template <typename R, int RZ>
@@ -70,6 +100,7 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
                                                const int *__restrict__ ntiles_p, int s)
 {
     __shared__ rtm::TileLds<R, RZ> L;
+    __shared__ float red[4];
     const int ntiles = *ntiles_p;
     const int per = (ntiles + 7) >> 3;
     const int b = blockIdx.x;
@@ -84,10 +115,10 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0>(L, S, V, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1>(L, S, V, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2>(L, S, V, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3>(L, S, V, tile.y, tile.z, tx, ty); break;
+    case 0: run_tile<R, RZ, 0>(L, red, S, V, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1>(L, red, S, V, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2>(L, red, S, V, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3>(L, red, S, V, tile.y, tile.z, tx, ty); break;
     }
 }
 
@@ -164,7 +195,7 @@ __device__ __forceinline__ void col_fetch(rtm::Pair<R> (&w)[16], ColSet<R> &st, 
 // receiver, snapshot or imaging term.
 template <typename R, int B, bool EXTRAS>
 __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const ColSet<R> &st, const ColCtx<R> &X,
-                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0)
+                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, float &umax)
 {
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
@@ -184,7 +215,7 @@ __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const Col
         R2.a = lane_shl1(R1.a, H.z);
         R2.b = lane_shl1(R1.b, H.w);
         rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, st.C[rr], st.O[rr]);
-        if (EXTRAS) rtm::col_extras(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr]);
+        if (EXTRAS) rtm::col_extras(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr], c, umax);
         if (X.ok) rtm::stg(X.U2 + k * X.Ph, X.xh, U);
     }
 }
@@ -194,15 +225,15 @@ __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const Col
 template <typename R, int CI>
 __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, ColSet<R> &nxt, const ColCtx<R> &X,
                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, int zB,
-                                         int ex_lo, int ex_hi)
+                                         int ex_lo, int ex_hi, float &umax)
 {
     col_fetch<R, (4 * CI + 12) & 15>(w, nxt, X, k0 + 4, zB);
     // rows [ex_lo, ex_hi) carry snapshot/imaging terms; single rows the
     // source or receiver terms
     const bool extras = (k0 < ex_hi && k0 + 4 > ex_lo) || (X.srow >= k0 && X.srow < k0 + 4)
                         || (S.rec_row >= k0 && S.rec_row < k0 + 4 && (!V.forward || V.rec_row_out));
-    if (extras) col_chunk<R, 4 * CI, true>(w, cur, X, S, V, k0);
-    else col_chunk<R, 4 * CI, false>(w, cur, X, S, V, k0);
+    if (extras) col_chunk<R, 4 * CI, true>(w, cur, X, S, V, k0, umax);
+    else col_chunk<R, 4 * CI, false>(w, cur, X, S, V, k0, umax);
 }
 
 // The interior column kernel (rtm_column.h): four independent wavefronts
@@ -246,22 +277,25 @@ __global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ 
     X.hq = rtm::clampi(lane == 63 ? x0 + 128 : x0 - 4, 0, P - 4) >> 2;
     X.ok = lane < (rtm::col_half(it.x) ? 32 : 64);
     X.srow = rtm::col_src_row(S, V);
-    const int ex_lo = V.snap ? S.TU : 0, ex_hi = V.snap ? S.TU + S.nz : 0;
+    const bool img = V.snap || V.qsnap || V.smax;
+    const int ex_lo = img ? S.TU : 0, ex_hi = img ? S.TU + S.nz : 0;
+    float umax = 0.0f;
     rtm::Pair<R> w[16];
 #pragma unroll
     for (int r = 0; r < 8; r++) w[r] = rtm::ldg(X.u2 + (zA - 4 + r) * X.Ph, X.xh);
     ColSet<R> A, Bs;
     col_fetch<R, 8>(w, A, X, zA, zB);
     for (int k0 = zA;; k0 += 16) {
-        col_step<R, 0>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi);
+        col_step<R, 0>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi, umax);
         if (k0 + 4 >= zB) break;
-        col_step<R, 1>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi);
+        col_step<R, 1>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi, umax);
         if (k0 + 8 >= zB) break;
-        col_step<R, 2>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi);
+        col_step<R, 2>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi, umax);
         if (k0 + 12 >= zB) break;
-        col_step<R, 3>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi);
+        col_step<R, 3>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi, umax);
         if (k0 + 16 >= zB) break;
     }
+    if (V.smax) commit_snap_max(V.smax, umax, item);
 }
 
 // n0[i] += block for every simulation (the last node of a captured block)
@@ -417,7 +451,10 @@ struct SimBuffers {
     float *wav = nullptr;               // [n_w]
     float *seis_t = nullptr;            // [nt][nrec] backward input (t-major)
     float *seis_rec = nullptr;          // [nt][nrec] forward recording
-    void *snaps = nullptr;              // [nsnap][nz][nx]
+    void *snaps = nullptr;              // [nsnap][nz][nx] (snapshot mode FP32)
+    int qmode = rtm::Q_NONE;            // 16-bit snapshot store (modes FP16, BF16)
+    unsigned short *q16 = nullptr;      // [nsnap][nz][nx]
+    unsigned *smax = nullptr;           // [nsnap] max |u| bits per snapshot
     void *I = nullptr, *Is = nullptr, *Ic = nullptr, *Isc = nullptr;
     std::vector<float> wav_host;
     bool have_velocity = false, have_seis = false, have_snaps = false;
@@ -434,6 +471,9 @@ struct SimBuffers {
     int set_wavelet(const float *w, int n);
     int set_seismogram(const float *d, hipStream_t s);
     void reset_state(hipStream_t s);      // zero u, psi, csi (a pass starts at rest)
+    void reset_snap_scales(hipStream_t s); // zero the 16-bit store's max |u| slots
+    // the 16-bit store's last snapshot when it is the pass's final field
+    int flush_last_snapshot(hipStream_t s);
     void zero_image(hipStream_t s);
     template <typename R>
     SimDesc<R> desc() const;

@@ -18,7 +18,10 @@ size_t sim_bytes(const rtm_params &p, size_t esz)
     b += (size_t)X * 4 + 3 * nrec * 4;              // receiver tables
     b += (size_t)p.sampling_wavelet * 4;
     b += 2 * nt * nrec * 4;                         // input and recorded traces
-    b += nsnap * nint * esz;                        // snapshot store
+    if (p.snapshot_mode == RTM_SNAP_FP16 || p.snapshot_mode == RTM_SNAP_BF16)
+        b += nsnap * nint * 2 + nsnap * rtm::kSnapWords * 4;   // 16-bit snapshot store + max slots
+    else
+        b += nsnap * nint * esz;                    // snapshot store
     b += 4 * nint * esz;                            // I, Is and their compensations
     return b;
 }
@@ -54,7 +57,22 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
     if (!rc) rc = get(&psiZ[1], fb);
     if (!rc) rc = get(&csiX, fb);
     if (!rc) rc = get(&csiZ, fb);
-    if (!rc) rc = get(&snaps, nsnap * nint * esz);
+    qmode = p.snapshot_mode == RTM_SNAP_FP16 ? rtm::Q_FP16
+            : p.snapshot_mode == RTM_SNAP_BF16 ? rtm::Q_BF16 : rtm::Q_NONE;
+    if (!rc && p.snapshot_mode == RTM_SNAP_BOUNDARY) {
+        rtm_set_error("snapshot mode %d (boundary saving) is not available", p.snapshot_mode);
+        rc = -1;
+    }
+    if (!rc && qmode != rtm::Q_NONE && real == RTM_REAL_F64) {
+        rtm_set_error("16-bit snapshot stores are for FP32 stepping (the FP64 plan is the oracle run)");
+        rc = -1;
+    }
+    if (qmode == rtm::Q_NONE) {
+        if (!rc) rc = get(&snaps, nsnap * nint * esz);
+    } else {
+        if (!rc) rc = get((void **)&q16, nsnap * nint * 2);
+        if (!rc) rc = get((void **)&smax, nsnap * rtm::kSnapWords * 4);
+    }
     if (!rc) rc = get(&I, nint * esz);
     if (!rc) rc = get(&Is, nint * esz);
     if (!rc) rc = get(&Ic, nint * esz);
@@ -106,6 +124,8 @@ void SimBuffers::release()
     recmap = rec_j = out_j = out_r = nullptr;
     wav = seis_t = seis_rec = nullptr;
     snaps = I = Is = Ic = Isc = nullptr;
+    q16 = nullptr;
+    smax = nullptr;
 }
 
 template <typename R>
@@ -239,6 +259,37 @@ void SimBuffers::reset_state(hipStream_t s)
     }
 }
 
+void SimBuffers::reset_snap_scales(hipStream_t s)
+{
+    if (smax) fill_zero(smax, nsnap * rtm::kSnapWords, s);
+}
+
+// The 16-bit store is filled one step late, from u^n of the step after the
+// snapshot step (rtm::step_view); a snapshot taken on the pass's last step
+// (TS = 1) is filled here from the final field instead.
+template <typename R>
+__global__ void k_quantize_interior(const R *__restrict__ u, int P, int TL, int TU, int nx, int nz,
+                                    unsigned short *__restrict__ q, const unsigned *__restrict__ slots, int mode)
+{
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x, kk = blockIdx.y;
+    if (ii >= nx || kk >= nz) return;
+    const float scale = rtm::pow2f(14 - rtm::snap_exp(rtm::snap_max_bits(slots)));
+    q[(size_t)kk * nx + ii] = rtm::quant16(float(u[(size_t)(kk + TU) * P + ii + TL]), scale, mode);
+}
+
+int SimBuffers::flush_last_snapshot(hipStream_t s)
+{
+    const int nt = p.number_of_time_steps, TS = p.time_slices;
+    if (qmode == rtm::Q_NONE || (nt - 1) % TS != 0) return 0;
+    const int c = (nt - 1) / TS;
+    const dim3 grid((p.nx + 255) / 256, p.nz);
+    hipLaunchKernelGGL(k_quantize_interior<float>, grid, dim3(256), 0, s, (const float *)u[nt & 1], P,
+                       p.thickness_left, p.thickness_up, p.nx, p.nz, q16 + (size_t)c * nint,
+                       smax + (size_t)c * rtm::kSnapWords, qmode);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 void SimBuffers::zero_image(hipStream_t s)
 {
     if (real == RTM_REAL_F64) {
@@ -282,6 +333,9 @@ SimDesc<R> SimBuffers::desc() const
     d.seis_t = seis_t;
     d.seis_rec = seis_rec;
     d.snaps = (R *)snaps;
+    d.qmode = qmode;
+    d.q16 = q16;
+    d.smax = smax;
     d.I = (R *)I; d.Is = (R *)Is; d.Ic = (R *)Ic; d.Isc = (R *)Isc;
     return d;
 }

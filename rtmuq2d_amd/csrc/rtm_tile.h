@@ -27,6 +27,8 @@
 #ifndef RTM_TILE_H
 #define RTM_TILE_H
 
+#include <math.h>
+
 #include "rtm_step.h"
 
 namespace rtm {
@@ -94,7 +96,110 @@ struct SimDesc {
     float *seis_rec;        /* [nt][nrec] recorded traces (forward)    */
     R *snaps;               /* [nt/TS][nz][nx]                          */
     R *I, *Is, *Ic, *Isc;   /* [nz][nx]                                 */
+    int qmode;              /* snapshot store: 0 as R, Q_FP16, Q_BF16   */
+    unsigned short *q16;    /* [nt/TS][nz][nx] scaled 16-bit store      */
+    unsigned *smax;         /* [nt/TS][kSnapSlots * kSlotStride] max |u| slots (float bits) */
 };
+
+/* 16-bit snapshot store.  Each snapshot c is stored as u * 2^(14-e_c),
+ * rounded to FP16 or BF16 (nearest, ties to even), where e_c is the binary
+ * exponent of the snapshot's max |u| (clamped to [-100, 100]): the largest
+ * value lands in [2^14, 2^15), well inside FP16's range, and the power-of-two
+ * scale and its inverse are exact, so the only error is the rounding.  The
+ * conversions are integer code shared by the device and the CPU harness. */
+enum QMode : int { Q_NONE = 0, Q_FP16 = 1, Q_BF16 = 2 };
+
+/* A snapshot's max |u| is accumulated with atomic max over kSnapSlots
+ * counters, kSlotStride words (256 bytes) apart so that concurrent
+ * workgroups update different lines; its value is the max of the slots. */
+constexpr int kSnapSlots = 32, kSlotStride = 64, kSnapWords = kSnapSlots * kSlotStride;
+
+RTM_HD unsigned f32_bits(float f)
+{
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+    return u;
+}
+
+RTM_HD float bits_f32(unsigned u)
+{
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+RTM_HD int snap_exp(unsigned maxabs_bits)
+{
+    const int e = (int)((maxabs_bits >> 23) & 0xff) - 127;
+    return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
+RTM_HD float pow2f(int k) { return bits_f32((unsigned)(k + 127) << 23); }   /* k in [-126, 127] */
+
+RTM_HD unsigned short f32_to_f16(float f)
+{
+    const unsigned x = f32_bits(f);
+    const unsigned sign = (x >> 16) & 0x8000u;
+    unsigned mant = x & 0x7fffffu;
+    const int ex = (int)((x >> 23) & 0xff);
+    if (ex == 0xff) return (unsigned short)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+    const int e = ex - 112;                   /* FP16 biased exponent */
+    if (e >= 31) return (unsigned short)(sign | 0x7c00u);
+    if (e <= 0) {                             /* FP16 subnormal or zero */
+        if (e < -10) return (unsigned short)sign;
+        mant |= 0x800000u;
+        const int shift = 14 - e;
+        unsigned h = mant >> shift;
+        const unsigned rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return (unsigned short)(sign | h);
+    }
+    unsigned h = ((unsigned)e << 10) | (mant >> 13);
+    const unsigned rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;   /* a carry rounds into the exponent */
+    return (unsigned short)(sign | h);
+}
+
+RTM_HD float f16_to_f32(unsigned short h)
+{
+    const unsigned sign = ((unsigned)h & 0x8000u) << 16;
+    const unsigned e = ((unsigned)h >> 10) & 0x1fu, m = (unsigned)h & 0x3ffu;
+    if (e == 0) {                             /* m * 2^-24, exact */
+        const float v = (float)m * 5.9604644775390625e-8f;
+        return sign ? -v : v;
+    }
+    if (e == 31) return bits_f32(sign | 0x7f800000u | (m << 13));
+    return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+RTM_HD unsigned short f32_to_bf16(float f)
+{
+    const unsigned x = f32_bits(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((x >> 16) | 0x40u);   /* NaN */
+    return (unsigned short)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+RTM_HD float bf16_to_f32(unsigned short b) { return bits_f32((unsigned)b << 16); }
+
+RTM_HD unsigned short quant16(float v, float scale, int mode)
+{
+    return mode == Q_FP16 ? f32_to_f16(v * scale) : f32_to_bf16(v * scale);
+}
+
+RTM_HD float dequant16(unsigned short q, float scale, int mode)
+{
+    return (mode == Q_FP16 ? f16_to_f32(q) : bf16_to_f32(q)) * scale;
+}
+
+RTM_HD unsigned snap_max_bits(const unsigned *slots)
+{
+    unsigned m = 0;
+    for (int t = 0; t < kSnapSlots; t++) {
+        const unsigned v = ldg(slots, t * kSlotStride);
+        m = v > m ? v : m;
+    }
+    return m;
+}
 
 template <typename R, int RZ>
 struct TileLds {
@@ -117,6 +222,12 @@ struct StepView {
     const float *seis_row;  /* backward: traces at tau */
     float *rec_row_out;     /* forward recording row   */
     R *snap;                /* snapshot written (forward) or read (backward) */
+    /* 16-bit store: forward, the slot of the previous step's snapshot, filled
+     * this step from u^n (= that step's result) with scale 2^(14-e); backward,
+     * the slot read, with scale 2^(e-14) */
+    unsigned short *qsnap;
+    float qscale;
+    unsigned *smax;         /* forward, 16-bit store: this step's snapshot max |U| */
 };
 
 template <typename R>
@@ -138,11 +249,28 @@ RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
         v.src = n < S.nw && S.src_j >= 0;
         v.src_w = v.src ? R(ldg(S.wav, n)) : R(0);
         v.rec_row_out = (S.flags & F_RECORD) ? S.seis_rec + (size_t)n * S.nrec : nullptr;
-        v.snap = ((S.flags & F_SNAP) && n % S.TS == 0) ? S.snaps + (size_t)(n / S.TS) * nint : nullptr;
+        const bool store = (S.flags & F_SNAP) && n % S.TS == 0;
+        if (S.qmode == Q_NONE) {
+            v.snap = store ? S.snaps + (size_t)(n / S.TS) * nint : nullptr;
+        } else {
+            v.smax = store ? S.smax + (size_t)(n / S.TS) * kSnapWords : nullptr;
+            if ((S.flags & F_SNAP) && n >= 1 && (n - 1) % S.TS == 0) {
+                const int c = (n - 1) / S.TS;
+                v.qsnap = S.q16 + (size_t)c * nint;
+                v.qscale = pow2f(14 - snap_exp(snap_max_bits(S.smax + (size_t)c * kSnapWords)));
+            }
+        }
     } else {
         const int tau = S.nt - 1 - n;
         v.seis_row = S.seis_t + (size_t)tau * S.nrec;
-        v.snap = ((S.flags & F_IMAGE) && tau % S.TS == 0) ? S.snaps + (size_t)(tau / S.TS) * nint : nullptr;
+        const bool img = (S.flags & F_IMAGE) && tau % S.TS == 0;
+        if (S.qmode == Q_NONE) {
+            v.snap = img ? S.snaps + (size_t)(tau / S.TS) * nint : nullptr;
+        } else if (img) {
+            const int c = tau / S.TS;
+            v.qsnap = S.q16 + (size_t)c * nint;
+            v.qscale = pow2f(snap_exp(snap_max_bits(S.smax + (size_t)c * kSnapWords)) - 14);
+        }
     }
     return v;
 }
@@ -257,13 +385,14 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
     }
-    if (!V.forward && V.snap) {
+    if (!V.forward && (V.snap || V.qsnap)) {
         const int ii = clampi(i - S.TL, 0, S.nx - 1);
 #pragma unroll
         for (int rr = 0; rr < RZ; rr++) {
             const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
             const int q = kk * S.nx + ii;
-            G.sv[rr] = ldg(V.snap, q);
+            if (V.snap) G.sv[rr] = ldg(V.snap, q);
+            else G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
             G.Iv[rr] = ldg(S.I, q);
             G.Isv[rr] = ldg(S.Is, q);
             if (S.flags & F_KAHAN) {
@@ -343,10 +472,11 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
 
 /* Phase 3: the fused update of this thread's RZ points. */
 template <typename R, int RZ, int CLS>
-RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
-                        const StepView<R> &V, int x0, int z0, int tx, int ty)
+RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
+                         const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4;
+    float umax = 0.0f;   /* max |U| over the thread's image points (16-bit store) */
     (void)HX;
     const int i = x0 + tx;
     const bool iok = i < S.X - kHalf;
@@ -415,10 +545,12 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const
             }
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
-            if (V.snap && img_col && kk >= 0 && kk < S.nz) {
+            if ((V.snap || V.qsnap || V.smax) && img_col && kk >= 0 && kk < S.nz) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
-                    stg(V.snap, q, U);
+                    if (V.snap) stg(V.snap, q, U);
+                    if (V.smax) umax = fmaxf(umax, fabsf(float(U)));
+                    if (V.qsnap) stg(V.qsnap, q, quant16(float(uc), V.qscale, S.qmode));
                 } else if (S.flags & F_KAHAN) {
                     R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
                     const R s = G.sv[rr];
@@ -433,10 +565,9 @@ RTM_HD void tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const
             }
         }
     }
+    return umax;
 }
 
-/* The whole tile for a class known at compile time (the kernel and the CPU
- * harness dispatch on the tile's class). */
 }  // namespace rtm
 
 #endif /* RTM_TILE_H */

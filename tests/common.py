@@ -127,3 +127,39 @@ def rel_l2(a, b) -> float:
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 -> nearest BF16 value (ties to even), computed in float64 from
+    the definition: 8 significant bits, or multiples of 2^-133 below 2^-126."""
+    x = np.asarray(x, np.float64)
+    _, e = np.frexp(x)                      # x = m * 2^e, 0.5 <= |m| < 1
+    quantum = np.exp2(np.maximum(e - 8, -133))   # 8 significant bits; BF16's subnormal grid 2^-133
+    return (np.round(x / quantum) * quantum).astype(np.float32)   # numpy rounds half to even
+
+
+def quantize_snapshots(S, mode: int):
+    """The 16-bit snapshot store's rule, restated independently: snapshot c
+    is multiplied by 2^(14-e), e the binary exponent of max |S_c| (clamped to
+    [-100, 100]), rounded to FP16 (numpy's IEEE conversion, nearest-even) or
+    BF16 (bf16_round), and read back times 2^(e-14).  Returns the read-back
+    float32 snapshots, the 16-bit codes and the max-abs float bits."""
+    S = np.asarray(S, np.float32)
+    deq = np.empty_like(S)
+    codes = np.empty(S.shape, np.uint16)
+    mx = np.empty(len(S), np.uint32)
+    for c in range(len(S)):
+        m = np.float32(np.max(np.abs(S[c])))
+        bits = int(np.array([m], np.float32).view(np.uint32)[0])
+        e = min(max(((bits >> 23) & 0xFF) - 127, -100), 100)
+        x = (S[c] * np.float32(2.0 ** (14 - e))).astype(np.float32)
+        if mode == 1:
+            h = x.astype(np.float16)
+            codes[c] = h.view(np.uint16)
+            back = h.astype(np.float32)
+        else:
+            back = bf16_round(x)
+            codes[c] = (back.view(np.uint32) >> 16).astype(np.uint16)
+        deq[c] = back * np.float32(2.0 ** (e - 14))
+        mx[c] = bits
+    return deq, codes, mx

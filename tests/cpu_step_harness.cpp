@@ -32,6 +32,14 @@ static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const 
             for (int c = 0; c < 8; c++) rtm::cpml_top_edge_point(A, c < 4 ? c : A.X - 8 + c, k);
 }
 
+// The kernels' max |U| commit (a wavefront max and an atomic max on the
+// float bits) done sequentially.
+template <typename R>
+static void commit_max(const rtm::StepView<R> &V, float m)
+{
+    if (V.smax && rtm::f32_bits(m) > *V.smax) *V.smax = rtm::f32_bits(m);
+}
+
 // Emulation of k_tiles for one simulation at step n: every tile, phase by
 // phase, all 256 threads of a phase before the next (the barriers).
 template <typename R, int RZ, int CLS>
@@ -46,8 +54,13 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
     if (CLS)
         for (int ty = 0; ty < 4; ty++)
             for (int tx = 0; tx < 64; tx++) rtm::tile_psi<R, RZ, CLS>(L, G[ty * 64 + tx], S, x0, z0, tx, ty);
+    float m = 0.0f;
     for (int ty = 0; ty < 4; ty++)
-        for (int tx = 0; tx < 64; tx++) rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+        for (int tx = 0; tx < 64; tx++) {
+            const float t = rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            m = t > m ? t : m;
+        }
+    commit_max(V, m);
 }
 
 // Emulation of k_columns for one item: every row, all 64 lanes, with the
@@ -59,6 +72,7 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
     const int P = S.P;
     const int half = rtm::col_half(packed);
     const int srow = rtm::col_src_row(S, V);
+    float umax = 0.0f;
     auto at = [&](const R *f, int k, int x) { return f[(size_t)k * P + x]; };
     for (int k = zA; k < zB; k++) {
         P2 c[64], L1[64], L2[64], R1[64], R2[64];
@@ -82,13 +96,14 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
             const P2 Op = {at(V.U, k, xc[l]), at(V.U, k, xc[l] + 1)};
             P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, Op);
             const bool ok = l < (half ? 32 : 64);
-            rtm::col_extras(S, V, k, xl[l], ok, srow, U, Cp);
+            rtm::col_extras(S, V, k, xl[l], ok, srow, U, Cp, w[4], umax);
             if (ok) {
                 V.U[(size_t)k * P + xl[l]] = U.a;
                 V.U[(size_t)k * P + xl[l] + 1] = U.b;
             }
         }
     }
+    commit_max(V, umax);
 }
 
 // One step of one simulation over the work items of the library's builder
@@ -126,6 +141,9 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.nx = g[7]; S.nz = g[8]; S.nt = g[9]; S.TS = g[10]; S.nw = g[11]; S.nrec = g[12];
     S.phase = g[13]; S.flags = g[14]; S.src_j = g[15]; S.rec_row = g[16];
     const int n = g[17], rz = g[18], seg_rows = g[19], interior = g[20];
+    S.qmode = g[21];
+    S.q16 = (unsigned short *)(void *)ptr[18];
+    S.smax = (unsigned *)(void *)ptr[19];
     S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
     S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
     S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];
@@ -142,6 +160,16 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
 }
 
 extern "C" {
+/* the library's final-snapshot fill of the 16-bit store (k_quantize_interior) */
+void hx_quantize_interior(const float *u, int P, int TL, int TU, int nx, int nz, unsigned short *q,
+                          const unsigned *slots, int mode)
+{
+    const float scale = rtm::pow2f(14 - rtm::snap_exp(rtm::snap_max_bits(slots)));
+    for (int kk = 0; kk < nz; kk++)
+        for (int ii = 0; ii < nx; ii++)
+            q[(size_t)kk * nx + ii] = rtm::quant16(u[(size_t)(kk + TU) * P + ii + TL], scale, mode);
+}
+
 void hx_tiles_f32(const int *g, const float *hs, float *const *ptr, const int *recmap, const float *wav,
                   const float *seis_t, float *seis_rec)
 {

@@ -18,6 +18,7 @@ import common
 import oracle as O
 
 M_STENCIL, M_CPML, M_SOURCE, M_INJECT, M_SNAP, M_IMAGE, M_KAHAN = 1, 2, 4, 8, 16, 32, 64
+SNAP_WORDS = 32 * 64   # max-abs slots per snapshot (rtm::kSnapWords)
 
 
 def build_harness(outdir: str) -> C.CDLL:
@@ -132,7 +133,7 @@ class DevSim:
     # ---- the tiled kernel (rtm_tile.h), emulated phase by phase ----
 
     def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False,
-                    interior=0, seg=0):
+                    interior=0, seg=0, qmode=0, q16=None, smax=None):
         op = self.op
         self.reset()
         nint = op.nx * op.nz
@@ -146,27 +147,49 @@ class DevSim:
             snaps = np.zeros((op.nt // TS, op.nz, op.nx), self.dt)
         I = np.zeros((op.nz, op.nx), self.dt)
         Is, Ic, Isc = np.zeros_like(I), np.zeros_like(I), np.zeros_like(I)
+        if q16 is None:
+            q16 = np.zeros((op.nt // TS, op.nz, op.nx), np.uint16)
+            smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
         arrs = [self.u[0], self.u[1], self.C, self.ax, self.bx, self.az, self.bz, self.psiX[0],
-                self.psiX[1], self.psiZ[0], self.psiZ[1], self.csiX, self.csiZ, snaps, I, Is, Ic, Isc]
+                self.psiX[1], self.psiZ[0], self.psiZ[1], self.csiX, self.csiZ, snaps, I, Is, Ic, Isc,
+                q16, smax]
         ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
         hs = self.hs[:3].copy()
         for n in range(op.nt):
             g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx,
                           op.nz, op.nt, TS, op.n_wavelet, op.n_receivers, phase, flags, src_j,
-                          self.rec_row, n, rz, seg, interior], np.int32)
+                          self.rec_row, n, rz, seg, interior, qmode], np.int32)
             fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
         self.cur = op.nt & 1
+        if qmode and phase == 1 and (op.nt - 1) % TS == 0:
+            c = (op.nt - 1) // TS
+            qs = q16[c]
+            self.hx.hx_quantize_interior(_p(self.u[op.nt & 1]), C.c_int(self.P), C.c_int(op.t_left),
+                                         C.c_int(op.t_up), C.c_int(op.nx), C.c_int(op.nz), _p(qs),
+                                         _p(smax[c]), C.c_int(qmode))
         return snaps, I, Is, seis_rec
 
-    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0):
+    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0, qmode=0):
+        """Forward pass; with qmode (1 FP16, 2 BF16) the snapshots are the
+        16-bit store and its max-abs slots: returns (q16, smax), rec."""
         op = self.op
         src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
         flags = 4 | (2 if record else 0)
+        TS = op.time_slices
+        q16 = np.zeros((op.nt // TS, op.nz, op.nx), np.uint16)
+        smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
         snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record,
-                                            interior=interior, seg=seg)
+                                            interior=interior, seg=seg, qmode=qmode, q16=q16, smax=smax)
+        if qmode:
+            return (q16, smax), np.ascontiguousarray(rec.T)
         return snaps, np.ascontiguousarray(rec.T)
 
-    def backward_tiled(self, seis, snaps, rz=4, kahan=False, interior=0, seg=0):
+    def backward_tiled(self, seis, snaps, rz=4, kahan=False, interior=0, seg=0, qmode=0):
         flags = 8 | (1 if kahan else 0)
-        _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, interior=interior, seg=seg)
+        if qmode:
+            q16, smax = snaps
+            _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, interior=interior, seg=seg, qmode=qmode,
+                                           q16=q16, smax=smax)
+        else:
+            _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, interior=interior, seg=seg)
         return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)

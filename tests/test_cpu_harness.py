@@ -150,3 +150,40 @@ def test_kahan_image_is_closer_to_fp64(hx):
     Ik, _ = ds.backward(d, S, kahan=True)
     ep, ek = common.rel_l2(Ip, I64), common.rel_l2(Ik, I64)
     assert ek <= ep * 1.0001, (ek, ep)
+
+
+def test_bf16_rule_matches_bit_rounding():
+    """bf16_round (from the definition, in float64) equals round-to-nearest-
+    even on the high 16 bits of the float32 pattern, ties included."""
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(200000) * np.exp2(rng.integers(-30, 30, 200000))).astype(np.float32)
+    u = x.view(np.uint32)
+    ties = ((u & 0xFFFF0000) | 0x8000).view(np.float32)           # exact halfway cases
+    x = np.concatenate([x, ties, np.float32([0.0, -0.0, 1.0, 65504.0, 3e-39])])
+    u = x.view(np.uint32).astype(np.uint64)
+    bits = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    assert np.array_equal(common.bf16_round(x), bits)
+
+
+@pytest.mark.parametrize("qmode", [1, 2])
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0)])
+def test_sixteen_bit_store_bitexact(hx, qmode, ts, interior):
+    """FP16 / BF16 snapshot store (per-snapshot power-of-two max-abs scale,
+    filled one step late from u^n, the last snapshot flushed when TS = 1):
+    codes and max-abs slots equal the numpy restatement of the rule applied
+    to the oracle's FP32 snapshots, and the image equals the oracle's
+    backward pass over the read-back snapshots."""
+    cfg, g = _case("wide")
+    op = common.orc_params(cfg, time_slices=ts)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=np.float32, record=False)
+    deq, codes, mx = common.quantize_snapshots(S, qmode)
+    ds = DL.DevSim(hx, op, vpad, np.float32)
+    (q16, smax), _ = ds.forward_tiled(sx, sz, g["wavelet"], rz=4, interior=interior, seg=12, qmode=qmode)
+    assert np.array_equal(smax.max(axis=1), mx)
+    assert np.array_equal(q16, codes.transpose(0, 2, 1))
+    d = g["seismograms"][0]
+    I_ref, Is_ref, _, _ = O.backward(op, vpad, d, deq)
+    I2, Is2 = ds.backward_tiled(d, (q16, smax), rz=4, interior=interior, seg=12, qmode=qmode)
+    assert np.array_equal(I2, I_ref) and np.array_equal(Is2, Is_ref)

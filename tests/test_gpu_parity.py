@@ -196,6 +196,58 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
     assert np.allclose(var, stack.var(0, ddof=1), rtol=1e-10, atol=1e-12 * stack.var(0).max())
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0)])
+def test_sixteen_bit_snapshots_bitexact(gpu_available, mode, ts, interior):
+    """FP16 / BF16 snapshot store with per-snapshot max-abs scale: the image
+    equals the oracle's backward pass over the oracle's FP32 snapshots put
+    through the store's rule (numpy restatement), bit for bit; TS = 1 covers
+    the final-snapshot flush."""
+    say(f"[gpu] 16-bit store mode={mode} TS={ts} interior={interior}")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=ts)
+    p = common.rtm_params(cfg, time_slices=ts, snapshot_mode=mode)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+    deq, _, _ = common.quantize_snapshots(S, mode)
+    I, Is, _, _ = O.backward(op, vpad, seis, deq)
+    with R.Plan(p) as plan:
+        plan.set_tiling(4, 100, interior, 16)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I2, Is2 = plan.image()
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
+def test_batch_sixteen_bit_snapshots_bitexact(gpu_available):
+    """A batch with the FP16 store: each realization's image equals the
+    oracle's over its own quantized snapshots (each realization keeps its own
+    scales)."""
+    say("[gpu] batch FP16 store")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=1)
+    sx, sz = common.shot_positions(cfg)[0]
+    vpads = [O.extend(op, model * np.float32(1 + 0.05 * k)) for k in range(3)]
+    with R.Batch(p, len(vpads)) as b:
+        b.set_tiling(4, 100, 1, 16)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(wav)
+        b.set_seismogram(seis)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k, v in enumerate(vpads):
+            S, _, _, _ = O.forward(op, sx, sz, v, wav, record=False)
+            I, Is, _, _ = O.backward(op, v, seis, common.quantize_snapshots(S, 1)[0])
+            I2, Is2 = b.image(k)
+            assert np.array_equal(I2, I) and np.array_equal(Is2, Is), k
+
+
 def test_injection_outside_stepped_region(gpu_available):
     """Source and receivers on row 2 of the padded grid (outside the stepped
     region 4 <= k < Z-4, inside the top CPML): the per-point path with the
