@@ -65,6 +65,7 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
         append_work(tiles, inner, s, q->sims[s].p, rz, interior, seg_rows);
     q->ex.release();
     if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+    q->ex.q16 = q->sims[0].qmode != rtm::Q_NONE;
     if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = block;

@@ -62,7 +62,7 @@ RTM_HD Pair<R> col_update(const Pair<R> *w, Pair<R> L1, Pair<R> L2, Pair<R> R1, 
  * the 16-bit store's max |U| and quantization of u^n) and imaging for the
  * lane's two points at row k (in the order of tile_points); the row tests
  * are uniform across the wavefront.  uc = u^n at the two points. */
-template <typename R>
+template <typename R, bool Q16 = false>
 RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl, bool ok, int src_row, Pair<R> &U,
                        Pair<R> Cp, Pair<R> uc, float &umax)
 {
@@ -85,7 +85,7 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         }
     }
     const int kk = k - S.TU;
-    if (!(V.snap || V.qsnap || V.smax) || kk < 0 || kk >= S.nz) return;
+    if (!(Q16 ? (V.qsnap || V.smax) : V.snap != nullptr) || kk < 0 || kk >= S.nz) return;
     if (!ok) return;
     const R vals[2] = {U.a, U.b};
     const R us[2] = {uc.a, uc.b};
@@ -95,11 +95,14 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         if (ii < 0 || ii >= S.nx) continue;
         const int q = kk * S.nx + ii;
         if (V.forward) {
-            if (V.snap) stg(V.snap, q, vals[e]);
-            if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
-            if (V.qsnap) stg(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
+            if (!Q16) {
+                stg(V.snap, q, vals[e]);
+            } else {
+                if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
+                if (V.qsnap) stg(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
+            }
         } else {
-            const R s = V.snap ? ldg(V.snap, q) : R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            const R s = !Q16 ? ldg(V.snap, q) : R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
             if (S.flags & F_KAHAN) {
                 R I = ldg(S.I, q), Ic = ldg(S.Ic, q), Is = ldg(S.Is, q), Isc = ldg(S.Isc, q);
                 kahan_add(I, Ic, s * vals[e]);

@@ -80,6 +80,7 @@ static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, i
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
     q->ex.release();
     if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+    q->ex.q16 = q->b.qmode != rtm::Q_NONE;
     if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = graph_block;

@@ -77,23 +77,40 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
     }
 }
 
-template <typename R, int RZ, int CLS>
-__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S,
-                                         const rtm::StepView<R> &V, int x0, int z0, int tx, int ty)
+template <typename R, int RZ, int CLS, bool Q16>
+__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n, int x0,
+                                         int z0, int tx, int ty)
 {
+    const rtm::StepView<R> V = rtm::step_view_q<R, Q16>(S, n);
+    if (!V.active) return;
     rtm::TileRegs<R, RZ> G;
-    rtm::tile_fetch<R, RZ, CLS>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_fetch<R, RZ, CLS, Q16>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
         rtm::tile_psi<R, RZ, CLS>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS>(L, G, S, V, x0, z0, tx, ty);
-    if (V.smax) commit_snap_max_wg(V.smax, m, red);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16>(L, G, S, V, x0, z0, tx, ty);
+    if (Q16 && V.smax) commit_snap_max_wg(V.smax, m, red);
 }
 
-template <typename R, int RZ>
+template <typename R, int RZ, bool Q16>
+__device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n,
+                                             int4 tile, int tx, int ty)
+{
+    switch (tile.w) {
+    case 0: run_tile<R, RZ, 0, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    }
+}
+
+// Q16: every sim of the launch keeps a 16-bit snapshot store (the store
+// kind is a property of the whole plan or batch, so the FP32-store kernels
+// carry none of that code).
+template <typename R, int RZ, bool Q16>
 __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                const int *__restrict__ n0,
                                                const int4 *__restrict__ tiles,
@@ -110,16 +127,10 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     // read through the restrict pointer: uniform fields become scalar loads
     // that no store of the step can invalidate
     const SimDesc<R> &S = sims[tile.x];
-    const rtm::StepView<R> V = rtm::step_view(S, n0[tile.x] + s);
-    if (!V.active) return;
+    const int n = n0[tile.x] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    switch (tile.w) {
-    case 0: run_tile<R, RZ, 0>(L, red, S, V, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1>(L, red, S, V, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2>(L, red, S, V, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3>(L, red, S, V, tile.y, tile.z, tx, ty); break;
-    }
+    run_tile_cls<R, RZ, Q16>(L, red, S, n, tile, tx, ty);
 }
 
 // Wavefront shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
@@ -193,7 +204,7 @@ __device__ __forceinline__ void col_fetch(rtm::Pair<R> (&w)[16], ColSet<R> &st, 
 // Rows k0 .. k0+3; row k0+rr's window rows k-4 .. k+4 are slots
 // (B + rr + j) & 15, j = 0..8.  EXTRAS: some row of the chunk has a source,
 // receiver, snapshot or imaging term.
-template <typename R, int B, bool EXTRAS>
+template <typename R, int B, bool EXTRAS, bool Q16>
 __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const ColSet<R> &st, const ColCtx<R> &X,
                                           const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, float &umax)
 {
@@ -215,14 +226,14 @@ __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const Col
         R2.a = lane_shl1(R1.a, H.z);
         R2.b = lane_shl1(R1.b, H.w);
         rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, st.C[rr], st.O[rr]);
-        if (EXTRAS) rtm::col_extras(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr], c, umax);
+        if (EXTRAS) rtm::col_extras<R, Q16>(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr], c, umax);
         if (X.ok) rtm::stg(X.U2 + k * X.Ph, X.xh, U);
     }
 }
 
 // Chunk c of the four-chunk cycle (rows k0 .. k0+3): prefetch the next
 // chunk into the other set, then compute this one.
-template <typename R, int CI>
+template <typename R, int CI, bool Q16>
 __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, ColSet<R> &nxt, const ColCtx<R> &X,
                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, int zB,
                                          int ex_lo, int ex_hi, float &umax)
@@ -232,8 +243,8 @@ __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, 
     // source or receiver terms
     const bool extras = (k0 < ex_hi && k0 + 4 > ex_lo) || (X.srow >= k0 && X.srow < k0 + 4)
                         || (S.rec_row >= k0 && S.rec_row < k0 + 4 && (!V.forward || V.rec_row_out));
-    if (extras) col_chunk<R, 4 * CI, true>(w, cur, X, S, V, k0, umax);
-    else col_chunk<R, 4 * CI, false>(w, cur, X, S, V, k0, umax);
+    if (extras) col_chunk<R, 4 * CI, true, Q16>(w, cur, X, S, V, k0, umax);
+    else col_chunk<R, 4 * CI, false, Q16>(w, cur, X, S, V, k0, umax);
 }
 
 // The interior column kernel (rtm_column.h): four independent wavefronts
@@ -243,24 +254,12 @@ __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, 
 // unrolled by four chunks and nothing is copied between registers.  Blocks
 // are remapped so that each XCD walks a contiguous run of items
 // (consecutive segments of a column share rows).
-template <typename R>
-__global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ sims,
-                                                 const int *__restrict__ n0,
-                                                 const int4 *__restrict__ items,
-                                                 const int *__restrict__ nitems_p, int s)
+// One column item (the body of k_columns), for a sim without (Q16 false)
+// or with a 16-bit snapshot store.
+template <typename R, bool Q16>
+__device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it, int item)
 {
-    const int nitems = *nitems_p;
-    const int nblk = (nitems + 3) >> 2;
-    const int per = (nblk + 7) >> 3;
-    const int b = blockIdx.x;
-    const int t = (b & 7) * per + (b >> 3);
-    if ((b >> 3) >= per || t >= nblk) return;
-    const int item = t * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (item >= nitems) return;
-    const int4 it = items[item];
-    const int sim = rtm::col_sim(it.x);
-    const SimDesc<R> &S = sims[sim];
-    const rtm::StepView<R> V = rtm::step_view(S, n0[sim] + s);
+    const rtm::StepView<R> V = rtm::step_view_q<R, Q16>(S, n);
     if (!V.active) return;
     const int lane = threadIdx.x & 63;
     const int x0 = it.y, zA = it.z, zB = it.w;
@@ -277,7 +276,7 @@ __global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ 
     X.hq = rtm::clampi(lane == 63 ? x0 + 128 : x0 - 4, 0, P - 4) >> 2;
     X.ok = lane < (rtm::col_half(it.x) ? 32 : 64);
     X.srow = rtm::col_src_row(S, V);
-    const bool img = V.snap || V.qsnap || V.smax;
+    const bool img = Q16 ? (V.qsnap || V.smax) : V.snap != nullptr;
     const int ex_lo = img ? S.TU : 0, ex_hi = img ? S.TU + S.nz : 0;
     float umax = 0.0f;
     rtm::Pair<R> w[16];
@@ -286,16 +285,36 @@ __global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ 
     ColSet<R> A, Bs;
     col_fetch<R, 8>(w, A, X, zA, zB);
     for (int k0 = zA;; k0 += 16) {
-        col_step<R, 0>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi, umax);
+        col_step<R, 0, Q16>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi, umax);
         if (k0 + 4 >= zB) break;
-        col_step<R, 1>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi, umax);
+        col_step<R, 1, Q16>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi, umax);
         if (k0 + 8 >= zB) break;
-        col_step<R, 2>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi, umax);
+        col_step<R, 2, Q16>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi, umax);
         if (k0 + 12 >= zB) break;
-        col_step<R, 3>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi, umax);
+        col_step<R, 3, Q16>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi, umax);
         if (k0 + 16 >= zB) break;
     }
-    if (V.smax) commit_snap_max(V.smax, umax, item);
+    if (Q16 && V.smax) commit_snap_max(V.smax, umax, item);
+}
+
+template <typename R, bool Q16>
+__global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ sims,
+                                                 const int *__restrict__ n0,
+                                                 const int4 *__restrict__ items,
+                                                 const int *__restrict__ nitems_p, int s)
+{
+    const int nitems = *nitems_p;
+    const int nblk = (nitems + 3) >> 2;
+    const int per = (nblk + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= nblk) return;
+    const int item = t * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int4 it = items[item];
+    const int sim = rtm::col_sim(it.x);
+    const SimDesc<R> &S = sims[sim];
+    column_item<R, Q16>(S, n0[sim] + s, it, item);
 }
 
 // n0[i] += block for every simulation (the last node of a captured block)
@@ -501,6 +520,7 @@ struct Exec {
     int real = RTM_REAL_F32;
     int rz = 4;                         // rows per thread (tile height 4*rz)
     int interior = rtm::IN_TILES;       // kernel of the interior items
+    bool q16 = false;                   // the sims keep 16-bit snapshot stores
     int cap_sims = 0, cap_tiles = 0, cap_inner = 0;
     int nsims = 0, ntiles = 0, ninner = 0;
     void *d_desc = nullptr;             // SimDesc<R>[cap_sims]

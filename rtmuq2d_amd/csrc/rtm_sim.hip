@@ -469,7 +469,7 @@ int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
     return 0;
 }
 
-template <typename R>
+template <typename R, bool Q16>
 static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
 {
     const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
@@ -482,18 +482,20 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
         (void)hipStreamWaitEvent(e.side, e.ev_fork, 0);
         ts = e.side;
     }
-    if (e.ninner && e.interior == rtm::IN_COLUMNS) {
+    if (e.ninner) {
         // four items per workgroup
         const int cgrid = 8 * (((e.cap_inner + 3) / 4 + 7) / 8);
-        hipLaunchKernelGGL((k_columns<R>), dim3(cgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block);
+        hipLaunchKernelGGL((k_columns<R, Q16>), dim3(cgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner,
+                           s_in_block);
     }
-    if (!e.ntiles) return;
-    const int grid = 8 * ((e.cap_tiles + 7) / 8);
-    switch (e.rz) {
-    case 1: hipLaunchKernelGGL((k_tiles<R, 1>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    case 2: hipLaunchKernelGGL((k_tiles<R, 2>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    case 8: hipLaunchKernelGGL((k_tiles<R, 8>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-    default: hipLaunchKernelGGL((k_tiles<R, 4>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+    if (e.ntiles) {
+        const int grid = 8 * ((e.cap_tiles + 7) / 8);
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
     }
     if (fork) {
         (void)hipEventRecord(e.ev_join, e.side);
@@ -503,8 +505,9 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
 
 int Exec::launch_one(int s_in_block, hipStream_t s)
 {
-    if (real == RTM_REAL_F64) launch_tiles<double>(*this, s_in_block, s);
-    else launch_tiles<float>(*this, s_in_block, s);
+    if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
+    else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
+    else launch_tiles<float, false>(*this, s_in_block, s);
     return 0;
 }
 

@@ -230,8 +230,11 @@ struct StepView {
     unsigned *smax;         /* forward, 16-bit store: this step's snapshot max |U| */
 };
 
-template <typename R>
-RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
+/* Q16: the sim keeps a 16-bit snapshot store (SimDesc::qmode != 0).  The
+ * kernels dispatch on it per sim, so the FP32-store code carries none of the
+ * 16-bit fields. */
+template <typename R, bool Q16>
+RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 {
     StepView<R> v{};
     v.n = n;
@@ -250,7 +253,7 @@ RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
         v.src_w = v.src ? R(ldg(S.wav, n)) : R(0);
         v.rec_row_out = (S.flags & F_RECORD) ? S.seis_rec + (size_t)n * S.nrec : nullptr;
         const bool store = (S.flags & F_SNAP) && n % S.TS == 0;
-        if (S.qmode == Q_NONE) {
+        if (!Q16) {
             v.snap = store ? S.snaps + (size_t)(n / S.TS) * nint : nullptr;
         } else {
             v.smax = store ? S.smax + (size_t)(n / S.TS) * kSnapWords : nullptr;
@@ -264,7 +267,7 @@ RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
         const int tau = S.nt - 1 - n;
         v.seis_row = S.seis_t + (size_t)tau * S.nrec;
         const bool img = (S.flags & F_IMAGE) && tau % S.TS == 0;
-        if (S.qmode == Q_NONE) {
+        if (!Q16) {
             v.snap = img ? S.snaps + (size_t)(tau / S.TS) * nint : nullptr;
         } else if (img) {
             const int c = tau / S.TS;
@@ -273,6 +276,12 @@ RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
         }
     }
     return v;
+}
+
+template <typename R>
+RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
+{
+    return S.qmode ? step_view_q<R, true>(S, n) : step_view_q<R, false>(S, n);
 }
 
 template <typename R>
@@ -323,7 +332,7 @@ struct TileRegs {
 
 /* Phase 1a: every global load of the step.  tx in [0,64), ty in [0,4) (the
  * wavefront index: uniform within a wavefront).  CLS is the tile class. */
-template <typename R, int RZ, int CLS>
+template <typename R, int RZ, int CLS, bool Q16 = false>
 RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
                        int tx, int ty)
 {
@@ -385,13 +394,13 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
     }
-    if (!V.forward && (V.snap || V.qsnap)) {
+    if (!V.forward && (Q16 ? V.qsnap != nullptr : V.snap != nullptr)) {
         const int ii = clampi(i - S.TL, 0, S.nx - 1);
 #pragma unroll
         for (int rr = 0; rr < RZ; rr++) {
             const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
             const int q = kk * S.nx + ii;
-            if (V.snap) G.sv[rr] = ldg(V.snap, q);
+            if (!Q16) G.sv[rr] = ldg(V.snap, q);
             else G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
             G.Iv[rr] = ldg(S.I, q);
             G.Isv[rr] = ldg(S.Is, q);
@@ -471,7 +480,7 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-template <typename R, int RZ, int CLS>
+template <typename R, int RZ, int CLS, bool Q16 = false>
 RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
@@ -545,12 +554,15 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             }
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
-            if ((V.snap || V.qsnap || V.smax) && img_col && kk >= 0 && kk < S.nz) {
+            if ((Q16 ? (V.qsnap || V.smax) : V.snap != nullptr) && img_col && kk >= 0 && kk < S.nz) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
-                    if (V.snap) stg(V.snap, q, U);
-                    if (V.smax) umax = fmaxf(umax, fabsf(float(U)));
-                    if (V.qsnap) stg(V.qsnap, q, quant16(float(uc), V.qscale, S.qmode));
+                    if (!Q16) {
+                        stg(V.snap, q, U);
+                    } else {
+                        if (V.smax) umax = fmaxf(umax, fabsf(float(U)));
+                        if (V.qsnap) stg(V.qsnap, q, quant16(float(uc), V.qscale, S.qmode));
+                    }
                 } else if (S.flags & F_KAHAN) {
                     R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
                     const R s = G.sv[rr];

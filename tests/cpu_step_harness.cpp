@@ -42,13 +42,13 @@ static void commit_max(const rtm::StepView<R> &V, float m)
 
 // Emulation of k_tiles for one simulation at step n: every tile, phase by
 // phase, all 256 threads of a phase before the next (the barriers).
-template <typename R, int RZ, int CLS>
+template <typename R, int RZ, int CLS, bool Q16>
 static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm::SimDesc<R> &S,
                      const rtm::StepView<R> &V, int x0, int z0)
 {
     for (int ty = 0; ty < 4; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            rtm::tile_fetch<R, RZ, CLS>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            rtm::tile_fetch<R, RZ, CLS, Q16>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             rtm::tile_stage<R, RZ, CLS>(L, G[ty * 64 + tx], tx, ty);
         }
     if (CLS)
@@ -57,7 +57,7 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
     float m = 0.0f;
     for (int ty = 0; ty < 4; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            const float t = rtm::tile_points<R, RZ, CLS>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            const float t = rtm::tile_points<R, RZ, CLS, Q16>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             m = t > m ? t : m;
         }
     commit_max(V, m);
@@ -65,7 +65,7 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
 
 // Emulation of k_columns for one item: every row, all 64 lanes, with the
 // DPP lane shifts done as array shifts (lane 0 / 63 take the halo quad).
-template <typename R>
+template <typename R, bool Q16>
 static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V, int packed, int x0, int zA, int zB)
 {
     using P2 = rtm::Pair<R>;
@@ -96,7 +96,7 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
             const P2 Op = {at(V.U, k, xc[l]), at(V.U, k, xc[l] + 1)};
             P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, Op);
             const bool ok = l < (half ? 32 : 64);
-            rtm::col_extras(S, V, k, xl[l], ok, srow, U, Cp, w[4], umax);
+            rtm::col_extras<R, Q16>(S, V, k, xl[l], ok, srow, U, Cp, w[4], umax);
             if (ok) {
                 V.U[(size_t)k * P + xl[l]] = U.a;
                 V.U[(size_t)k * P + xl[l] + 1] = U.b;
@@ -108,8 +108,8 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
 
 // One step of one simulation over the work items of the library's builder
 // (rtm::build_work): tiles, plus column interior items.
-template <typename R, int RZ>
-static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
+template <typename R, int RZ, bool Q16>
+static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
 {
     const rtm::StepView<R> V = rtm::step_view(S, n);
     if (!V.active) return;
@@ -120,15 +120,22 @@ static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_row
     for (const auto &t : tiles) {
         // the LDS holds stale data from the previous tile, as on the GPU
         switch (t.d) {
-        case 0: run_tile<R, RZ, 0>(*L, G, S, V, t.b, t.c); break;
-        case 1: run_tile<R, RZ, 1>(*L, G, S, V, t.b, t.c); break;
-        case 2: run_tile<R, RZ, 2>(*L, G, S, V, t.b, t.c); break;
-        default: run_tile<R, RZ, 3>(*L, G, S, V, t.b, t.c); break;
+        case 0: run_tile<R, RZ, 0, Q16>(*L, G, S, V, t.b, t.c); break;
+        case 1: run_tile<R, RZ, 1, Q16>(*L, G, S, V, t.b, t.c); break;
+        case 2: run_tile<R, RZ, 2, Q16>(*L, G, S, V, t.b, t.c); break;
+        default: run_tile<R, RZ, 3, Q16>(*L, G, S, V, t.b, t.c); break;
         }
     }
-    for (const auto &it : inner) run_column_item<R>(S, V, it.a, it.b, it.c, it.d);
+    for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
     delete[] G;
     delete L;
+}
+
+template <typename R, int RZ>
+static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
+{
+    if (S.qmode) run_tiles_q<R, RZ, true>(S, n, interior, seg_rows);
+    else run_tiles_q<R, RZ, false>(S, n, interior, seg_rows);
 }
 
 template <typename R>
