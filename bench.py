@@ -143,6 +143,11 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-precision", action="store_true", help="skip the snapshot-precision variants")
+    ap.add_argument("--serve", type=int, default=200, help="requests in the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.5,
+                    help="arrival rate as a multiple of the one-at-a-time plain rate")
+    ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
+    ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--batch", type=int, default=64, help="realizations in the batched run (0: skip)")
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
     ap.add_argument("--profile", action="store_true",
@@ -161,6 +166,9 @@ def main() -> None:
         sys.exit(1)
     if args.profile:
         run_profile(R, args.profile_nt, args.profile_sims)
+        return
+    if args.only_serve:
+        print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb)), flush=True)
         return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
@@ -263,6 +271,8 @@ def main() -> None:
         out["batched"] = run_batch(R, p, v_true, w, seis, args.batch, args.batch_ts)
     if not args.no_precision:
         out["precision"] = run_precision(R, p, v_mig, w, seis, best[1])
+    if args.serve > 0:
+        out["serving"] = run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb)
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
@@ -355,6 +365,110 @@ def run_precision(R, p1, v_mig, w, seis, shape):
         log(f"[bench] c3 {name}: relL2 {v['relL2_vs_fp64']:.3e} max-abs {v['max_abs_err']:.3e} "
             f"({v['max_abs_err_rel_to_max']:.2e} of max) {dt * 1e3:.1f} ms/image")
     out["boundary_saving"] = "not built"
+    return out
+
+
+# c4 request classes: (label, nx, nz, h, dt, nt, CPML, Ricker f0); nt covers a
+# 3.2 s record at each dt
+SERVE_CLASSES = [
+    ("600x200 h10", 600, 200, 10.0, 0.001, 3200, 40, 15.0),
+    ("1001x301 h10", 1001, 301, 10.0, 0.0008, 4000, 50, 20.0),
+    ("1701x351 h10", 1701, 351, 10.0, 0.0008, 4000, 50, 20.0),
+    ("3401x701 h5", 3401, 701, 5.0, 0.0004, 8000, 50, 20.0),
+]
+
+
+def serve_params(R, cls):
+    label, nx, nz, h, dt, nt, T, f0 = cls
+    return R.make_params(
+        nx=nx, nz=nz, thickness_left=T, thickness_right=T, thickness_up=T, thickness_down=T,
+        spacing_grid=h, sampling=dt, number_of_time_steps=nt, sampling_wavelet=nt,
+        first_location_shot=nx // 2, depth_shot=2, spacing_shot=1, number_of_shots=1,
+        first_location_receiver=0, depth_receiver=2, spacing_receiver=2, number_of_receivers=nx,
+        number_of_simulations=1, zfp_tolerance=1e-4, time_slices=4, snapshot_mode=R.SNAP_FP16, image_kahan=1)
+
+
+def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
+    """Config c4: a Poisson stream of `nreq` migration requests, grid class
+    drawn uniformly from SERVE_CLASSES, each with its own seeded folded /
+    faulted velocity realization and its own source position along the
+    surface, FP16 snapshot store every 4 steps, Kahan image.  The traces are
+    one seeded synthetic gather per class (the cost of a migration does not
+    depend on their values).  Arrivals come at `rate_factor` times the rate
+    that one-at-a-time plain execution sustains; latency = completion -
+    arrival on the wall clock; the plain comparison uses each class's
+    measured one-at-a-time time in the same arrival order."""
+    from rtmuq2d_amd import serve, synth
+    rng = np.random.default_rng(seed)
+    ctx = []
+    for cls in SERVE_CLASSES:
+        p = serve_params(R, cls)
+        wav = synth.ricker(cls[7], p.sampling, p.sampling_wavelet)
+        g = np.random.default_rng(seed + cls[1])
+        seis = (g.standard_normal((p.number_of_receivers, p.number_of_time_steps)) * 1e-3).astype(np.float32)
+        ctx.append((p, wav, seis))
+    # one-at-a-time plain service time per class
+    service = []
+    for k, cls in enumerate(SERVE_CLASSES):
+        p, wav, seis = ctx[k]
+        v = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, seed)
+        with R.Plan(p) as plan:
+            plan.set_tiling(2, 100, 0, 0)
+            plan.set_velocity(R.extend(p, v))
+            plan.set_wavelet(wav)
+            plan.set_seismogram(seis)
+            plan.zero_image()
+            plan.migrate(p.nx // 2, 2)          # warm
+            plan.synchronize()
+            t0 = time.perf_counter()
+            plan.zero_image()
+            plan.migrate(p.nx // 2, 2)
+            plan.synchronize()
+            service.append(time.perf_counter() - t0)
+        log(f"[bench] c4 plain one-at-a-time {cls[0]}: {service[-1] * 1e3:.1f} ms, "
+            f"footprint {R.request_footprint(p) / 1e9:.2f} GB")
+    classes = rng.integers(0, len(SERVE_CLASSES), nreq)
+    plain_rate = nreq / float(np.sum([service[c] for c in classes]))
+    rate = rate_factor * plain_rate
+    arrivals = serve.poisson_arrivals(nreq, rate, seed)
+
+    def maker(i, c):
+        def make():
+            p, wav, seis = ctx[c]
+            v = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, 1000 + i)
+            sx = int(np.random.default_rng(5000 + i).integers(0, p.nx))
+            return p, R.extend(p, v), wav, seis, sx, 2
+        return make
+
+    reqs = [serve.Request(float(arrivals[i]), maker(i, int(classes[i])), SERVE_CLASSES[classes[i]][0])
+            for i in range(nreq)]
+    with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
+        srv.set_tiling(4, 100, 0, 0)
+        res = serve.run(srv, reqs)
+    seq = serve.sequential_latencies(arrivals, [service[c] for c in classes])
+    seq_wall = max(a + l for a, l in zip(sorted(arrivals), seq))
+    out = {
+        "workload": (f"c4 serving: {nreq} Poisson requests at {rate:.2f}/s ({rate_factor}x the one-at-a-time "
+                     "plain rate), classes " + ", ".join(c[0] for c in SERVE_CLASSES)
+                     + "; FP16 snapshots every 4 steps, Kahan image"),
+        "requests": nreq, "arrival_rate_per_s": round(rate, 3),
+        "class_counts": [int((classes == k).sum()) for k in range(len(SERVE_CLASSES))],
+        "latency_s": {k: round(v, 4) for k, v in res["latency_s"].items()},
+        "images_per_hour": round(res["images_per_hour"], 1),
+        "wall_seconds": round(res["wall_seconds"], 3),
+        "max_active": res["max_active"], "max_queued": res["max_queued"],
+        "plain_sequential": {
+            "service_ms_per_class": {SERVE_CLASSES[k][0]: round(service[k] * 1e3, 2) for k in range(len(service))},
+            "images_per_hour": round(nreq * 3600.0 / seq_wall, 1),
+            "latency_s": {k: round(v, 4) for k, v in serve.percentiles(seq).items()},
+        },
+    }
+    out["served_over_plain"] = round(out["images_per_hour"] / out["plain_sequential"]["images_per_hour"], 3)
+    log(f"[bench] c4 served {nreq} requests in {res['wall_seconds']:.1f} s: {out['images_per_hour']:.0f} "
+        f"images/h, p50/p95/p99 {out['latency_s']['p50']:.2f}/{out['latency_s']['p95']:.2f}/"
+        f"{out['latency_s']['p99']:.2f} s; plain sequential {out['plain_sequential']['images_per_hour']:.0f} "
+        f"images/h, p50/p95/p99 {out['plain_sequential']['latency_s']['p50']:.2f}/"
+        f"{out['plain_sequential']['latency_s']['p95']:.2f}/{out['plain_sequential']['latency_s']['p99']:.2f} s")
     return out
 
 

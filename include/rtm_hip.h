@@ -204,6 +204,40 @@ int rtm_batch_get_image_f64(rtm_batch *batch, int sim, double *I, double *Is);
 int rtm_batch_welford(rtm_batch *batch, double *mean, double *var);
 int rtm_batch_last_timing(rtm_batch *batch, double *fwd_step_ms, double *bwd_step_ms, int *steps);
 
+/* ---------------------------------------------------------------------------
+ * 3. Request server (no reference counterpart): continuous batching of
+ *    migrations of any grid size on one device.
+ * ------------------------------------------------------------------------- */
+
+typedef struct rtm_server rtm_server;
+
+/* A server owning an HBM arena of hbm_bytes and max_slots request slots;
+ * every request must use snapshot_mode (enum rtm_snapshot_mode). */
+int rtm_server_create(int device, size_t hbm_bytes, int max_slots, int snapshot_mode, rtm_server **out);
+int rtm_server_destroy(rtm_server *server);
+/* Kernel shape shared by all requests (as rtm_plan_set_tiling); graph_block
+ * is also the number of time steps per tick. */
+int rtm_server_set_tiling(rtm_server *server, int rows_per_thread, int graph_block, int interior, int seg_rows);
+/* Arena bytes a request with params p occupies while active. */
+size_t rtm_server_request_footprint(const rtm_params *p);
+/* Queue one migration: params (grid, time, receivers; snapshot_mode as the
+ * server's), padded velocity [X][Z], wavelet [sampling_wavelet], seismogram
+ * [nrec][nt] and the shot position.  The inputs are copied.  Returns the
+ * request id (>= 0), or < 0 when the request can never run (the source or a
+ * receiver outside the stepped region, or larger than the arena). */
+int rtm_server_submit(rtm_server *server, const rtm_params *p, const float *vpad, const float *wavelet,
+                      const float *seismogram, int shot_x, int shot_z);
+/* Admit queued requests that fit (arrival order), advance every active
+ * request by one block of time steps in shared launches, switch finished
+ * forward passes to their backward pass and retire finished requests.
+ * Returns the number of requests still queued or active (< 0 on error). */
+int rtm_server_tick(rtm_server *server);
+/* 1 and *id = the oldest completed request not yet taken, or 0 if none. */
+int rtm_server_next_done(rtm_server *server, int *id);
+/* Copy a completed request's image ([Nx][Nz], n = Nx*Nz floats) and forget it. */
+int rtm_server_take_image(rtm_server *server, int id, float *image, size_t n);
+int rtm_server_stats(rtm_server *server, int *active, int *queued, size_t *bytes_in_use, long long *steps);
+
 #ifdef __cplusplus
 }
 #endif

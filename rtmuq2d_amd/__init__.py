@@ -142,6 +142,15 @@ def lib() -> C.CDLL:
         "rtm_batch_get_image_f64": (C.c_int, [plan_p, C.c_int, f64p, f64p]),
         "rtm_batch_welford": (C.c_int, [plan_p, f64p, f64p]),
         "rtm_batch_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
+        "rtm_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, P(plan_p)]),
+        "rtm_server_destroy": (C.c_int, [plan_p]),
+        "rtm_server_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "rtm_server_request_footprint": (C.c_size_t, [P(RtmParams)]),
+        "rtm_server_submit": (C.c_int, [plan_p, P(RtmParams), f32p, f32p, f32p, C.c_int, C.c_int]),
+        "rtm_server_tick": (C.c_int, [plan_p]),
+        "rtm_server_next_done": (C.c_int, [plan_p, i32p]),
+        "rtm_server_take_image": (C.c_int, [plan_p, C.c_int, f32p, C.c_size_t]),
+        "rtm_server_stats": (C.c_int, [plan_p, i32p, i32p, P(C.c_size_t), P(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -401,3 +410,84 @@ class Batch:
 
 def batch_footprint(p: RtmParams, nsims: int, real: int = REAL_F32) -> int:
     return int(lib().rtm_batch_footprint(C.byref(p), int(nsims), real))
+
+
+def request_footprint(p: RtmParams) -> int:
+    """HBM bytes a request with params p occupies in a Server while active."""
+    return int(lib().rtm_server_request_footprint(C.byref(p)))
+
+
+class Server:
+    """The request server (rtm_server_*): migrations of any grid size are
+    queued with submit(), admitted by HBM footprint and advanced together by
+    tick() (one block of time steps per call, continuous batching); finished
+    images are collected with done()."""
+
+    def __init__(self, hbm_bytes: int, max_slots: int = 64, snapshot_mode: int = SNAP_FP16, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().rtm_server_create(int(device), int(hbm_bytes), int(max_slots), int(snapshot_mode),
+                                       C.byref(h)), "rtm_server_create")
+        self.h = h
+        self.snapshot_mode = snapshot_mode
+        self.params = {}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().rtm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, interior: int = 0,
+                   seg_rows: int = 0) -> None:
+        _check(lib().rtm_server_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
+                                           int(seg_rows)), "set_tiling")
+
+    def submit(self, p: RtmParams, vpad: np.ndarray, wavelet: np.ndarray, seis: np.ndarray, shot_x: int,
+               shot_z: int) -> int:
+        q = p.copy().update(snapshot_mode=self.snapshot_mode)
+        v = np.ascontiguousarray(vpad, dtype=np.float32)
+        assert v.shape == (q.X, q.Z), v.shape
+        w = np.ascontiguousarray(wavelet, dtype=np.float32)
+        assert w.size >= q.sampling_wavelet
+        d = np.ascontiguousarray(seis, dtype=np.float32)
+        assert d.shape == (q.number_of_receivers, q.number_of_time_steps), d.shape
+        rid = lib().rtm_server_submit(self.h, C.byref(q), fptr(v), fptr(w), fptr(d), int(shot_x), int(shot_z))
+        if rid < 0:
+            _check(rid, "rtm_server_submit")
+        self.params[rid] = (q.nx, q.nz)
+        return rid
+
+    def tick(self) -> int:
+        n = lib().rtm_server_tick(self.h)
+        if n < 0:
+            _check(n, "rtm_server_tick")
+        return n
+
+    def done(self):
+        """Completed requests not yet collected: [(id, image [Nx][Nz])]."""
+        out = []
+        rid = C.c_int()
+        while lib().rtm_server_next_done(self.h, C.byref(rid)) == 1:
+            nx, nz = self.params.pop(rid.value)
+            I = np.empty((nx, nz), np.float32)
+            _check(lib().rtm_server_take_image(self.h, rid.value, fptr(I), I.size), "take_image")
+            out.append((rid.value, I))
+        return out
+
+    def stats(self):
+        a, q = C.c_int(), C.c_int()
+        b = C.c_size_t()
+        st = C.c_longlong()
+        _check(lib().rtm_server_stats(self.h, C.byref(a), C.byref(q), C.byref(b), C.byref(st)), "stats")
+        return {"active": a.value, "queued": q.value, "bytes_in_use": b.value, "steps": st.value}

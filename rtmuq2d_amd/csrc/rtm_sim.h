@@ -448,6 +448,26 @@ void from_device_layout(const Src *dev, Dst *ref, int X, int Z, int P)
 }
 
 // ---------------------------------------------------------------------------
+// device memory arena
+// ---------------------------------------------------------------------------
+
+// One hipMalloc up front, first-fit sub-allocation in 256-byte units with
+// coalescing on free, so that simulations come and go (the request server)
+// without hipMalloc / hipFree, the latter of which synchronises the device.
+struct DeviceArena {
+    char *base = nullptr;
+    size_t size = 0, in_use = 0;
+    std::map<size_t, size_t> free_;     // offset -> bytes
+    std::map<size_t, size_t> live_;     // offset -> bytes
+    int init(size_t bytes);
+    void release();
+    void *alloc(size_t bytes);           // nullptr when no free block is large enough
+    void free(void *ptr);
+    size_t largest_free() const;
+    static size_t rounded(size_t bytes) { return (bytes + 255) / 256 * 256; }
+};
+
+// ---------------------------------------------------------------------------
 // one simulation's device buffers
 // ---------------------------------------------------------------------------
 
@@ -478,6 +498,7 @@ struct SimBuffers {
     std::vector<float> wav_host;
     bool have_velocity = false, have_seis = false, have_snaps = false;
     std::vector<void *> owned;
+    DeviceArena *arena = nullptr;       // allocate from here instead of hipMalloc
 
     // `share` (same geometry): take its profiles, receiver tables, wavelet
     // and trace buffers instead of allocating copies (an ensemble over one

@@ -3,6 +3,71 @@
 
 namespace rtmd {
 
+int DeviceArena::init(size_t bytes)
+{
+    release();
+    size = rounded(bytes);
+    HIP_TRY(hipMalloc((void **)&base, size));
+    free_[0] = size;
+    return 0;
+}
+
+void DeviceArena::release()
+{
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    size = in_use = 0;
+    free_.clear();
+    live_.clear();
+}
+
+void *DeviceArena::alloc(size_t bytes)
+{
+    const size_t need = rounded(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < need) continue;
+        const size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > need) free_[off + need] = len - need;
+        live_[off] = need;
+        in_use += need;
+        return base + off;
+    }
+    return nullptr;
+}
+
+void DeviceArena::free(void *ptr)
+{
+    if (!ptr || !base) return;
+    const size_t off = (size_t)((char *)ptr - base);
+    auto lv = live_.find(off);
+    if (lv == live_.end()) return;
+    size_t start = off, len = lv->second;
+    in_use -= len;
+    live_.erase(lv);
+    auto nx = free_.lower_bound(start);
+    if (nx != free_.end() && start + len == nx->first) {   // merge with the next block
+        len += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {                            // merge with the previous block
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == start) {
+            start = pv->first;
+            len += pv->second;
+            free_.erase(pv);
+        }
+    }
+    free_[start] = len;
+}
+
+size_t DeviceArena::largest_free() const
+{
+    size_t m = 0;
+    for (const auto &f : free_) m = f.second > m ? f.second : m;
+    return m;
+}
+
 size_t sim_bytes(const rtm_params &p, size_t esz)
 {
     const int X = p.thickness_left + p.nx + p.thickness_right;
@@ -39,10 +104,18 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
     nsnap = (size_t)(p.number_of_time_steps / p.time_slices);
     const size_t nrec = (size_t)p.number_of_receivers, nt = (size_t)p.number_of_time_steps;
     auto get = [&](void **ptr, size_t bytes) -> int {
-        hipError_t e = hipMalloc(ptr, bytes ? bytes : 4);
-        if (e != hipSuccess) {
-            rtm_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-            return -1;
+        if (arena) {
+            *ptr = arena->alloc(bytes ? bytes : 4);
+            if (!*ptr) {
+                rtm_set_error("device arena: no free block of %zu bytes", bytes);
+                return -1;
+            }
+        } else {
+            hipError_t e = hipMalloc(ptr, bytes ? bytes : 4);
+            if (e != hipSuccess) {
+                rtm_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+                return -1;
+            }
         }
         owned.push_back(*ptr);
         return 0;
@@ -117,7 +190,10 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
 
 void SimBuffers::release()
 {
-    for (void *ptr : owned) (void)hipFree(ptr);
+    for (void *ptr : owned) {
+        if (arena) arena->free(ptr);
+        else (void)hipFree(ptr);
+    }
     owned.clear();
     u[0] = u[1] = C = psiX[0] = psiX[1] = psiZ[0] = psiZ[1] = csiX = csiZ = nullptr;
     ax = bx = az = bz = nullptr;

@@ -449,3 +449,84 @@ def test_c0_image_matches_reference(gpu_available):
     assert sha(b) == meta["stackedBrute"][0], f"relL2 vs native {common.rel_l2(b, native)}"
     assert sha(lap) == meta["stackedLaplacian"][0]
     assert common.rel_l2(b, native) <= 1e-5
+
+
+def _server_requests():
+    """Requests on three grids (tiny, asym, wide), each with its own velocity,
+    shot position and seismogram, FP16 snapshot store every 4 steps."""
+    from rtmuq2d_amd import synth
+    rng = np.random.default_rng(77)
+    out = []
+    for k, name in enumerate(["tiny", "asym", "wide", "tiny", "asym"]):
+        if name == "wide":
+            cfg, model, wav, seis = _wide_case()
+        else:
+            cfg = common.REF_CONFIGS[name]
+            g = golden(name)
+            model = (g["models"][0] * np.float32(1 + 0.04 * k)).astype(np.float32)
+            wav = g["wavelet"]
+            seis = rng.standard_normal((cfg["numberOfReceivers"], cfg["numberOfTimeStep"])).astype(np.float32)
+        op = common.orc_params(cfg, time_slices=4)
+        p = common.rtm_params(cfg, time_slices=4, snapshot_mode=1)
+        sx = int(rng.integers(6, cfg["Nx"] - 6))
+        sz = common.shot_positions(cfg)[0][1]
+        out.append((cfg, op, p, O.extend(op, model), wav, seis, sx, sz))
+    return out
+
+
+def test_server_each_result_same_alone_and_in_the_mix(gpu_available):
+    """Continuous batching: requests of three grid sizes arrive while others
+    are in flight, are admitted by HBM footprint (the arena holds about two at
+    a time), share every launch, and each image equals the oracle's for that
+    request alone (FP16 store rule applied), bit for bit; so it is also
+    independent of the mix."""
+    say("[gpu] request server: alone vs in the mix")
+    reqs = _server_requests()
+    refs = []
+    for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
+        S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+        refs.append(O.backward(op, vpad, seis, common.quantize_snapshots(S, 1)[0])[0])
+    fp = [R.request_footprint(r[2]) for r in reqs]
+    budget = int(sorted(fp)[-1] + sorted(fp)[-2] + (1 << 20))
+    got = {}
+    with R.Server(budget, max_slots=4, snapshot_mode=1) as srv:
+        srv.set_tiling(4, 50, 1, 16)
+        ids = []
+        ids.append(srv.submit(*_sub(reqs[0])))
+        ids.append(srv.submit(*_sub(reqs[1])))
+        srv.tick()
+        ids.append(srv.submit(*_sub(reqs[2])))          # arrives while two are in flight
+        srv.tick()
+        st = srv.stats()
+        assert st["active"] <= 2 and st["bytes_in_use"] <= budget
+        ids.append(srv.submit(*_sub(reqs[3])))
+        ids.append(srv.submit(*_sub(reqs[4])))
+        while srv.tick() > 0:
+            for rid, I in srv.done():
+                got[rid] = I
+        for rid, I in srv.done():
+            got[rid] = I
+    assert sorted(got) == sorted(ids)
+    for k, rid in enumerate(ids):
+        assert np.array_equal(got[rid], refs[k]), k
+
+
+def _sub(req):
+    cfg, op, p, vpad, wav, seis, sx, sz = req
+    return p, vpad, wav, seis, sx, sz
+
+
+def test_server_rejects_requests_it_can_never_run(gpu_available):
+    """Requests larger than the arena, with the source outside the stepped
+    region, or with another snapshot store are refused at submission."""
+    cfg, op, p, vpad, wav, seis, sx, sz = _server_requests()[2]
+    assert R.request_footprint(p) > 2 << 20
+    with R.Server(R.request_footprint(p) // 2, max_slots=2, snapshot_mode=1) as srv:
+        with pytest.raises(R.RtmError, match="arena"):
+            srv.submit(p, vpad, wav, seis, sx, sz)
+    with R.Server(4 * R.request_footprint(p), max_slots=2, snapshot_mode=1) as srv:
+        with pytest.raises(R.RtmError, match="outside"):
+            srv.submit(p, vpad, wav, seis, sx, -cfg["thicknessUp"])
+        srv.snapshot_mode = 0
+        with pytest.raises(R.RtmError, match="snapshot mode"):
+            srv.submit(p, vpad, wav, seis, sx, sz)
