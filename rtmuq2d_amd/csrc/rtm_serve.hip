@@ -211,8 +211,10 @@ static int server_admit(rtm_server *s)
             rtm_set_error("");
             break;
         }
-        if (r.b.set_velocity(r.vpad.data(), s->stream) || r.b.set_wavelet(r.wav.data(), (int)r.wav.size()) ||
-            r.b.set_seismogram(r.seis.data(), s->stream))
+        // the host inputs stay with the request until it completes, so the
+        // uploads need not wait
+        if (r.b.set_velocity(r.vpad.data(), s->stream, false) || r.b.set_wavelet(r.wav.data(), (int)r.wav.size()) ||
+            r.b.set_seismogram(r.seis.data(), s->stream, false))
             return -1;
         r.b.reset_state(s->stream);
         r.b.reset_snap_scales(s->stream);
@@ -222,8 +224,6 @@ static int server_admit(rtm_server *s)
         r.n = 0;
         s->descs[slot] = forward_desc(r);
         s->ex.h_n0[slot] = 0;
-        r.vpad.clear(); r.vpad.shrink_to_fit();
-        r.seis.clear(); r.seis.shrink_to_fit();
         s->slots[slot] = std::move(s->queue.front());
         s->queue.pop_front();
         s->lists_dirty = s->descs_dirty = true;
@@ -264,6 +264,8 @@ static int server_finish(rtm_server *s, int slot)
     for (int i = 0; i < nx; i++)
         for (int k = 0; k < nz; k++) r->image[(size_t)i * nz + k] = h[(size_t)k * nx + i];
     r->b.release();
+    r->vpad.clear(); r->vpad.shrink_to_fit();
+    r->seis.clear(); r->seis.shrink_to_fit();
     r->slot = -1;
     s->descs[slot] = SimDesc<float>{};
     s->done.push_back(std::move(r));

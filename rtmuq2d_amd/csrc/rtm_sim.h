@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "rtm_internal.h"
@@ -372,17 +373,16 @@ inline void launch_top_edges(const StepArgs<R> &A, hipStream_t s)
     hipLaunchKernelGGL(k_top_edges<R>, dim3((n + 63) / 64), dim3(64), 0, s, A);
 }
 
-// coef = (v*aux)*(v*aux) on the padded velocity (already in device layout),
-// SRC/forward_modeling.c:79.
+// coef = (v*aux)*(v*aux) from the padded velocity in the reference's layout
+// ([X][Z], z fastest) into the device layout, SRC/forward_modeling.c:79.
 template <typename R>
 __global__ void k_coef(const float *__restrict__ v, R *__restrict__ C, R aux, int X, int Z, int P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (i >= X || k >= Z) return;
-    const size_t j = (size_t)k * P + i;
-    const R vv = R(v[j]);
-    C[j] = (vv * aux) * (vv * aux);
+    const R vv = R(v[(size_t)i * Z + k]);
+    C[(size_t)k * P + i] = (vv * aux) * (vv * aux);
 }
 
 // receiver replacement at points outside the stepped region,
@@ -507,9 +507,11 @@ struct SimBuffers {
     void release();
     int upload_profiles();
     int upload_receivers();
-    int set_velocity(const float *vpad, hipStream_t s);
+    // uploads; with wait == false the host data must stay alive until the
+    // stream has run the copy (the caller synchronises later)
+    int set_velocity(const float *vpad, hipStream_t s, bool wait = true);
     int set_wavelet(const float *w, int n);
-    int set_seismogram(const float *d, hipStream_t s);
+    int set_seismogram(const float *d, hipStream_t s, bool wait = true);
     void reset_state(hipStream_t s);      // zero u, psi, csi (a pass starts at rest)
     void reset_snap_scales(hipStream_t s); // zero the 16-bit store's max |u| slots
     // the 16-bit store's last snapshot when it is the pass's final field
@@ -554,6 +556,11 @@ struct Exec {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int gblock = 0;
+    // the captured launches depend on the lists only through their
+    // capacities (grid sizes) and on which lists are non-empty (the kernels
+    // read the items and their counts from device memory), so new list
+    // contents keep the graph unless that changes
+    bool g_tiles = false, g_inner = false;
     // the strip tiles run on `side` beside the interior kernel (they write
     // disjoint points); fork/join through two events, also under capture
     hipStream_t side = nullptr;
@@ -572,6 +579,29 @@ struct Exec {
     int launch_one(int s_in_block, hipStream_t s);
     int drop_graph();
 };
+
+// ---------------------------------------------------------------------------
+// autotuner: candidate kernel shapes, timing, and a per-grid-class cache
+// ---------------------------------------------------------------------------
+
+struct Shape {
+    int rz, interior, seg;
+};
+
+// the shapes tried: tile heights 4..32 rows, and the column kernel beside
+// strip tiles with whole or segmented columns
+const std::vector<Shape> &tuner_candidates();
+
+// cache key of a launch: element type, store kind and every sim's grid
+std::string tuner_key(int real, bool q16, const std::vector<const rtm_params *> &grids);
+bool tuner_lookup(const std::string &key, Shape *out);
+void tuner_store(const std::string &key, const Shape &s);
+
+// ms per step of `steps` forward steps of `descs` (from rest, no source, no
+// stores) with the executor's current work lists; descriptors and step
+// counters are left as uploaded here, so the caller re-uploads its own
+template <typename R>
+int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms);
 
 }  // namespace rtmd
 

@@ -1,4 +1,6 @@
 // rtm_sim.hip -- per-simulation device buffers and the tiled step executor.
+#include <mutex>
+
 #include "rtm_sim.h"
 
 namespace rtmd {
@@ -271,32 +273,31 @@ int SimBuffers::upload_receivers()
     return 0;
 }
 
+// The padded velocity arrives in the reference's layout ([X][Z], z fastest)
+// and is staged in u[0] (X*Z floats fit in the field, and every pass zeroes
+// u before stepping); k_coef reads it transposed.  No allocation and no
+// host-side transposition or wait, so requests can be admitted while others
+// run.  The wavefields read back before the next pass are overwritten.
 template <typename R>
-static int set_velocity_t(SimBuffers &q, const float *vpad, hipStream_t s)
+static int set_velocity_t(SimBuffers &q, const float *vpad, hipStream_t s, bool wait)
 {
-    std::vector<float> v(q.nfield);
-    to_device_layout(vpad, v.data(), q.X, q.Z, q.P);
-    float *dv = nullptr;
-    HIP_TRY(hipMalloc(&dv, sizeof(float) * q.nfield));
-    hipError_t e = hipMemcpyAsync(dv, v.data(), sizeof(float) * q.nfield, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) {
-        // aux = dt/h rounded in R, as SRC/forward_modeling.c:71
-        const R aux = sizeof(R) == 4 ? R(q.p.sampling / q.p.spacing_grid)
-                                     : R((double)q.p.sampling / (double)q.p.spacing_grid);
-        hipLaunchKernelGGL(k_coef<R>, dim3((q.X + 255) / 256, q.Z), dim3(256), 0, s, dv, (R *)q.C, aux, q.X,
-                           q.Z, q.P);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(dv);
-    if (e != hipSuccess) { rtm_set_error("set_velocity: %s", hipGetErrorString(e)); return -1; }
+    float *stage = (float *)q.u[0];
+    HIP_TRY(hipMemcpyAsync(stage, vpad, sizeof(float) * (size_t)q.X * q.Z, hipMemcpyHostToDevice, s));
+    // aux = dt/h rounded in R, as SRC/forward_modeling.c:71
+    const R aux = sizeof(R) == 4 ? R(q.p.sampling / q.p.spacing_grid)
+                                 : R((double)q.p.sampling / (double)q.p.spacing_grid);
+    hipLaunchKernelGGL(k_coef<R>, dim3((q.X + 255) / 256, q.Z), dim3(256), 0, s, stage, (R *)q.C, aux, q.X, q.Z,
+                       q.P);
+    HIP_TRY(hipGetLastError());
+    if (wait) HIP_TRY(hipStreamSynchronize(s));
     q.have_velocity = true;
     return 0;
 }
 
-int SimBuffers::set_velocity(const float *vpad, hipStream_t s)
+int SimBuffers::set_velocity(const float *vpad, hipStream_t s, bool wait)
 {
-    return real == RTM_REAL_F64 ? set_velocity_t<double>(*this, vpad, s) : set_velocity_t<float>(*this, vpad, s);
+    return real == RTM_REAL_F64 ? set_velocity_t<double>(*this, vpad, s, wait)
+                                : set_velocity_t<float>(*this, vpad, s, wait);
 }
 
 int SimBuffers::set_wavelet(const float *w, int n)
@@ -310,14 +311,33 @@ int SimBuffers::set_wavelet(const float *w, int n)
     return 0;
 }
 
-int SimBuffers::set_seismogram(const float *d, hipStream_t s)
+// [nrec][nt] -> [nt][nrec] through 32 x 32 LDS tiles
+__global__ void k_transpose(const float *__restrict__ in, float *__restrict__ out, int rows, int cols)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int r = r0 + j, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) t[j][threadIdx.x] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += 8) {
+        const int c = c0 + j, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = t[threadIdx.x][j];
+    }
+}
+
+// The traces arrive [nrec][nt] and are staged in the recording buffer (the
+// forward pass's recorded traces are overwritten), then transposed on the
+// device: no host-side transposition or wait.
+int SimBuffers::set_seismogram(const float *d, hipStream_t s, bool wait)
 {
     const int nrec = p.number_of_receivers, nt = p.number_of_time_steps;
-    std::vector<float> t((size_t)nt * nrec);
-    for (int r = 0; r < nrec; r++)
-        for (int n = 0; n < nt; n++) t[(size_t)n * nrec + r] = d[(size_t)r * nt + n];
-    HIP_TRY(hipMemcpyAsync(seis_t, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpyAsync(seis_rec, d, sizeof(float) * (size_t)nrec * nt, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_transpose, dim3((nt + 31) / 32, (nrec + 31) / 32), dim3(32, 8), 0, s, seis_rec, seis_t,
+                       nrec, nt);
+    HIP_TRY(hipGetLastError());
+    if (wait) HIP_TRY(hipStreamSynchronize(s));
     have_seis = true;
     return 0;
 }
@@ -508,7 +528,7 @@ int Exec::set_inner(const std::vector<int4> &items, hipStream_t s)
     if (ninner) HIP_TRY(hipMemcpyAsync(d_inner, items.data(), sizeof(int4) * ninner, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ninner, &ninner, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    drop_graph();
+    if (gexec && (ninner > 0) != g_inner) drop_graph();
     return 0;
 }
 
@@ -522,7 +542,7 @@ int Exec::set_tiles(const std::vector<int4> &tiles, hipStream_t s)
     if (ntiles) HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * ntiles, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    drop_graph();
+    if (gexec && (ntiles > 0) != g_tiles) drop_graph();
     return 0;
 }
 
@@ -601,6 +621,8 @@ int Exec::run(int steps, int block, hipStream_t s)
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
             HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
             gblock = block;
+            g_tiles = ntiles > 0;
+            g_inner = ninner > 0;
         }
         while (steps - done >= block) {
             HIP_TRY(hipGraphLaunch(gexec, s));
@@ -614,5 +636,82 @@ int Exec::run(int steps, int block, hipStream_t s)
     for (int i = 0; i < cap_sims; i++) h_n0[i] += steps;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+
+const std::vector<Shape> &tuner_candidates()
+{
+    static const std::vector<Shape> c = {
+        {1, rtm::IN_TILES, 0}, {2, rtm::IN_TILES, 0}, {4, rtm::IN_TILES, 0}, {8, rtm::IN_TILES, 0},
+        {4, rtm::IN_COLUMNS, 0}, {4, rtm::IN_COLUMNS, 64}, {2, rtm::IN_COLUMNS, 32}, {1, rtm::IN_COLUMNS, 8},
+    };
+    return c;
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, Shape> g_tune;
+}
+
+std::string tuner_key(int real, bool q16, const std::vector<const rtm_params *> &grids)
+{
+    std::string k = std::to_string(real) + (q16 ? "q" : "f");
+    for (const rtm_params *p : grids) {
+        char b[96];
+        snprintf(b, sizeof b, "|%d,%d,%d,%d,%d,%d", p->nx, p->nz, p->thickness_left, p->thickness_right,
+                 p->thickness_up, p->thickness_down);
+        k += b;
+    }
+    return k;
+}
+
+bool tuner_lookup(const std::string &key, Shape *out)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void tuner_store(const std::string &key, const Shape &s)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = s;
+}
+
+template <typename R>
+int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms)
+{
+    for (auto &d : descs) {
+        d.phase = rtm::PH_FORWARD;
+        d.flags = 0;
+        d.src_j = -1;
+    }
+    if (ex.upload_descs(descs.data(), (int)descs.size(), s)) return -1;
+    if (ex.set_n0(std::vector<int>(descs.size(), 0), s)) return -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = ex.run(block > 1 ? block : 10, block, s);      // warm (and capture)
+    if (!rc) rc = ex.set_n0(std::vector<int>(descs.size(), 0), s);
+    if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = -1;
+    if (!rc) rc = ex.run(steps, block, s);
+    float t = 0.f;
+    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess)) {
+        rtm_set_error("time_steps: event failure");
+        rc = -1;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / (steps > 0 ? steps : 1);
+    return rc;
+}
+
+template int time_steps<float>(Exec &, std::vector<SimDesc<float>>, int, int, hipStream_t, double *);
+template int time_steps<double>(Exec &, std::vector<SimDesc<double>>, int, int, hipStream_t, double *);
 
 }  // namespace rtmd

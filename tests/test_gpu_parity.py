@@ -530,3 +530,31 @@ def test_server_rejects_requests_it_can_never_run(gpu_available):
         srv.snapshot_mode = 0
         with pytest.raises(R.RtmError, match="snapshot mode"):
             srv.submit(p, vpad, wav, seis, sx, sz)
+
+
+def test_autotune_keeps_results_and_caches_per_grid_class(gpu_available):
+    """The autotuner times every candidate shape, keeps the fastest (the
+    image stays bit-identical to the oracle's), and a later plan of the same
+    grid class starts with that shape."""
+    say("[gpu] autotune")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=10)
+    p = common.rtm_params(cfg, time_slices=10)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    I_orc, Is_orc = O.rtm_shot(op, sx, sz, vpad, wav, seis)
+    with R.Plan(p) as plan:
+        plan.set_velocity(vpad)
+        res = plan.autotune(50)
+        assert len(res) >= 4 and all(r["ms_per_step"] > 0 for r in res)
+        best = min(res, key=lambda r: r["ms_per_step"])
+        chosen = plan.tiling()
+        assert chosen == (best["rows_per_thread"], best["graph_block"], best["interior"], best["seg_rows"])
+        plan.set_wavelet(wav)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I, Is = plan.image()
+    assert np.array_equal(I, I_orc) and np.array_equal(Is, Is_orc)
+    with R.Plan(p) as plan2:
+        assert plan2.tiling() == chosen
