@@ -314,8 +314,11 @@ def run_precision(R, p1, v_mig, w, seis, shape):
     """Config c3: the c1 workload with the source snapshot stored every 4
     steps in FP32, in FP16 or in BF16 (per-snapshot max-abs power-of-two
     scale), stepping FP32, image accumulated in FP32 with Kahan compensation
-    (and, for reference, FP32 without it); each image against the FP64 GPU
-    run of the same scheme (FP64 stepping, FP64 snapshots, FP64 image)."""
+    (and, for reference, FP32 without it), plus boundary saving (no snapshot
+    store: the 4-point ring around the image region saved every step in FP32
+    and the source field rebuilt in time reverse during the backward pass);
+    each image against the FP64 GPU run of the same scheme (FP64 stepping,
+    FP64 snapshots, FP64 image)."""
     p = p1.copy().update(time_slices=4)
     sx, sz = p.first_location_shot, p.depth_shot
     vpad = R.extend(p, v_mig)
@@ -342,19 +345,27 @@ def run_precision(R, p1, v_mig, w, seis, shape):
                        "errors against the FP64 GPU run of the same scheme",
            "fp64_seconds": round(t64, 4), "variants": []}
     nint = p.nx * p.nz
+    nt = p.number_of_time_steps
+    ring = 8 * (p.nx + 8) + 8 * p.nz
+    X = p.nx + p.thickness_left + p.thickness_right
+    Z = p.nz + p.thickness_up + p.thickness_down
     for name, mode, kahan in (("fp32", R.SNAP_FP32, 1), ("fp16_scaled", R.SNAP_FP16, 1),
-                              ("bf16_scaled", R.SNAP_BF16, 1), ("fp32_no_kahan", R.SNAP_FP32, 0)):
+                              ("bf16_scaled", R.SNAP_BF16, 1), ("fp32_no_kahan", R.SNAP_FP32, 0),
+                              ("boundary_saving", R.SNAP_BOUNDARY, 1)):
         q = p.copy().update(snapshot_mode=mode, image_kahan=kahan)
         I, dt = image(q)
         err = np.abs(I.astype(np.float64) - I64)
+        if mode == R.SNAP_BOUNDARY:     # ring saves + the two source-field buffers (pitch 64)
+            store = nt * ring * 4 + 2 * Z * ((X + 63) // 64 * 64) * 4
+        else:
+            store = (nt // 4) * nint * (2 if mode in (R.SNAP_FP16, R.SNAP_BF16) else 4)
         v = {"variant": name, "kahan_image": bool(kahan), "seconds_per_image": round(dt, 4),
-             "snapshot_store_GB": round(1000 * nint * (2 if mode in (R.SNAP_FP16, R.SNAP_BF16) else 4) / 1e9, 3),
+             "snapshot_store_GB": round(store / 1e9, 3),
              "relL2_vs_fp64": rel_l2(I, I64), "max_abs_err": float(err.max()),
              "max_abs_err_rel_to_max": float(err.max() / nrm)}
         out["variants"].append(v)
         log(f"[bench] c3 {name}: relL2 {v['relL2_vs_fp64']:.3e} max-abs {v['max_abs_err']:.3e} "
             f"({v['max_abs_err_rel_to_max']:.2e} of max) {dt * 1e3:.1f} ms/image")
-    out["boundary_saving"] = "not built"
     return out
 
 

@@ -96,7 +96,7 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     *out = nullptr;
     if (rtm_params_check(p)) return -1;
     if (real != RTM_REAL_F32 && real != RTM_REAL_F64) { rtm_set_error("rtm_batch_create: bad real type"); return -1; }
-    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BF16) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BOUNDARY) {
         rtm_set_error("rtm_batch_create: snapshot mode %d not available", p->snapshot_mode);
         return -1;
     }
@@ -182,17 +182,25 @@ static int batch_pass(rtm_batch *q, int phase, int src_j)
     std::vector<SimDesc<R>> d(N);
     for (int s = 0; s < N; s++) {
         SimBuffers &b = q->sims[s];
-        b.reset_state(q->stream);
+        b.reset_state(q->stream, phase == rtm::PH_FORWARD && b.ringmode);
         if (phase == rtm::PH_FORWARD) b.reset_snap_scales(q->stream);
         d[s] = b.desc<R>();
         d[s].phase = phase;
         if (phase == rtm::PH_FORWARD) {
-            d[s].flags = rtm::F_SNAP;
+            d[s].flags = b.ringmode ? rtm::F_RING : rtm::F_SNAP;
             d[s].src_j = src_j;
+            if (b.ringmode) {
+                d[s].ubuf[0] = (R *)b.sbuf[0];
+                d[s].ubuf[1] = (R *)b.sbuf[1];
+            }
         } else {
-            d[s].flags = rtm::F_IMAGE | (b.p.image_kahan ? rtm::F_KAHAN : 0);
+            d[s].flags = rtm::F_IMAGE | (b.p.image_kahan ? rtm::F_KAHAN : 0) | (b.ringmode ? rtm::F_RING : 0);
+            if (b.ringmode) d[s].src_j = src_j;   // only the reconstruction reads it in this phase
         }
     }
+    const SimBuffers &b0 = q->sims[0];
+    q->ex.set_ring_phase(b0.ringmode ? phase : 0,
+                         (int)((phase == rtm::PH_FORWARD ? b0.NR : b0.nint + b0.NR) + 255) / 256);
     if (q->ex.upload_descs(d.data(), N, q->stream)) return -1;
     if (q->ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
     const int nt = q->sims[0].p.number_of_time_steps;
@@ -233,8 +241,8 @@ extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
     int rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_FORWARD, src_j)
                                      : batch_pass<float>(q, rtm::PH_FORWARD, src_j);
     if (rc == 0)
-        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_BACKWARD, -1)
-                                     : batch_pass<float>(q, rtm::PH_BACKWARD, -1);
+        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_BACKWARD, src_j)
+                                     : batch_pass<float>(q, rtm::PH_BACKWARD, src_j);
     for (auto &s : q->sims) s.have_snaps = true;
     return rc;
 }

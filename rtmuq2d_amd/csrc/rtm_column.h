@@ -85,7 +85,7 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         }
     }
     const int kk = k - S.TU;
-    if (!(Q16 ? (V.qsnap || V.smax) : V.snap != nullptr) || kk < 0 || kk >= S.nz) return;
+    if (!(Q16 ? (V.qsnap || V.smax) : (V.snap || V.sfield)) || kk < 0 || kk >= S.nz) return;
     if (!ok) return;
     const R vals[2] = {U.a, U.b};
     const R us[2] = {uc.a, uc.b};
@@ -96,13 +96,15 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         const int q = kk * S.nx + ii;
         if (V.forward) {
             if (!Q16) {
-                stg(V.snap, q, vals[e]);
+                if (V.snap) stg(V.snap, q, vals[e]);
             } else {
                 if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
                 if (V.qsnap) stg(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
             }
         } else {
-            const R s = !Q16 ? ldg(V.snap, q) : R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            const R s = Q16        ? R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode))
+                        : V.sfield ? ldg(V.sfield, k * S.P + xl + e)
+                                   : ldg(V.snap, q);
             if (S.flags & F_KAHAN) {
                 R I = ldg(S.I, q), Ic = ldg(S.Ic, q), Is = ldg(S.Is, q), Isc = ldg(S.Isc, q);
                 kahan_add(I, Ic, s * vals[e]);

@@ -55,6 +55,8 @@ struct rtm_plan {
     int interior = rtm::IN_TILES;   // interior kernel (rtm::Interior)
     int seg_rows = 0;               // rows per interior column segment
     int last_nt = 0;                // steps of the last pass
+    bool last_fwd_ring = false;     // the last pass was a boundary-saving forward pass
+    int fwd_src_j = -1;             // source index of the last forward pass (the reconstruction re-adds it)
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
 };
@@ -98,7 +100,7 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
         rtm_set_error("rtm_plan_create: unknown real type %d", real);
         return -1;
     }
-    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BF16) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BOUNDARY) {
         rtm_set_error("rtm_plan_create: snapshot mode %d is not available", p->snapshot_mode);
         return -1;
     }
@@ -283,19 +285,24 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
     const bool src_inside = xs >= rtm::kHalf && xs < b.X - rtm::kHalf && zs >= rtm::kHalf && zs < b.Z - rtm::kHalf;
     const int src_j = zs * b.P + xs;
     const int nt = p.number_of_time_steps;
-    b.reset_state(q->stream);
+    b.reset_state(q->stream, b.ringmode);
     b.reset_snap_scales(q->stream);
     const bool tiled = src_inside;   // the tiled kernel covers only the stepped region
-    if (!tiled && b.qmode != rtm::Q_NONE) {
-        rtm_set_error("16-bit snapshot stores need the source inside the stepped region");
+    if (!tiled && (b.qmode != rtm::Q_NONE || b.ringmode)) {
+        rtm_set_error("16-bit snapshot stores and boundary saving need the source inside the stepped region");
         return -1;
     }
+    q->ex.set_ring_phase(b.ringmode ? rtm::PH_FORWARD : 0, (b.NR + 255) / 256);
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (tiled) {
         SimDesc<R> d = b.desc<R>();
         d.phase = rtm::PH_FORWARD;
-        d.flags = rtm::F_SNAP | (record && !b.rec_dup && b.n_out == 0 ? rtm::F_RECORD : 0);
+        d.flags = (b.ringmode ? rtm::F_RING : rtm::F_SNAP) | (record && !b.rec_dup && b.n_out == 0 ? rtm::F_RECORD : 0);
         d.src_j = src_j;
+        if (b.ringmode) {   // the source field runs on its own buffers, kept for the backward pass
+            d.ubuf[0] = (R *)b.sbuf[0];
+            d.ubuf[1] = (R *)b.sbuf[1];
+        }
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         if (record && !(d.flags & rtm::F_RECORD)) {
             // receivers sharing a point or outside: record each one per step
@@ -320,6 +327,8 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
     q->fwd_ms = ms;
     q->last_steps = nt;
     q->last_nt = nt;
+    q->last_fwd_ring = b.ringmode;
+    q->fwd_src_j = src_j;
     b.have_snaps = true;
     return 0;
 }
@@ -333,15 +342,17 @@ static int backward_t(rtm_plan *q)
     if (!b.have_seis) { rtm_set_error("rtm_plan_backward: no seismogram"); return -1; }
     const int nt = p.number_of_time_steps;
     b.reset_state(q->stream);
+    q->ex.set_ring_phase(b.ringmode ? rtm::PH_BACKWARD : 0, (int)((b.nint + b.NR + 255) / 256));
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (b.n_out == 0) {
         SimDesc<R> d = b.desc<R>();
         d.phase = rtm::PH_BACKWARD;
-        d.flags = rtm::F_IMAGE | (p.image_kahan ? rtm::F_KAHAN : 0);
+        d.flags = rtm::F_IMAGE | (p.image_kahan ? rtm::F_KAHAN : 0) | (b.ringmode ? rtm::F_RING : 0);
+        if (b.ringmode) d.src_j = q->fwd_src_j;   // only the reconstruction reads it in this phase
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
-    } else if (b.qmode != rtm::Q_NONE) {
-        rtm_set_error("16-bit snapshot stores need every receiver inside the stepped region");
+    } else if (b.qmode != rtm::Q_NONE || b.ringmode) {
+        rtm_set_error("16-bit snapshot stores and boundary saving need every receiver inside the stepped region");
         return -1;
     } else {
         v1_backward<R>(q);
@@ -354,6 +365,8 @@ static int backward_t(rtm_plan *q)
     q->bwd_ms = ms;
     q->last_steps = nt;
     q->last_nt = nt;
+    q->last_fwd_ring = false;
+    if (b.ringmode) b.have_snaps = false;   // the reconstruction consumed the saved source field
     return 0;
 }
 
@@ -442,7 +455,8 @@ static int get_fields_t(rtm_plan *q, O *u_prev, O *u_last)
     HIP_TRY(hipStreamSynchronize(q->stream));
     // after nt steps u[nt & 1] holds u^{nt} and the other buffer u^{nt-1}
     const int nt = q->last_nt;
-    void *src[2] = {b.u[(nt + 1) & 1], b.u[nt & 1]};
+    void *const *w = q->last_fwd_ring ? b.sbuf : b.u;
+    void *src[2] = {w[(nt + 1) & 1], w[nt & 1]};
     O *dst[2] = {u_prev, u_last};
     for (int t = 0; t < 2; t++) {
         if (!dst[t]) continue;

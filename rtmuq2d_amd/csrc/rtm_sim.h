@@ -318,6 +318,57 @@ __global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ 
     column_item<R, Q16>(S, n0[sim] + s, it, item);
 }
 
+// Boundary saving, forward: after step n, U = u^{n+1} on the ring (rtm::ring_point)
+// into ring[n][*].  grid = (ring blocks, sims).
+template <typename R>
+__global__ void k_ring_save(const SimDesc<R> *__restrict__ sims, const int *__restrict__ n0, int s)
+{
+    const SimDesc<R> &S = sims[blockIdx.y];
+    if (!(S.flags & rtm::F_RING) || S.phase != rtm::PH_FORWARD) return;
+    const int n = n0[blockIdx.y] + s;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 0 || n >= S.nt || t >= S.NR) return;
+    int i, k;
+    rtm::ring_point(S, t, i, k);
+    S.ring[(size_t)n * S.NR + t] = S.ubuf[(n + 1) & 1][(size_t)k * S.P + i];
+}
+
+// Boundary saving, backward step n (1 <= n <= nt-2): the source field one
+// step further back.  With m = nt - n, sbuf[m & 1] holds u^m and
+// sbuf[(m+1) & 1] u^{m+1}; inside the image region u^{m-1} is the forward
+// update solved for its old value, u^{m-1} = stencil(C, u^{m+1}, u^m) plus
+// the source term of forward step m (its stencil subtracted U_old and then
+// added w[m] C), written over u^{m+1}; on the ring u^{m-1} is the value
+// saved by forward step m-2.  The step's imaging reads sbuf[m & 1] only.
+template <typename R>
+__global__ void k_recon(const SimDesc<R> *__restrict__ sims, const int *__restrict__ n0, int s)
+{
+    const SimDesc<R> &S = sims[blockIdx.y];
+    if (!(S.flags & rtm::F_RING) || S.phase != rtm::PH_BACKWARD) return;
+    const int n = n0[blockIdx.y] + s;
+    if (n < 1 || n > S.nt - 2) return;
+    const int m = S.nt - n;
+    const R *cur = S.sbuf[m & 1];
+    R *out = S.sbuf[(m + 1) & 1];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nD = S.nx * S.nz;
+    if (t < nD) {
+        const int i = S.TL + t % S.nx, k = S.TU + t / S.nx;
+        const size_t j = (size_t)k * S.P + i;
+        const size_t P = S.P;
+        R v = rtm::stencil_val(S.C[j], out[j], cur[j], cur[j - 1], cur[j + 1], cur[j - P], cur[j + P],
+                               cur[j - 2], cur[j + 2], cur[j - 2 * P], cur[j + 2 * P],
+                               cur[j - 3], cur[j + 3], cur[j - 3 * P], cur[j + 3 * P],
+                               cur[j - 4], cur[j + 4], cur[j - 4 * P], cur[j + 4 * P]);
+        if (m < S.nw && (int)j == S.src_j) v = v + R(S.wav[m]) * S.C[j];
+        out[j] = v;
+    } else if (t < nD + S.NR) {
+        int i, k;
+        rtm::ring_point(S, t - nD, i, k);
+        out[(size_t)k * S.P + i] = S.ring[(size_t)(m - 2) * S.NR + (t - nD)];
+    }
+}
+
 // n0[i] += block for every simulation (the last node of a captured block)
 static __global__ void k_advance(int *__restrict__ n0, int nsims, int block)
 {
@@ -494,6 +545,10 @@ struct SimBuffers {
     int qmode = rtm::Q_NONE;            // 16-bit snapshot store (modes FP16, BF16)
     unsigned short *q16 = nullptr;      // [nsnap][nz][nx]
     unsigned *smax = nullptr;           // [nsnap] max |u| bits per snapshot
+    bool ringmode = false;              // boundary saving (snapshot mode BOUNDARY)
+    void *sbuf[2] = {nullptr, nullptr}; // the source wavefield's buffers
+    void *ring = nullptr;               // [nt][NR] ring values of every forward step
+    int NR = 0;
     void *I = nullptr, *Is = nullptr, *Ic = nullptr, *Isc = nullptr;
     std::vector<float> wav_host;
     bool have_velocity = false, have_seis = false, have_snaps = false;
@@ -512,7 +567,9 @@ struct SimBuffers {
     int set_velocity(const float *vpad, hipStream_t s, bool wait = true);
     int set_wavelet(const float *w, int n);
     int set_seismogram(const float *d, hipStream_t s, bool wait = true);
-    void reset_state(hipStream_t s);      // zero u, psi, csi (a pass starts at rest)
+    // zero the wavefield (u, or with source_field the boundary-saving source
+    // buffers), psi and csi: a pass starts at rest
+    void reset_state(hipStream_t s, bool source_field = false);
     void reset_snap_scales(hipStream_t s); // zero the 16-bit store's max |u| slots
     // the 16-bit store's last snapshot when it is the pass's final field
     int flush_last_snapshot(hipStream_t s);
@@ -544,6 +601,9 @@ struct Exec {
     int rz = 4;                         // rows per thread (tile height 4*rz)
     int interior = rtm::IN_TILES;       // kernel of the interior items
     bool q16 = false;                   // the sims keep 16-bit snapshot stores
+    // boundary saving: the pass's extra kernel (rtm::PH_FORWARD: ring save,
+    // rtm::PH_BACKWARD: reconstruction, 0: none) and its grid width
+    int ring_phase = 0, ring_blocks = 0;
     int cap_sims = 0, cap_tiles = 0, cap_inner = 0;
     int nsims = 0, ntiles = 0, ninner = 0;
     void *d_desc = nullptr;             // SimDesc<R>[cap_sims]
@@ -578,6 +638,8 @@ struct Exec {
     int run(int steps, int block, hipStream_t s);
     int launch_one(int s_in_block, hipStream_t s);
     int drop_graph();
+    // the boundary-saving kernel of the coming pass (a change re-captures)
+    int set_ring_phase(int phase, int blocks);
 };
 
 // ---------------------------------------------------------------------------

@@ -87,6 +87,8 @@ size_t sim_bytes(const rtm_params &p, size_t esz)
     b += 2 * nt * nrec * 4;                         // input and recorded traces
     if (p.snapshot_mode == RTM_SNAP_FP16 || p.snapshot_mode == RTM_SNAP_BF16)
         b += nsnap * nint * 2 + nsnap * rtm::kSnapWords * 4;   // 16-bit snapshot store + max slots
+    else if (p.snapshot_mode == RTM_SNAP_BOUNDARY)
+        b += 2 * nfield * esz + nt * (size_t)(8 * (p.nx + 8) + 8 * p.nz) * esz;   // source field + ring saves
     else
         b += nsnap * nint * esz;                    // snapshot store
     b += 4 * nint * esz;                            // I, Is and their compensations
@@ -134,15 +136,23 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
     if (!rc) rc = get(&csiZ, fb);
     qmode = p.snapshot_mode == RTM_SNAP_FP16 ? rtm::Q_FP16
             : p.snapshot_mode == RTM_SNAP_BF16 ? rtm::Q_BF16 : rtm::Q_NONE;
-    if (!rc && p.snapshot_mode == RTM_SNAP_BOUNDARY) {
-        rtm_set_error("snapshot mode %d (boundary saving) is not available", p.snapshot_mode);
-        rc = -1;
-    }
+    ringmode = p.snapshot_mode == RTM_SNAP_BOUNDARY;
+    NR = 8 * (p.nx + 8) + 8 * p.nz;
     if (!rc && qmode != rtm::Q_NONE && real == RTM_REAL_F64) {
         rtm_set_error("16-bit snapshot stores are for FP32 stepping (the FP64 plan is the oracle run)");
         rc = -1;
     }
-    if (qmode == rtm::Q_NONE) {
+    if (!rc && ringmode && (p.thickness_left < rtm::kHalf || p.thickness_right < rtm::kHalf ||
+                            p.thickness_up < rtm::kHalf || p.thickness_down < rtm::kHalf)) {
+        rtm_set_error("boundary saving needs absorbing layers at least %d points thick (the saved ring lies in them)",
+                      rtm::kHalf);
+        rc = -1;
+    }
+    if (ringmode) {
+        if (!rc) rc = get(&sbuf[0], fb);
+        if (!rc) rc = get(&sbuf[1], fb);
+        if (!rc) rc = get(&ring, nt * (size_t)NR * esz);
+    } else if (qmode == rtm::Q_NONE) {
         if (!rc) rc = get(&snaps, nsnap * nint * esz);
     } else {
         if (!rc) rc = get((void **)&q16, nsnap * nint * 2);
@@ -204,6 +214,7 @@ void SimBuffers::release()
     snaps = I = Is = Ic = Isc = nullptr;
     q16 = nullptr;
     smax = nullptr;
+    sbuf[0] = sbuf[1] = ring = nullptr;
 }
 
 template <typename R>
@@ -342,14 +353,15 @@ int SimBuffers::set_seismogram(const float *d, hipStream_t s, bool wait)
     return 0;
 }
 
-void SimBuffers::reset_state(hipStream_t s)
+void SimBuffers::reset_state(hipStream_t s, bool source_field)
 {
+    void *w0 = source_field ? sbuf[0] : u[0], *w1 = source_field ? sbuf[1] : u[1];
     if (real == RTM_REAL_F64) {
-        double *f[] = {(double *)u[0], (double *)u[1], (double *)psiX[0], (double *)psiX[1],
+        double *f[] = {(double *)w0, (double *)w1, (double *)psiX[0], (double *)psiX[1],
                        (double *)psiZ[0], (double *)psiZ[1], (double *)csiX, (double *)csiZ};
         for (double *ptr : f) fill_zero(ptr, nfield, s);
     } else {
-        float *f[] = {(float *)u[0], (float *)u[1], (float *)psiX[0], (float *)psiX[1],
+        float *f[] = {(float *)w0, (float *)w1, (float *)psiX[0], (float *)psiX[1],
                       (float *)psiZ[0], (float *)psiZ[1], (float *)csiX, (float *)csiZ};
         for (float *ptr : f) fill_zero(ptr, nfield, s);
     }
@@ -432,6 +444,10 @@ SimDesc<R> SimBuffers::desc() const
     d.qmode = qmode;
     d.q16 = q16;
     d.smax = smax;
+    d.sbuf[0] = (R *)sbuf[0];
+    d.sbuf[1] = (R *)sbuf[1];
+    d.ring = (R *)ring;
+    d.NR = NR;
     d.I = (R *)I; d.Is = (R *)Is; d.Ic = (R *)Ic; d.Isc = (R *)Isc;
     return d;
 }
@@ -599,11 +615,34 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
     }
 }
 
+template <typename R>
+static void launch_ring(const Exec &e, int s_in_block, hipStream_t s)
+{
+    const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
+    const dim3 grid(e.ring_blocks, e.cap_sims);
+    if (e.ring_phase == rtm::PH_FORWARD)
+        hipLaunchKernelGGL(k_ring_save<R>, grid, dim3(256), 0, s, d, e.d_n0, s_in_block);
+    else if (e.ring_phase == rtm::PH_BACKWARD)
+        hipLaunchKernelGGL(k_recon<R>, grid, dim3(256), 0, s, d, e.d_n0, s_in_block);
+}
+
 int Exec::launch_one(int s_in_block, hipStream_t s)
 {
     if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
     else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
     else launch_tiles<float, false>(*this, s_in_block, s);
+    if (ring_phase && ring_blocks > 0) {
+        if (real == RTM_REAL_F64) launch_ring<double>(*this, s_in_block, s);
+        else launch_ring<float>(*this, s_in_block, s);
+    }
+    return 0;
+}
+
+int Exec::set_ring_phase(int phase, int blocks)
+{
+    if (phase != ring_phase || blocks != ring_blocks) drop_graph();
+    ring_phase = phase;
+    ring_blocks = blocks;
     return 0;
 }
 

@@ -71,6 +71,8 @@ enum SimFlags : int {
     F_RECORD = 2,   /* fused seismogram recording (forward)     */
     F_SNAP = 4,     /* store snapshots (forward)                */
     F_IMAGE = 8,    /* image against snapshots (backward)       */
+    F_RING = 16,    /* boundary saving: forward saves the ring, backward
+                       images against the reconstructed source field     */
 };
 enum TileClass : int { T_XSTRIP = 1, T_ZSTRIP = 2 };
 
@@ -99,7 +101,42 @@ struct SimDesc {
     int qmode;              /* snapshot store: 0 as R, Q_FP16, Q_BF16   */
     unsigned short *q16;    /* [nt/TS][nz][nx] scaled 16-bit store      */
     unsigned *smax;         /* [nt/TS][kSnapSlots * kSlotStride] max |u| slots (float bits) */
+    /* boundary saving: the source wavefield's own buffers (u^n in
+     * sbuf[n & 1]) and the ring of width 4 around the image region saved
+     * after every forward step, [nt][NR] */
+    R *sbuf[2];
+    R *ring;
+    int NR;
 };
+
+/* The ring of boundary saving: every point within 4 of the image region
+ * [TL, TL+nx) x [TU, TU+nz) and outside it (its values, saved every forward
+ * step, let the region's interior be stepped backward in time).  Bands: top
+ * and bottom (4 rows of nx+8 points each), then left and right (4 columns of
+ * nz points each). */
+template <typename R>
+RTM_HD int ring_count(const SimDesc<R> &S)
+{
+    return 8 * (S.nx + 8) + 8 * S.nz;
+}
+
+template <typename R>
+RTM_HD void ring_point(const SimDesc<R> &S, int t, int &i, int &k)
+{
+    const int W = S.nx + 8;
+    if (t < 4 * W) {
+        k = S.TU - 4 + t / W; i = S.TL - 4 + t % W;
+    } else if (t < 8 * W) {
+        t -= 4 * W;
+        k = S.TU + S.nz + t / W; i = S.TL - 4 + t % W;
+    } else if (t < 8 * W + 4 * S.nz) {
+        t -= 8 * W;
+        i = S.TL - 4 + t / S.nz; k = S.TU + t % S.nz;
+    } else {
+        t -= 8 * W + 4 * S.nz;
+        i = S.TL + S.nx + t / S.nz; k = S.TU + t % S.nz;
+    }
+}
 
 /* 16-bit snapshot store.  Each snapshot c is stored as u * 2^(14-e_c),
  * rounded to FP16 or BF16 (nearest, ties to even), where e_c is the binary
@@ -228,6 +265,7 @@ struct StepView {
     unsigned short *qsnap;
     float qscale;
     unsigned *smax;         /* forward, 16-bit store: this step's snapshot max |U| */
+    const R *sfield;        /* backward, boundary saving: the reconstructed source field (padded) */
 };
 
 /* Q16: the sim keeps a 16-bit snapshot store (SimDesc::qmode != 0).  The
@@ -267,7 +305,9 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
         const int tau = S.nt - 1 - n;
         v.seis_row = S.seis_t + (size_t)tau * S.nrec;
         const bool img = (S.flags & F_IMAGE) && tau % S.TS == 0;
-        if (!Q16) {
+        if (!Q16 && (S.flags & F_RING)) {
+            v.sfield = img ? S.sbuf[(S.nt - n) & 1] : nullptr;   /* u^{tau+1} */
+        } else if (!Q16) {
             v.snap = img ? S.snaps + (size_t)(tau / S.TS) * nint : nullptr;
         } else if (img) {
             const int c = tau / S.TS;
@@ -394,14 +434,15 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
     }
-    if (!V.forward && (Q16 ? V.qsnap != nullptr : V.snap != nullptr)) {
+    if (!V.forward && (Q16 ? V.qsnap != nullptr : (V.snap || V.sfield))) {
         const int ii = clampi(i - S.TL, 0, S.nx - 1);
 #pragma unroll
         for (int rr = 0; rr < RZ; rr++) {
             const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
             const int q = kk * S.nx + ii;
-            if (!Q16) G.sv[rr] = ldg(V.snap, q);
-            else G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            if (Q16) G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            else if (V.sfield) G.sv[rr] = ldg(V.sfield, clampi(z0 + ty * RZ + rr, 0, S.Z - 1) * S.P + ic);
+            else G.sv[rr] = ldg(V.snap, q);
             G.Iv[rr] = ldg(S.I, q);
             G.Isv[rr] = ldg(S.Is, q);
             if (S.flags & F_KAHAN) {
@@ -554,11 +595,11 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             }
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
-            if ((Q16 ? (V.qsnap || V.smax) : V.snap != nullptr) && img_col && kk >= 0 && kk < S.nz) {
+            if ((Q16 ? (V.qsnap || V.smax) : (V.snap || V.sfield)) && img_col && kk >= 0 && kk < S.nz) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
                     if (!Q16) {
-                        stg(V.snap, q, U);
+                        if (V.snap) stg(V.snap, q, U);
                     } else {
                         if (V.smax) umax = fmaxf(umax, fabsf(float(U)));
                         if (V.qsnap) stg(V.qsnap, q, quant16(float(uc), V.qscale, S.qmode));

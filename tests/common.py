@@ -163,3 +163,81 @@ def quantize_snapshots(S, mode: int):
         deq[c] = back * np.float32(2.0 ** (e - 14))
         mx[c] = bits
     return deq, codes, mx
+
+
+# ---- boundary saving: the reconstructed source field, from the oracle ----
+
+_LAP_W = np.float32([-2.97399944, 1.70507669, -0.25861812, 0.04577745, -0.00523630])
+
+
+def ring_points(op):
+    """(i, k) of the 4-wide ring around the image region in its saved order:
+    top band (4 rows of nx+8 points, row by row), bottom band, then the left
+    and right bands (4 columns of nz points, column by column)."""
+    W = op.nx + 8
+    xs = np.arange(W) + op.t_left - 4
+    zs = np.arange(op.nz) + op.t_up
+    ii, kk = [], []
+    for r in range(4):
+        ii.append(xs); kk.append(np.full(W, op.t_up - 4 + r))
+    for r in range(4):
+        ii.append(xs); kk.append(np.full(W, op.t_up + op.nz + r))
+    for c in range(4):
+        ii.append(np.full(op.nz, op.t_left - 4 + c)); kk.append(zs)
+    for c in range(4):
+        ii.append(np.full(op.nz, op.t_left + op.nx + c)); kk.append(zs)
+    return np.concatenate(ii), np.concatenate(kk)
+
+
+def stencil_old(C, old, cur, i0, i1, k0, k1):
+    """(C lap(cur) + 2 cur) - old on [i0,i1) x [k0,k1) of [X][Z] float32
+    fields, each operation rounded to float32 in the order of stencil_val
+    (bracket operands x-, x+, z-, z+)."""
+    f = np.float32
+    c = cur[i0:i1, k0:k1]
+    s = []
+    for m in range(1, 5):
+        s.append(((cur[i0 - m:i1 - m, k0:k1] + cur[i0 + m:i1 + m, k0:k1]) + cur[i0:i1, k0 - m:k1 - m])
+                 + cur[i0:i1, k0 + m:k1 + m])
+    lap = ((((f(2.0) * _LAP_W[0]) * c + _LAP_W[1] * s[0]) + _LAP_W[2] * s[1]) + _LAP_W[3] * s[2]) \
+        + _LAP_W[4] * s[3]
+    return (C[i0:i1, k0:k1] * lap + f(2.0) * c) - old[i0:i1, k0:k1]
+
+
+def boundary_saving_snapshots(op, sx, sz, vpad, wav, plant: str = ""):
+    """The source field the boundary-saving backward pass images against,
+    as snapshots [nt/TS][nx][nz] of u^{tau+1} (the layout of the oracle's
+    stored snapshots).  The ring saves U^{n+1} come from oracle forward runs
+    of n+1 steps; the last two fields from the full run; inside the image
+    region u^{m-1} = (C lap(u^m) + 2 u^m) - u^{m+1} + w[m] C at the source
+    (forward step m solved for its old value), on the ring the save of
+    forward step m-2.  ``plant`` puts a mistake in for the tests that show
+    the check catches it: "no_source" drops the source term, "ring_late"
+    restores the ring from the save one step later."""
+    nt, TS = op.nt, op.time_slices
+    ri, rk = ring_points(op)
+    saves = []
+    for n in range(max(nt - 2, 0)):
+        q = O.OrcParams.from_buffer_copy(op)
+        q.nt, q.time_slices = n + 1, 1    # no snapshots are taken; nt must be a multiple of TS
+        _, _, _, lU = O.forward(q, sx, sz, vpad, wav, snaps=False, record=False)
+        saves.append(lU[ri, rk].copy())
+    _, _, lu, lU = O.forward(op, sx, sz, vpad, wav, snaps=False, record=False)
+    C = O.coef(op, vpad, np.float32)
+    i0, i1, k0, k1 = op.t_left, op.t_left + op.nx, op.t_up, op.t_up + op.nz
+    si, sk = sx + op.t_left, sz + op.t_up
+    fields = {nt: lU, nt - 1: lu}
+    snaps = np.zeros((nt // TS, op.nx, op.nz), np.float32)
+    for n in range(nt):
+        m = nt - n
+        fields.pop(m + 2, None)
+        if (m - 1) % TS == 0:
+            snaps[(m - 1) // TS] = fields[m][i0:i1, k0:k1]
+        if 1 <= n <= nt - 2:
+            new = fields[m + 1].copy()
+            new[i0:i1, k0:k1] = stencil_old(C, fields[m + 1], fields[m], i0, i1, k0, k1)
+            if m < op.n_wavelet and i0 <= si < i1 and k0 <= sk < k1 and plant != "no_source":
+                new[si, sk] = np.float32(new[si, sk] + np.float32(wav[m]) * C[si, sk])
+            new[ri, rk] = saves[min(m - 1, len(saves) - 1)] if plant == "ring_late" else saves[m - 2]
+            fields[m - 1] = new
+    return snaps

@@ -187,3 +187,43 @@ def test_sixteen_bit_store_bitexact(hx, qmode, ts, interior):
     I_ref, Is_ref, _, _ = O.backward(op, vpad, d, deq)
     I2, Is2 = ds.backward_tiled(d, (q16, smax), rz=4, interior=interior, seg=12, qmode=qmode)
     assert np.array_equal(I2, I_ref) and np.array_equal(Is2, Is_ref)
+
+
+
+def test_boundary_saving_reconstruction_close_to_snapshots():
+    """Boundary saving reconstructs the source field backward in time inside
+    the image region from the final two fields and the saved ring: its
+    snapshots track the oracle's stored ones to FP32 time-reversal roundoff
+    (error norm <= 1e-5 of the largest snapshot's norm, on every snapshot),
+    and the image is within relL2 1e-4 of the stored-snapshot image."""
+    cfg, g = _case("asym")
+    op = common.orc_params(cfg, time_slices=4)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    wav = g["wavelet"]
+    S_ref, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+    S_bs = common.boundary_saving_snapshots(op, sx, sz, vpad, wav)
+    big = max(np.linalg.norm(S) for S in S_ref)
+    for c in range(len(S_ref)):
+        assert np.linalg.norm(S_bs[c] - S_ref[c]) <= 1e-5 * big, c
+    assert not np.array_equal(S_bs[0], S_ref[0])    # the reconstruction really ran
+    d = g["seismograms"][0]
+    I_ref, _, _, _ = O.backward(op, vpad, d, S_ref)
+    I_bs, _, _, _ = O.backward(op, vpad, d, S_bs)
+    assert common.rel_l2(I_bs, I_ref) <= 1e-4
+
+
+
+@pytest.mark.parametrize("plant", ["no_source", "ring_late"])
+def test_boundary_saving_reconstruction_catches_a_wrong_term(plant):
+    """The reconstruction check is sharp: dropping the source term, or taking
+    the ring one step off, moves the snapshots far beyond the roundoff bound."""
+    cfg, g = _case("asym")
+    op = common.orc_params(cfg, time_slices=4)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    wav = g["wavelet"]
+    S_ref, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+    big = max(np.linalg.norm(S) for S in S_ref)
+    S_bad = common.boundary_saving_snapshots(op, sx, sz, vpad, wav, plant=plant)
+    assert max(np.linalg.norm(a - b) for a, b in zip(S_bad, S_ref)) > 1e-3 * big

@@ -196,6 +196,89 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
     assert np.allclose(var, stack.var(0, ddof=1), rtol=1e-10, atol=1e-12 * stack.var(0).max())
 
 
+_BS_CACHE = {}
+
+
+def _boundary_saving_expected(ts):
+    """asym, shot 0: the oracle's forward fields and its backward pass over
+    the reconstructed source field (numpy restatement of the reconstruction
+    over oracle forward runs)."""
+    if ts not in _BS_CACHE:
+        cfg = common.REF_CONFIGS["asym"]
+        g = golden("asym")
+        op = common.orc_params(cfg, time_slices=ts)
+        vpad = O.extend(op, g["models"][0])
+        sx, sz = common.shot_positions(cfg)[0]
+        _, _, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], snaps=False, record=False)
+        S_bs = common.boundary_saving_snapshots(op, sx, sz, vpad, g["wavelet"])
+        I, Is, bu, bU = O.backward(op, vpad, g["seismograms"][0], S_bs)
+        _BS_CACHE[ts] = (lu, lU, I, Is, bu, bU)
+    return _BS_CACHE[ts]
+
+
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0), (5, 1)])
+def test_boundary_saving_bitexact(gpu_available, ts, interior):
+    """Boundary-saving snapshot mode: the forward pass saves only the ring
+    around the image region each step and the backward pass rebuilds the
+    source field in time reverse.  Forward and receiver fields equal the
+    oracle's, and the image equals the oracle's backward pass over the
+    reconstructed source field (numpy restatement), bit for bit.  A second
+    backward pass without a new forward pass is refused (the reconstruction
+    consumed the saved state)."""
+    say(f"[gpu] boundary saving TS={ts} interior={interior}")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg, time_slices=ts)
+    p = common.rtm_params(cfg, time_slices=ts, snapshot_mode=R.SNAP_BOUNDARY)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    lu, lU, I, Is, bu, bU = _boundary_saving_expected(ts)
+    with R.Plan(p) as plan:
+        plan.set_tiling(4, 100, interior, 16)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.forward(sx, sz)
+        a, b = plan.fields()
+        assert np.array_equal(a, lu) and np.array_equal(b, lU), "forward fields differ"
+        plan.set_seismogram(g["seismograms"][0])
+        plan.zero_image()
+        plan.backward()
+        a, b = plan.fields()
+        assert np.array_equal(a, bu) and np.array_equal(b, bU), "receiver fields differ"
+        I2, Is2 = plan.image()
+        assert np.array_equal(I2, I), f"image relL2={common.rel_l2(I2, I)}"
+        assert np.array_equal(Is2, Is)
+        with pytest.raises(RuntimeError):
+            plan.backward()
+
+
+def test_batch_boundary_saving_bitexact(gpu_available):
+    """A batch in boundary-saving mode: each realization's image equals the
+    oracle's backward pass over its own reconstructed source field."""
+    say("[gpu] batch boundary saving")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=R.SNAP_BOUNDARY)
+    sx, sz = common.shot_positions(cfg)[0]
+    d = g["seismograms"][0]
+    vpads = [O.extend(op, g["models"][0] * np.float32(1 + 0.05 * k)) for k in range(3)]
+    with R.Batch(p, len(vpads)) as b:
+        b.set_tiling(4, 100, 1, 16)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(g["wavelet"])
+        b.set_seismogram(d)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k, v in enumerate(vpads):
+            S_bs = common.boundary_saving_snapshots(op, sx, sz, v, g["wavelet"])
+            I, Is, _, _ = O.backward(op, v, d, S_bs)
+            I2, Is2 = b.image(k)
+            assert np.array_equal(I2, I), (k, common.rel_l2(I2, I))
+            assert np.array_equal(Is2, Is), k
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0)])
 def test_sixteen_bit_snapshots_bitexact(gpu_available, mode, ts, interior):
