@@ -12,6 +12,7 @@
 from __future__ import annotations
 
 import glob
+import json
 import os
 import subprocess
 import sys
@@ -67,15 +68,43 @@ def build_zfp() -> str | None:
     return out
 
 
+RESOURCES = os.path.join(LIB, "kernel_resources.json")
+_RES_KEYS = {"VGPRs": "vgprs", "AGPRs": "agprs", "TotalSGPRs": "sgprs",
+             "ScratchSize [bytes/lane]": "scratch_bytes", "Occupancy [waves/SIMD]": "occupancy",
+             "LDS Size [bytes/block]": "lds_bytes"}
+
+
+def _resources(text: str) -> dict:
+    """Per-kernel register, scratch, LDS and occupancy figures from the
+    compiler's kernel-resource-usage remarks."""
+    out, name = {}, None
+    for line in text.splitlines():
+        if "remark:" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split(" [-Rpass", 1)[0].strip()
+        if body.startswith("Function Name:"):
+            name = body.split(":", 1)[1].strip()
+            out[name] = {}
+        elif name and ":" in body:
+            k, v = body.rsplit(":", 1)
+            if k.strip() in _RES_KEYS:
+                out[name][_RES_KEYS[k.strip()]] = int(v.strip())
+    return out
+
+
 def build_hip() -> str:
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BUILD, exist_ok=True)
     inc = ["-I", os.path.join(REPO, "include"), "-I", CSRC]
     objs = []
+    res = {}
     for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
         o = os.path.join(BUILD, os.path.basename(src) + ".o")
-        _run([HIPCC, *HIP_FLAGS, *inc, "-c", src, "-o", o])
+        r = _run([HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", *inc, "-c", src, "-o", o])
+        res.update(_resources(r.stderr))
         objs.append(o)
+    with open(RESOURCES, "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
     for src in sorted(glob.glob(os.path.join(CSRC, "*.c"))):
         o = os.path.join(BUILD, os.path.basename(src) + ".o")
         _run(["gcc", *C_FLAGS, *inc, "-c", src, "-o", o])

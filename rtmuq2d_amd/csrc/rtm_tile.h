@@ -305,10 +305,14 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
         const int tau = S.nt - 1 - n;
         v.seis_row = S.seis_t + (size_t)tau * S.nrec;
         const bool img = (S.flags & F_IMAGE) && tau % S.TS == 0;
-        if (!Q16 && (S.flags & F_RING)) {
-            v.sfield = img ? S.sbuf[(S.nt - n) & 1] : nullptr;   /* u^{tau+1} */
-        } else if (!Q16) {
-            v.snap = img ? S.snaps + (size_t)(tau / S.TS) * nint : nullptr;
+        if (!Q16) {
+            /* both fields assigned from selected values: a store through a
+             * selected field address would put the whole view in scratch */
+            const bool ring = (S.flags & F_RING) != 0;
+            const R *sf = S.sbuf[(S.nt - n) & 1];               /* u^{tau+1} */
+            R *sn = S.snaps + (size_t)(tau / S.TS) * nint;
+            v.sfield = (img && ring) ? sf : nullptr;
+            v.snap = (img && !ring) ? sn : nullptr;
         } else if (img) {
             const int c = tau / S.TS;
             v.qsnap = S.q16 + (size_t)c * nint;

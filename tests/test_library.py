@@ -42,6 +42,24 @@ def test_exports_every_declared_symbol():
         getattr(L, n)
 
 
+def test_no_kernel_spills_to_scratch():
+    """The compiler's resource report from the last build: no kernel of the
+    library uses per-lane scratch (spilled registers or a stack array cost a
+    round trip to memory per access), and the FP32 step kernels keep at least
+    three waves per SIMD."""
+    path = os.path.join(common.REPO, "rtmuq2d_amd", "lib", "kernel_resources.json")
+    if not os.path.exists(path):
+        pytest.skip("no resource report: the library was not built by rtmuq2d_amd/build.py")
+    import json
+    res = json.load(open(path))
+    assert any("k_tiles" in k for k in res) and any("k_columns" in k for k in res)
+    scratch = {k: v for k, v in res.items() if v.get("scratch_bytes", 0)}
+    assert not scratch, scratch
+    for k, v in res.items():
+        if ("k_tiles" in k or "k_columns" in k) and "IfL" in k and "Li8E" not in k:
+            assert v["occupancy"] >= 3, (k, v)
+
+
 def test_default_params_are_the_reference_configuration():
     p = R.default_params()
     assert (p.nx, p.nz, p.number_of_time_steps, p.sampling_wavelet) == (2721, 561, 14000, 800)
