@@ -306,10 +306,11 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         if (record && !(d.flags & rtm::F_RECORD)) {
             // receivers sharing a point or outside: record each one per step
+            void *const *uf = b.ringmode ? b.sbuf : b.u;   // the buffers this pass steps
             for (int n = 0; n < nt; n++) {
                 q->ex.launch_one(n, q->stream);
                 hipLaunchKernelGGL(k_record<R>, dim3((p.number_of_receivers + 255) / 256), dim3(256), 0,
-                                   q->stream, (const R *)b.u[(n + 1) & 1], b.rec_j, p.number_of_receivers,
+                                   q->stream, (const R *)uf[(n + 1) & 1], b.rec_j, p.number_of_receivers,
                                    b.seis_rec + (size_t)n * p.number_of_receivers);
             }
         } else if (q->ex.run(nt, q->graph_block, q->stream)) {

@@ -631,6 +631,9 @@ int Exec::launch_one(int s_in_block, hipStream_t s)
     if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
     else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
     else launch_tiles<float, false>(*this, s_in_block, s);
+    // boundary saving: the ring save / reconstruction follows the step on the
+    // same stream (running them beside the step on a stream of their own cost
+    // more in graph edges than it hid)
     if (ring_phase && ring_blocks > 0) {
         if (real == RTM_REAL_F64) launch_ring<double>(*this, s_in_block, s);
         else launch_ring<float>(*this, s_in_block, s);

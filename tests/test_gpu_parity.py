@@ -252,6 +252,39 @@ def test_boundary_saving_bitexact(gpu_available, ts, interior):
             plan.backward()
 
 
+@pytest.mark.parametrize("spacing", [3, 1])
+def test_boundary_saving_records_traces(gpu_available, spacing):
+    """A boundary-saving forward pass records the same traces as the oracle:
+    fused into the step (distinct receivers) and per step (spacing 1 puts
+    every receiver on one point, the reference's spacing-1 indexing); with
+    distinct receivers the image is still the reconstruction's, bit for
+    bit."""
+    say(f"[gpu] boundary saving recording spacing={spacing}")
+    cfg = dict(common.REF_CONFIGS["asym"])
+    cfg["spacingReceiver"] = spacing
+    g = golden("asym")
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=R.SNAP_BOUNDARY)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    _, d_orc, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], snaps=False, record=True)
+    with R.Plan(p) as plan:
+        plan.set_tiling(4, 100, 1, 16)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.forward(sx, sz, record=True)
+        assert np.array_equal(plan.seismogram(), d_orc), "recorded traces differ"
+        a, b = plan.fields()
+        assert np.array_equal(a, lu) and np.array_equal(b, lU), "forward fields differ"
+        if spacing == 3:
+            _, _, I, Is, _, _ = _boundary_saving_expected(4)
+            plan.set_seismogram(g["seismograms"][0])
+            plan.zero_image()
+            plan.backward()
+            I2, Is2 = plan.image()
+            assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
 def test_batch_boundary_saving_bitexact(gpu_available):
     """A batch in boundary-saving mode: each realization's image equals the
     oracle's backward pass over its own reconstructed source field."""
