@@ -41,7 +41,7 @@ BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
 # mean of a forward and a backward step at TS=1: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --profile`, each scaled by what the
 # counter reads for a 1 GiB copy in the same run (x2.0 and x1.0).
-C1_TRAFFIC_BYTES = 24601475
+C1_TRAFFIC_BYTES = 24632569
 
 
 def full_step_bytes(p, ts):
@@ -138,7 +138,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-precision", action="store_true", help="skip the snapshot-precision variants")
-    ap.add_argument("--serve", type=int, default=200, help="requests in the serving run (0: skip)")
+    ap.add_argument("--serve", type=int, default=1000, help="requests in the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.5,
                     help="arrival rate as a multiple of the one-at-a-time plain rate")
     ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
