@@ -242,7 +242,9 @@ int rtm_server_submit(rtm_server *server, const rtm_params *p, const float *vpad
 /* Admit queued requests that fit (arrival order), advance every active
  * request by one block of time steps in shared launches, switch finished
  * forward passes to their backward pass and retire finished requests.
- * Returns the number of requests still queued or active (< 0 on error). */
+ * Returns the number of requests still queued or active (< 0 on error; a
+ * queued request that cannot be placed even in the empty arena is dropped
+ * with that error, and the next tick carries on with the rest). */
 int rtm_server_tick(rtm_server *server);
 /* 1 and *id = the oldest completed request not yet taken, or 0 if none. */
 int rtm_server_next_done(rtm_server *server, int *id);

@@ -206,9 +206,19 @@ static int server_admit(rtm_server *s)
         Request &r = *s->queue.front();
         if (rtm_server_request_footprint(&r.p) > s->arena.size - s->arena.in_use) break;
         r.b.arena = &s->arena;
-        if (r.b.alloc(r.p, RTM_REAL_F32)) {   // fragmentation: wait for memory to free up
+        if (r.b.alloc(r.p, RTM_REAL_F32)) {
             r.b.release();
-            rtm_set_error("");
+            bool any = false;
+            for (auto &a : s->slots) any = any || a;
+            if (!any) {
+                // nothing holds arena memory, so waiting cannot help: drop the
+                // request rather than stall the queue behind it
+                const int id = r.id;
+                s->queue.pop_front();
+                rtm_set_error("rtm_server_tick: request %d does not fit the empty arena; it was dropped", id);
+                return -1;
+            }
+            rtm_set_error("");   // fragmentation: wait for memory to free up
             break;
         }
         // the host inputs stay with the request until it completes, so the
