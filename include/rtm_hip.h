@@ -67,6 +67,11 @@ typedef struct rtm_params {
     int snapshot_mode;                       /* enum rtm_snapshot_mode */
     int image_kahan;                         /* 1: Kahan-compensated FP32 image accumulation */
     int device;                              /* HIP device ordinal */
+    /* rtm_routine: 1 divides the stacked image by the stacked self-correlation
+     * (the reference's full-survey line, commented out at SRC/rtm_iso2d.c:150;
+     * points the source never illuminated stay 0); 0 writes the raw stack as
+     * the shipped reference does */
+    int normalize_illumination;
 } rtm_params;
 
 void rtm_params_default(rtm_params *p);

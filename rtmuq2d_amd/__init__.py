@@ -52,6 +52,7 @@ class RtmParams(C.Structure):
         ("snapshot_mode", C.c_int),
         ("image_kahan", C.c_int),
         ("device", C.c_int),
+        ("normalize_illumination", C.c_int),
     ]
 
     # geometry helpers

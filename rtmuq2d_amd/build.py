@@ -117,8 +117,22 @@ def build_hip() -> str:
     return out
 
 
+DRIVER_SRC = os.path.join(HERE, "driver", "rtm_main.c")
+DRIVER = os.path.join(LIB, "rtmuq2d_hip")
+
+
+def build_driver() -> str:
+    """rtmuq2d_hip, the process driver, linked against librtm_hip.so in the
+    same directory."""
+    tmp = DRIVER + ".tmp"
+    _run(["gcc", "-std=gnu99", "-O2", "-Wall", "-I", os.path.join(REPO, "include"), DRIVER_SRC, "-o", tmp,
+          "-L", LIB, "-lrtm_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build() -> dict:
-    return {"zfp": build_zfp(), "rtm": build_hip()}
+    return {"zfp": build_zfp(), "rtm": build_hip(), "driver": build_driver()}
 
 
 if __name__ == "__main__":

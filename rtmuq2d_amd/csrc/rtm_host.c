@@ -53,6 +53,7 @@ void rtm_params_default(rtm_params *p)
     p->snapshot_mode = RTM_SNAP_FP32;
     p->image_kahan = 0;
     p->device = 0;
+    p->normalize_illumination = 0;
 }
 
 static rtm_params g_params;
@@ -581,6 +582,8 @@ void rtm_routine(int simulationId)
     }
     printf("\n\n");
     rtm_plan_destroy(plan);
+    if (p->normalize_illumination)   /* SRC/rtm_iso2d.c:150, with unlit points left at 0 */
+        for (size_t t = 0; t < nimg; t++) stack[t] = stack_self[t] != 0.0f ? stack[t] / stack_self[t] : 0.0f;
 
     snprintf(path, sizeof path, p->stacked_file, simulationId);
     recording_file(path, stack, p->nx, p->nz);

@@ -418,6 +418,85 @@ def test_rtm_routine_matches_reference_golden(gpu_available, name):
             assert np.array_equal(lap, g["laplacian"][s])
 
 
+def test_rtm_routine_illumination_normalization(gpu_available):
+    """normalize_illumination = 1: rtm_routine writes the shot stack divided
+    by the stacked self-correlation (the reference's full-survey line; a point
+    with zero illumination would stay 0), and its Laplacian; the stacks are
+    the oracle's per-shot images summed in shot order."""
+    say("[gpu] rtm_routine illumination normalization")
+    cfg = common.REF_CONFIGS["tiny"]
+    g = golden("tiny")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    p.normalize_illumination = 1
+    zs = [g[f"zfp_{s}"].tobytes() for s in range(cfg["numberOfShots"])]
+    vpad = O.extend(op, g["models"][0])
+    with tempfile.TemporaryDirectory() as run:
+        common.write_inputs(run, cfg, list(g["models"][:1]), g["wavelet"], zs)
+        stack = np.zeros((op.nx, op.nz), np.float32)
+        self_stack = np.zeros_like(stack)
+        for s, (sx, sz) in enumerate(common.shot_positions(cfg)):
+            seis = R.zfp_decompress_file(os.path.join(run, "INPUTS", f"seismogram_{s + 1:05d}.zfp"),
+                                         cfg["numberOfReceivers"], cfg["numberOfTimeStep"], cfg["zfp_tolerance"])
+            I, Is = O.rtm_shot(op, sx, sz, vpad, g["wavelet"], seis)
+            stack = stack + I
+            self_stack = self_stack + Is
+        assert np.array_equal(stack, g["brute"][0])
+        lit = self_stack != 0
+        expect = np.zeros_like(stack)
+        expect[lit] = stack[lit] / self_stack[lit]
+        cwd = os.getcwd()
+        os.chdir(run)
+        try:
+            R.set_params(p)
+            R.lib().rtm_routine(1)
+        finally:
+            os.chdir(cwd)
+            R.set_params(common.rtm_params(cfg))
+        b, lap = common.read_outputs(run, cfg, 1)
+    assert np.array_equal(b, expect)
+    assert np.array_equal(lap, O.laplacian(op, expect))
+
+
+@pytest.mark.parametrize("mode", ["ranks", "devices"])
+def test_driver_runs_the_reference_simulations(gpu_available, mode):
+    """rtmuq2d_hip, the MPI-free process driver, run on the reference's files
+    with a parameter file: every simulation's outputs equal the reference's
+    own, bit for bit, whether the ids run in one rank or in one child process
+    per device."""
+    import subprocess
+    say(f"[gpu] driver {mode}")
+    cfg = common.REF_CONFIGS["tiny"]
+    g = golden("tiny")
+    zs = [g[f"zfp_{s}"].tobytes() for s in range(cfg["numberOfShots"])]
+    keys = {"Nx": "Nx", "Nz": "Nz", "thicknessLeft": "thicknessLeft", "thicknessRight": "thicknessRight",
+            "thicknessUp": "thicknessUp", "thicknessDown": "thicknessDown", "spacingGrid": "spacingGrid",
+            "sampling": "sampling", "numberOfTimeStep": "numberOfTimeStep", "sampligWavelet": "sampligWavelet",
+            "firstLocationShot": "firstLocationShot", "depthShot": "depthShot", "spacingShot": "spacingShot",
+            "numberOfShots": "numberOfShots", "firstLocationReceiver": "firstLocationReceiver",
+            "depthReceiver": "depthReceiver", "spacingReceiver": "spacingReceiver",
+            "numberOfReceivers": "numberOfReceivers", "numberOfSimulations": "numberOfSimulations",
+            "zfp_tolerance": "zfp_tolerance"}
+    lines = [f"{k} = {cfg[v]}" for k, v in keys.items()]
+    lines += ["TIME_SLICES = 20", "velocity_file = ./INPUTS/vp_%05d.bin", "wavelet_file = ./INPUTS/wavelet.bin",
+              "seismogram_file = ./INPUTS/seismogram_%05d.zfp", "stacked_file = ./OUTPUTS/stackedBrute_%05d.bin",
+              "stacked_laplacian_file = ./OUTPUTS/stackedLaplacian_%05d.bin"]
+    driver = os.path.join(common.REPO, "rtmuq2d_amd", "lib", "rtmuq2d_hip")
+    with tempfile.TemporaryDirectory() as run:
+        common.write_inputs(run, cfg, list(g["models"]), g["wavelet"], zs)
+        with open(os.path.join(run, "params.txt"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        extra = ["--devices", "1"] if mode == "devices" else ["--ranks", "1", "--rank", "0"]
+        r = subprocess.run([driver, "--params", "params.txt", *extra], cwd=run, capture_output=True, text=True,
+                           timeout=100)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert "took" in r.stdout
+        for s in range(cfg["numberOfSimulations"]):
+            b, lap = common.read_outputs(run, cfg, s + 1)
+            assert np.array_equal(b, g["brute"][s]), s
+            assert np.array_equal(lap, g["laplacian"][s]), s
+
+
 def test_legacy_modeling_symbols(gpu_available):
     """isotropicAcousticModeling/adjointModeling keep the reference's calling
     convention (images accumulated with +=) and match the oracle."""

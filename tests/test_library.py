@@ -60,6 +60,23 @@ def test_no_kernel_spills_to_scratch():
             assert v["occupancy"] >= 3, (k, v)
 
 
+def test_params_struct_layout_matches_the_header(tmp_path):
+    """The ctypes mirror of rtm_params has the C struct's size and every
+    field at the C offset (compiled against include/rtm_hip.h)."""
+    fields = [f for f, _ in R.RtmParams._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"rtm_hip.h\"\nint main(void) {\n"
+                   "    printf(\"%zu\\n\", sizeof(rtm_params));\n"
+                   + "".join(f'    printf("%zu\\n", offsetof(rtm_params, {f}));\n' for f in fields)
+                   + "    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(common.REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == C.sizeof(R.RtmParams)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(R.RtmParams, f).offset == off, f
+
+
 def test_default_params_are_the_reference_configuration():
     p = R.default_params()
     assert (p.nx, p.nz, p.number_of_time_steps, p.sampling_wavelet) == (2721, 561, 14000, 800)
@@ -142,3 +159,56 @@ def test_short_file_exits_like_the_reference(tmp_path):
             (common.REPO, str(path).encode()))
     r = subprocess.run(["python", "-c", code], capture_output=True, text=True)
     assert r.returncode == 1 and "Error reading file" in r.stdout
+
+
+DRIVER = os.path.join(common.REPO, "rtmuq2d_amd", "lib", "rtmuq2d_hip")
+
+
+def _dry(*args):
+    r = subprocess.run([DRIVER, *args, "--dry-run"], capture_output=True, text=True)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_splits_simulations_like_the_reference(tmp_path):
+    """rtmuq2d_hip gives rank r of N the simulation ids of the reference's
+    MPI split, SRC/mpi_rtm_iso2d.c:39-43 (float arithmetic as there), for
+    --ranks and for the per-device fan-out; FIRST LAST shift the range."""
+    import math
+
+    def ref(nsim, world, rank):
+        per = np.float32(nsim) / np.float32(world)
+        return (math.ceil(np.float32(per * np.float32(rank)) + np.float32(1)),
+                math.ceil(np.float32(per * np.float32(rank + 1))))
+
+    for nsim in (1, 2, 3, 7, 10, 64, 100):
+        for world in (1, 2, 3, 4, 7, 8):
+            f = tmp_path / "p.txt"
+            f.write_text(f"numberOfSimulations = {nsim}\n")
+            rc, out, _ = _dry("--params", str(f), "--devices", str(world))
+            assert rc == 0
+            got = [tuple(map(int, l.split()[-2:])) for l in out.splitlines() if l.startswith("rank ")]
+            assert got == [ref(nsim, world, r) for r in range(world)], (nsim, world)
+    rc, out, _ = _dry("--ranks", "4", "--rank", "3", "11", "20")
+    assert rc == 0 and out.splitlines()[-1] == "rank 3 device 0 simulations %d %d" % tuple(
+        10 + v for v in ref(10, 4, 3))
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_params_file(tmp_path):
+    """Parameter files take rtm_params names or the reference's macro names,
+    comments and quoted paths; unknown names and bad values are refused."""
+    f = tmp_path / "p.txt"
+    f.write_text("# c0 grid\nNx = 600\nnz=200  # inline comment\nthicknessLeft = 40\nspacing_grid = 10.0\n"
+                 "stacked_file = \"./OUT/brute_%05d.bin\"\nnormalize_illumination = 1\n")
+    rc, out, _ = _dry("--params", str(f))
+    assert rc == 0
+    kv = dict(l.split(" = ", 1) for l in out.splitlines() if " = " in l)
+    assert kv["nx"] == "600" and kv["nz"] == "200" and kv["thickness_left"] == "40"
+    assert kv["spacing_grid"] == "10" and kv["stacked_file"] == "./OUT/brute_%05d.bin"
+    assert kv["normalize_illumination"] == "1" and kv["thickness_right"] == "50"   # default kept
+    for bad in ("nonsense = 3\n", "Nx = 6x\n", "Nx 600\n"):
+        f.write_text(bad)
+        rc, _, err = _dry("--params", str(f))
+        assert rc == 2 and err, bad
+    assert _dry("--ranks", "2", "--rank", "2")[0] == 2
