@@ -247,6 +247,7 @@ static int server_rebuild(rtm_server *s)
     std::vector<int4> tiles, inner;
     for (int i = 0; i < s->max_slots; i++)
         if (s->slots[i]) append_work(tiles, inner, i, s->slots[i]->p, s->rz, s->interior, s->seg_rows);
+    balance_tiles_for_xcds(tiles);
     if ((int)tiles.size() > s->ex.cap_tiles || (int)inner.size() > s->ex.cap_inner || s->ex.rz != s->rz ||
         s->ex.interior != s->interior) {
         std::vector<int> n0 = s->ex.h_n0;

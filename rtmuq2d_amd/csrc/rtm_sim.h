@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const int t = (b & 7) * per + (b >> 3);
     if ((b >> 3) >= per || t >= ntiles) return;
     const int4 tile = tiles[t];
+    if (tile.x < 0) return;   // padding of an XCD's share (balance_tiles_for_xcds)
     // read through the restrict pointer: uniform fields become scalar loads
     // that no store of the step can invalidate
     const SimDesc<R> &S = sims[tile.x];
@@ -584,6 +585,16 @@ size_t sim_bytes(const rtm_params &p, size_t esz);
 // Work lists covering the stepped region of a grid (rtm::build_work): tiles
 // {sim, x0, z0, class}, and interior items for the kernel `interior`
 // (rtm::Interior) in segments of at most seg_rows rows.
+// Reorder a tile list for the eight XCDs.  Workgroup b runs on XCD b & 7 and
+// k_tiles gives XCD k the k-th eighth of the list, so each eighth should cost
+// the same: every simulation's tiles are cut into eight slabs of whole tile
+// columns of equal cost (a strip tile, which moves about twice the bytes and
+// runs the longer chain, counts as three interior tiles), slab k of every
+// simulation goes to XCD k (row-major inside a slab, so neighbouring tiles
+// share the XCD's L2), and the eighths are padded to equal length with
+// entries k_tiles skips.
+void balance_tiles_for_xcds(std::vector<int4> &tiles);
+
 void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, const rtm_params &p,
                  int rz, int interior, int seg_rows);
 inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, int rz)
