@@ -590,9 +590,10 @@ size_t sim_bytes(const rtm_params &p, size_t esz);
 // the same: every simulation's tiles are cut into eight slabs of whole tile
 // columns of equal cost (a strip tile, which moves about twice the bytes and
 // runs the longer chain, counts as three interior tiles), slab k of every
-// simulation goes to XCD k (row-major inside a slab, so neighbouring tiles
-// share the XCD's L2), and the eighths are padded to equal length with
-// entries k_tiles skips.
+// simulation goes to XCD k, and the eighths are padded to equal length with
+// entries k_tiles skips.  Inside a slab the classes with the longest chains
+// are dispatched first (corner, z-strip, x-strip, interior), each row-major
+// so that neighbouring tiles share the XCD's L2.
 void balance_tiles_for_xcds(std::vector<int4> &tiles);
 
 void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, const rtm_params &p,

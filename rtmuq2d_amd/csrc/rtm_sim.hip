@@ -499,7 +499,10 @@ void balance_tiles_for_xcds(std::vector<int4> &tiles)
             c = e;
         }
         for (int k = 0; k < kXcds; k++) {
+            // the longest chains start first: corner tiles, then z-strip,
+            // x-strip and interior tiles, each class row-major
             std::sort(slab[k].begin(), slab[k].end(), [](const int4 &a, const int4 &b) {
+                if (a.w != b.w) return a.w > b.w;
                 return a.z != b.z ? a.z < b.z : a.y < b.y;
             });
             part[k].insert(part[k].end(), slab[k].begin(), slab[k].end());
