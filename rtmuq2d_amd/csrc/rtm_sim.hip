@@ -1,7 +1,6 @@
 // rtm_sim.hip -- per-simulation device buffers and the tiled step executor.
 #include <mutex>
 
-#include <algorithm>
 
 #include "rtm_sim.h"
 
@@ -470,52 +469,11 @@ void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, co
 
 void balance_tiles_for_xcds(std::vector<int4> &tiles)
 {
-    constexpr int kXcds = 8;
-    if (tiles.empty()) return;
-    std::vector<std::vector<int4>> part(kXcds);
-    auto cost = [](const int4 &t) { return t.w ? 3.0 : 1.0; };
-    size_t i = 0;
-    while (i < tiles.size()) {
-        // one simulation's tiles, by column then row
-        size_t j = i;
-        while (j < tiles.size() && tiles[j].x == tiles[i].x) j++;
-        std::vector<int4> sim(tiles.begin() + i, tiles.begin() + j);
-        std::sort(sim.begin(), sim.end(), [](const int4 &a, const int4 &b) {
-            return a.y != b.y ? a.y < b.y : a.z < b.z;
-        });
-        double total = 0.0;
-        for (const int4 &t : sim) total += cost(t);
-        double acc = 0.0;
-        std::vector<std::vector<int4>> slab(kXcds);
-        size_t c = 0;
-        while (c < sim.size()) {
-            size_t e = c;
-            double cw = 0.0;
-            while (e < sim.size() && sim[e].y == sim[c].y) cw += cost(sim[e++]);
-            int k = (int)((acc + 0.5 * cw) / total * kXcds);
-            k = k < 0 ? 0 : (k >= kXcds ? kXcds - 1 : k);
-            slab[k].insert(slab[k].end(), sim.begin() + c, sim.begin() + e);
-            acc += cw;
-            c = e;
-        }
-        for (int k = 0; k < kXcds; k++) {
-            // the longest chains start first: corner tiles, then z-strip,
-            // x-strip and interior tiles, each class row-major
-            std::sort(slab[k].begin(), slab[k].end(), [](const int4 &a, const int4 &b) {
-                if (a.w != b.w) return a.w > b.w;
-                return a.z != b.z ? a.z < b.z : a.y < b.y;
-            });
-            part[k].insert(part[k].end(), slab[k].begin(), slab[k].end());
-        }
-        i = j;
-    }
-    size_t per = 0;
-    for (const auto &p : part) per = p.size() > per ? p.size() : per;
-    tiles.clear();
-    for (const auto &p : part) {
-        tiles.insert(tiles.end(), p.begin(), p.end());
-        tiles.insert(tiles.end(), per - p.size(), make_int4(-1, 0, 0, 0));
-    }
+    std::vector<rtm::WorkItem> w(tiles.size());
+    for (size_t i = 0; i < tiles.size(); i++) w[i] = {tiles[i].x, tiles[i].y, tiles[i].z, tiles[i].w};
+    rtm::balance_for_xcds(w);
+    tiles.resize(w.size());
+    for (size_t i = 0; i < w.size(); i++) tiles[i] = make_int4(w[i].a, w[i].b, w[i].c, w[i].d);
 }
 
 // ---------------------------------------------------------------------------

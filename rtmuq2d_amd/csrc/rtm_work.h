@@ -14,6 +14,7 @@
 #ifndef RTM_WORK_H
 #define RTM_WORK_H
 
+#include <algorithm>
 #include <vector>
 
 #include "rtm_column.h"
@@ -80,6 +81,64 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
             const int e = z + seg < c.zB ? z + seg : c.zB;
             inner.push_back({col_pack(sim, c.half), c.x0, z, e});
         }
+}
+
+/* Reorder a tile list {sim, x0, z0, class} for kXcds XCDs.  Workgroup b runs
+ * on XCD b % kXcds and the tile kernel gives XCD k the k-th share of the
+ * list, so the shares should cost the same.  Every simulation's tiles are cut
+ * into kXcds slabs of whole tile columns of equal cost (a strip tile counts
+ * as three interior tiles), slab k of every simulation goes to share k, and
+ * the shares are padded to equal length with entries {-1, 0, 0, 0} that the
+ * kernel skips.  Inside each simulation's slab the classes with the longest
+ * chains come first (corner, z-strip, x-strip, interior), each row-major.
+ * The list must hold each simulation's tiles contiguously. */
+constexpr int kXcds = 8;
+constexpr double kStripCost = 3.0;
+
+inline void balance_for_xcds(std::vector<WorkItem> &tiles)
+{
+    if (tiles.empty()) return;
+    std::vector<std::vector<WorkItem>> part(kXcds);
+    auto cost = [](const WorkItem &t) { return t.d ? kStripCost : 1.0; };
+    size_t i = 0;
+    while (i < tiles.size()) {
+        size_t j = i;
+        while (j < tiles.size() && tiles[j].a == tiles[i].a) j++;
+        std::vector<WorkItem> sim(tiles.begin() + i, tiles.begin() + j);
+        std::sort(sim.begin(), sim.end(), [](const WorkItem &a, const WorkItem &b) {
+            return a.b != b.b ? a.b < b.b : a.c < b.c;
+        });
+        double total = 0.0;
+        for (const WorkItem &t : sim) total += cost(t);
+        double acc = 0.0;
+        std::vector<std::vector<WorkItem>> slab(kXcds);
+        size_t c = 0;
+        while (c < sim.size()) {
+            size_t e = c;
+            double cw = 0.0;
+            while (e < sim.size() && sim[e].b == sim[c].b) cw += cost(sim[e++]);
+            int k = (int)((acc + 0.5 * cw) / total * kXcds);
+            k = k < 0 ? 0 : (k >= kXcds ? kXcds - 1 : k);
+            slab[k].insert(slab[k].end(), sim.begin() + c, sim.begin() + e);
+            acc += cw;
+            c = e;
+        }
+        for (int k = 0; k < kXcds; k++) {
+            std::sort(slab[k].begin(), slab[k].end(), [](const WorkItem &a, const WorkItem &b) {
+                if (a.d != b.d) return a.d > b.d;
+                return a.c != b.c ? a.c < b.c : a.b < b.b;
+            });
+            part[k].insert(part[k].end(), slab[k].begin(), slab[k].end());
+        }
+        i = j;
+    }
+    size_t per = 0;
+    for (const auto &p : part) per = p.size() > per ? p.size() : per;
+    tiles.clear();
+    for (const auto &p : part) {
+        tiles.insert(tiles.end(), p.begin(), p.end());
+        tiles.insert(tiles.end(), per - p.size(), WorkItem{-1, 0, 0, 0});
+    }
 }
 
 }  // namespace rtm

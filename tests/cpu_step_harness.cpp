@@ -167,6 +167,22 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
 }
 
 extern "C" {
+/* rtm::balance_for_xcds over the tile lists of nsims same-sized grids (sim
+ * ids 0..nsims-1, tile interior mode): out receives the reordered list as
+ * {sim, x0, z0, class} quadruples (capacity `cap` items); returns its length
+ * or -1. */
+int hx_balance(int X, int Z, int TL, int TR, int TU, int TD, int rz, int nsims, int *out, int cap)
+{
+    std::vector<rtm::WorkItem> tiles, inner;
+    for (int s = 0; s < nsims; s++) rtm::build_work(tiles, inner, s, X, Z, TL, TR, TU, TD, rz, rtm::IN_TILES, 0);
+    rtm::balance_for_xcds(tiles);
+    if ((int)tiles.size() > cap) return -1;
+    for (size_t i = 0; i < tiles.size(); i++) {
+        out[4 * i] = tiles[i].a; out[4 * i + 1] = tiles[i].b; out[4 * i + 2] = tiles[i].c; out[4 * i + 3] = tiles[i].d;
+    }
+    return (int)tiles.size();
+}
+
 /* the library's final-snapshot fill of the 16-bit store (k_quantize_interior) */
 void hx_quantize_interior(const float *u, int P, int TL, int TU, int nx, int nz, unsigned short *q,
                           const unsigned *slots, int mode)

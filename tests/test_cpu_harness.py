@@ -227,3 +227,46 @@ def test_boundary_saving_reconstruction_catches_a_wrong_term(plant):
     big = max(np.linalg.norm(S) for S in S_ref)
     S_bad = common.boundary_saving_snapshots(op, sx, sz, vpad, wav, plant=plant)
     assert max(np.linalg.norm(a - b) for a, b in zip(S_bad, S_ref)) > 1e-3 * big
+
+
+@pytest.mark.parametrize("nsims", [1, 3])
+def test_xcd_balance_keeps_every_tile_once(hx, nsims):
+    """The tile list reordered for the eight XCDs (rtm::balance_for_xcds):
+    every tile of build_work's list appears exactly once; the eight shares
+    have one length, padded with skipped entries; each tile column of a
+    simulation stays in one share; the shares' costs (strip tile 3, interior
+    1) are within one tile column's cost of the mean; inside each
+    simulation's run of a share the classes come corner, z-strip, x-strip,
+    interior, each row-major."""
+    import ctypes as C
+    for X, Z, TL, TR, TU, TD in [(1801, 451, 50, 50, 50, 50), (100, 71, 14, 20, 18, 13), (680, 280, 40, 40, 40, 40)]:
+        for rz in (1, 2, 4):
+            cap = 200000
+            buf = np.zeros(4 * cap, np.int32)
+            n = hx.hx_balance(X, Z, TL, TR, TU, TD, rz, nsims, buf.ctypes.data_as(C.c_void_p), cap)
+            assert n > 0 and n % 8 == 0
+            t = buf[:4 * n].reshape(n, 4)
+            shares = t.reshape(8, n // 8, 4)
+            real = t[t[:, 0] >= 0]
+            assert (t[t[:, 0] < 0][:, 1:] == 0).all()
+            TZ = 4 * rz
+            across, down = (X - 8 + 63) // 64, (Z - 8 + TZ - 1) // TZ
+            assert len(real) == nsims * across * down
+            keys = {tuple(r[:3]) for r in real}
+            assert len(keys) == len(real)
+            assert keys == {(s, 4 + 64 * i, 4 + TZ * k) for s in range(nsims) for i in range(across) for k in range(down)}
+            col_share = {}
+            for k in range(8):
+                for r in shares[k]:
+                    if r[0] >= 0:
+                        assert col_share.setdefault((r[0], r[1]), k) == k
+            cost = [sum(3 if r[3] else 1 for r in shares[k] if r[0] >= 0) for k in range(8)]
+            col_cost = max(sum(3 if r[3] else 1 for r in real if r[0] == s and r[1] == x) for s, x in col_share)
+            mean = sum(cost) / 8
+            assert max(cost) - mean <= nsims * col_cost + 1e-9, cost
+            rank = {3: 0, 2: 1, 1: 2, 0: 3}
+            for k in range(8):
+                rows = [r for r in shares[k] if r[0] >= 0]
+                for s in range(nsims):
+                    mine = [(rank[r[3]], r[2], r[1]) for r in rows if r[0] == s]
+                    assert mine == sorted(mine)
