@@ -63,10 +63,10 @@ def full_step_bytes(p, ts):
 
 def tiling_dict(shape):
     """(tile rows per thread, graph block, interior kernel 0 tiles / 1
-    columns, rows per interior segment) as a dict."""
+    columns / 2 mixed tiles, rows per interior segment) as a dict."""
     rz, block, interior, seg = shape
     return {"rows_per_thread": rz, "graph_block": block,
-            "interior": ("tiles", "columns")[interior], "seg_rows": seg}
+            "interior": ("tiles", "columns", "mixed")[interior], "seg_rows": seg}
 
 
 def log(*a):
@@ -139,9 +139,11 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-precision", action="store_true", help="skip the snapshot-precision variants")
     ap.add_argument("--serve", type=int, default=1000, help="requests in the serving run (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=1.5,
+    ap.add_argument("--serve-rate", type=float, default=1.0,
                     help="arrival rate as a multiple of the one-at-a-time plain rate")
     ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
+    ap.add_argument("--serve-tiling", default="4,100,0,0",
+                    help="request server kernel shape: rows_per_thread,graph_block,interior,seg_rows")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--batch", type=int, default=64, help="realizations in the batched run (0: skip)")
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
@@ -163,7 +165,8 @@ def main() -> None:
         run_profile(R, args.profile_nt, args.profile_sims)
         return
     if args.only_serve:
-        print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb)), flush=True)
+        print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
+                                     tuple(int(v) for v in args.serve_tiling.split(",")))), flush=True)
         return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
@@ -262,7 +265,8 @@ def main() -> None:
     if not args.no_precision:
         out["precision"] = run_precision(R, p, v_mig, w, seis, best[1])
     if args.serve > 0:
-        out["serving"] = run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb)
+        out["serving"] = run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
+                                     tuple(int(v) for v in args.serve_tiling.split(",")))
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
@@ -389,7 +393,7 @@ def serve_params(R, cls):
         number_of_simulations=1, zfp_tolerance=1e-4, time_slices=4, snapshot_mode=R.SNAP_FP16, image_kahan=1)
 
 
-def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
+def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
     """Config c4: a Poisson stream of `nreq` migration requests, grid class
     drawn uniformly from SERVE_CLASSES, each with its own seeded folded /
     faulted velocity realization and its own source position along the
@@ -398,7 +402,9 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
     depend on their values).  Arrivals come at `rate_factor` times the rate
     that one-at-a-time plain execution sustains; latency = completion -
     arrival on the wall clock; the plain comparison uses each class's
-    measured one-at-a-time time in the same arrival order."""
+    measured one-at-a-time time (a plan autotuned on that class) in the same
+    arrival order.  `shape` is the server's kernel shape: one shape serves
+    every grid in a launch."""
     from rtmuq2d_amd import serve, synth
     rng = np.random.default_rng(seed)
     ctx = []
@@ -414,8 +420,8 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
         p, wav, seis = ctx[k]
         v = synth.folded_faulted(p.nx, p.nz, p.spacing_grid, seed)
         with R.Plan(p) as plan:
-            plan.set_tiling(2, 100, 0, 0)
             plan.set_velocity(R.extend(p, v))
+            plan.autotune(100)                  # each class at its own best shape
             plan.set_wavelet(wav)
             plan.set_seismogram(seis)
             plan.zero_image()
@@ -446,7 +452,7 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
     reqs = [serve.Request(float(arrivals[i]), (lambda i=i: inputs[i]), SERVE_CLASSES[classes[i]][0])
             for i in range(nreq)]
     with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
-        srv.set_tiling(4, 100, 0, 0)
+        srv.set_tiling(*shape)
         res = serve.run(srv, reqs)
         # capacity: the same requests all waiting at t = 0
         ncap = min(nreq, 100)
@@ -458,7 +464,7 @@ def run_serving(R, nreq, rate_factor, arena_gb, seed=11):
         "workload": (f"c4 serving: {nreq} Poisson requests at {rate:.2f}/s ({rate_factor}x the one-at-a-time "
                      "plain rate), classes " + ", ".join(c[0] for c in SERVE_CLASSES)
                      + "; FP16 snapshots every 4 steps, Kahan image"),
-        "requests": nreq, "arrival_rate_per_s": round(rate, 3),
+        "requests": nreq, "arrival_rate_per_s": round(rate, 3), "server_tiling": tiling_dict(shape),
         "class_counts": [int((classes == k).sum()) for k in range(len(SERVE_CLASSES))],
         "latency_s": {k: round(v, 4) for k, v in res["latency_s"].items()},
         "images_per_hour": round(res["images_per_hour"], 1),

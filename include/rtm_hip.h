@@ -150,14 +150,18 @@ size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan 
 /* Which kernel steps the interior (the points outside every CPML strip):
  * TILES   the LDS tile kernel, as everywhere else;
  * COLUMNS wavefronts march down 128-column strips with a register window
- *         and lane shifts (no LDS, no barrier), beside the strip tiles. */
-enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1 };
+ *         and lane shifts (no LDS, no barrier), beside the strip tiles;
+ * MIXED   one tile launch with 64 x 4*rows_per_thread interior tiles and
+ *         64 x 4 tiles wherever a CPML strip is touched. */
+enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1, RTM_INTERIOR_MIXED = 2 };
 
 /* Kernel shape: rows_per_thread in {1,2,4,8} gives tiles of 64 x
- * 4*rows_per_thread points; graph_block = time steps per captured hipGraph
- * block (<= 1: one eager launch per step); interior = enum rtm_interior;
- * seg_rows = rows per column segment (COLUMNS; <= 0: whole strips).
- * Tiles stay wherever a CPML strip is touched.  Defaults: 4, 100, TILES, 0. */
+ * 4*rows_per_thread points (MIXED: 2 or 4, for the interior tiles);
+ * graph_block = time steps per captured hipGraph block (<= 1: one eager
+ * launch per step); interior = enum rtm_interior; seg_rows = rows per column
+ * segment (COLUMNS; <= 0: whole strips).  Tiles stay wherever a CPML strip is
+ * touched.  Defaults: 4, 100, TILES, 0.  Results are bit-identical for every
+ * shape. */
 int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int interior, int seg_rows);
 int rtm_plan_get_tiling(rtm_plan *plan, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows);
 /* Autotuner: time every candidate shape (steps forward time steps from rest,

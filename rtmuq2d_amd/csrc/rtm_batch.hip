@@ -63,7 +63,7 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
     std::vector<int4> tiles, inner;
     for (int s = 0; s < (int)q->sims.size(); s++)
         append_work(tiles, inner, s, q->sims[s].p, rz, interior, seg_rows);
-    balance_tiles_for_xcds(tiles);
+    balance_tiles_for_xcds(tiles, 4 * rz);
     q->ex.release();
     if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
     q->ex.q16 = q->sims[0].qmode != rtm::Q_NONE;

@@ -80,7 +80,7 @@ static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, i
 {
     std::vector<int4> tiles, inner;
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
-    balance_tiles_for_xcds(tiles);
+    balance_tiles_for_xcds(tiles, 4 * rz);
     q->ex.release();
     if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
     q->ex.q16 = q->b.qmode != rtm::Q_NONE;

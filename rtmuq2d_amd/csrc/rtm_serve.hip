@@ -121,8 +121,12 @@ extern "C" int rtm_server_set_tiling(rtm_server *s, int rows_per_thread, int gra
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rows_per_thread);
         return -1;
     }
-    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS) {
-        rtm_set_error("interior kernel must be 0 (tiles) or 1 (columns) (got %d)", interior);
+    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED) {
+        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns) or 2 (mixed tiles) (got %d)", interior);
+        return -1;
+    }
+    if (interior == rtm::IN_MIXED && rows_per_thread != 2 && rows_per_thread != 4) {
+        rtm_set_error("mixed tiles take 2 or 4 rows per thread for the interior tiles (got %d)", rows_per_thread);
         return -1;
     }
     s->rz = rows_per_thread;
@@ -247,7 +251,7 @@ static int server_rebuild(rtm_server *s)
     std::vector<int4> tiles, inner;
     for (int i = 0; i < s->max_slots; i++)
         if (s->slots[i]) append_work(tiles, inner, i, s->slots[i]->p, s->rz, s->interior, s->seg_rows);
-    balance_tiles_for_xcds(tiles);
+    balance_tiles_for_xcds(tiles, 4 * s->rz);
     if ((int)tiles.size() > s->ex.cap_tiles || (int)inner.size() > s->ex.cap_inner || s->ex.rz != s->rz ||
         s->ex.interior != s->interior) {
         std::vector<int> n0 = s->ex.h_n0;

@@ -135,6 +135,39 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     run_tile_cls<R, RZ, Q16>(L, red, S, n, tile, tx, ty);
 }
 
+// A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run
+// one row per thread; full 64 x 4RZ tiles are interior ones.  The two shapes
+// share the LDS.
+template <typename R, int RZ, bool Q16>
+__global__ void __launch_bounds__(256) k_tiles_mix(const SimDesc<R> *__restrict__ sims,
+                                                   const int *__restrict__ n0,
+                                                   const int4 *__restrict__ tiles,
+                                                   const int *__restrict__ ntiles_p, int s)
+{
+    __shared__ union {
+        rtm::TileLds<R, 1> small;
+        rtm::TileLds<R, RZ> full;
+    } L;
+    __shared__ float red[4];
+    const int ntiles = *ntiles_p;
+    const int per = (ntiles + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= ntiles) return;
+    int4 tile = tiles[t];
+    if (tile.x < 0) return;   // padding of an XCD's share (balance_tiles_for_xcds)
+    const SimDesc<R> &S = sims[tile.x];
+    const int n = n0[tile.x] + s;
+    const int tx = threadIdx.x & 63;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (tile.w & rtm::T_SMALL) {
+        tile.w &= 3;
+        run_tile_cls<R, 1, Q16>(L.small, red, S, n, tile, tx, ty);
+    } else {
+        run_tile<R, RZ, 0, Q16>(L.full, red, S, n, tile.y, tile.z, tx, ty);
+    }
+}
+
 // Wavefront shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
 // lane l-1's (l+1's) value; lane 0 (63) keeps `old`.
 __device__ __forceinline__ int dpp_shr1_i(int v, int old)
@@ -594,7 +627,7 @@ size_t sim_bytes(const rtm_params &p, size_t esz);
 // entries k_tiles skips.  Inside a slab the classes with the longest chains
 // are dispatched first (corner, z-strip, x-strip, interior), each row-major
 // so that neighbouring tiles share the XCD's L2.
-void balance_tiles_for_xcds(std::vector<int4> &tiles);
+void balance_tiles_for_xcds(std::vector<int4> &tiles, int tz);
 
 void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, const rtm_params &p,
                  int rz, int interior, int seg_rows);

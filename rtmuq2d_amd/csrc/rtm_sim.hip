@@ -467,11 +467,11 @@ void append_work(std::vector<int4> &tiles, std::vector<int4> &inner, int sim, co
     for (const auto &w : in) inner.push_back(make_int4(w.a, w.b, w.c, w.d));
 }
 
-void balance_tiles_for_xcds(std::vector<int4> &tiles)
+void balance_tiles_for_xcds(std::vector<int4> &tiles, int tz)
 {
     std::vector<rtm::WorkItem> w(tiles.size());
     for (size_t i = 0; i < tiles.size(); i++) w[i] = {tiles[i].x, tiles[i].y, tiles[i].z, tiles[i].w};
-    rtm::balance_for_xcds(w);
+    rtm::balance_for_xcds(w, tz);
     tiles.resize(w.size());
     for (size_t i = 0; i < w.size(); i++) tiles[i] = make_int4(w[i].a, w[i].b, w[i].c, w[i].d);
 }
@@ -489,8 +489,12 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
         return -1;
     }
-    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS) {
-        rtm_set_error("interior kernel must be 0 (tiles) or 1 (columns) (got %d)", interior);
+    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED) {
+        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns) or 2 (mixed tiles) (got %d)", interior);
+        return -1;
+    }
+    if (interior == rtm::IN_MIXED && rz != 2 && rz != 4) {
+        rtm_set_error("mixed tiles take 2 or 4 rows per thread for the interior tiles (got %d)", rz);
         return -1;
     }
     cap_sims = max_sims > 0 ? max_sims : 1;
@@ -610,7 +614,13 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
         hipLaunchKernelGGL((k_columns<R, Q16>), dim3(cgrid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner,
                            s_in_block);
     }
-    if (e.ntiles) {
+    if (e.ntiles && e.interior == rtm::IN_MIXED) {
+        const int grid = 8 * ((e.cap_tiles + 7) / 8);
+        if (e.rz == 2)
+            hipLaunchKernelGGL((k_tiles_mix<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+        else
+            hipLaunchKernelGGL((k_tiles_mix<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+    } else if (e.ntiles) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
         case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
@@ -698,6 +708,7 @@ const std::vector<Shape> &tuner_candidates()
     static const std::vector<Shape> c = {
         {1, rtm::IN_TILES, 0}, {2, rtm::IN_TILES, 0}, {4, rtm::IN_TILES, 0}, {8, rtm::IN_TILES, 0},
         {4, rtm::IN_COLUMNS, 0}, {4, rtm::IN_COLUMNS, 64}, {2, rtm::IN_COLUMNS, 32}, {1, rtm::IN_COLUMNS, 8},
+        {4, rtm::IN_MIXED, 0}, {2, rtm::IN_MIXED, 0},
     };
     return c;
 }

@@ -75,6 +75,8 @@ enum SimFlags : int {
                        images against the reconstructed source field     */
 };
 enum TileClass : int { T_XSTRIP = 1, T_ZSTRIP = 2 };
+/* In a mixed tile list (IN_MIXED): a 64 x 4 tile, its class in the low bits */
+constexpr int T_SMALL = 4;
 
 template <typename R>
 struct SimDesc {

@@ -9,7 +9,11 @@
  *   IN_TILES    every tile goes to the tile kernel;
  *   IN_COLUMNS  class-0 runs of two adjacent tile columns with the same
  *               rows become one 128-column item, a lone column a 64-column
- *               item, in segments of at most seg_rows rows (rtm_column.h).
+ *               item, in segments of at most seg_rows rows (rtm_column.h);
+ *   IN_MIXED    class-0 tiles stay 64 x TZ and every other tile is cut into
+ *               64 x 4 tiles (class | T_SMALL, the class of the small tile's
+ *               own rows), all in one tile launch: the strip tiles' longer
+ *               chain runs on less work per workgroup.
  */
 #ifndef RTM_WORK_H
 #define RTM_WORK_H
@@ -21,7 +25,7 @@
 
 namespace rtm {
 
-enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1 };
+enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1, IN_MIXED = 2 };
 
 struct WorkItem {
     int a, b, c, d;   /* tile {sim, x0, z0, class}; interior {sim(+half), x0, zA, zB} */
@@ -56,7 +60,13 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
         for (int x0 = kHalf; x0 < X - kHalf; x0 += 64, t++) {
             const Run &r = runs[t];
             if (r.zA >= 0 && z0 >= r.zA && z0 < r.zB) continue;
-            tiles.push_back({sim, x0, z0, tile_class(d, x0, z0, TZ)});
+            const int cls = tile_class(d, x0, z0, TZ);
+            if (interior == IN_MIXED && cls != 0) {
+                for (int zs = z0; zs < z0 + TZ && zs < Z - kHalf; zs += 4)
+                    tiles.push_back({sim, x0, zs, tile_class(d, x0, zs, 4) | T_SMALL});
+                continue;
+            }
+            tiles.push_back({sim, x0, z0, cls});
         }
     }
     /* interior items {x0, zA, zB, half}: one run per tile column, two
@@ -87,19 +97,25 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
  * on XCD b % kXcds and the tile kernel gives XCD k the k-th share of the
  * list, so the shares should cost the same.  Every simulation's tiles are cut
  * into kXcds slabs of whole tile columns of equal cost (a strip tile counts
- * as three interior tiles), slab k of every simulation goes to share k, and
+ * as three interior tiles of its size), slab k of every simulation goes to
+ * share k, and
  * the shares are padded to equal length with entries {-1, 0, 0, 0} that the
  * kernel skips.  Inside each simulation's slab the classes with the longest
- * chains come first (corner, z-strip, x-strip, interior), each row-major.
+ * chains come first (small tiles before full ones; corner, z-strip, x-strip,
+ * interior), each row-major.
  * The list must hold each simulation's tiles contiguously. */
 constexpr int kXcds = 8;
 constexpr double kStripCost = 3.0;
 
-inline void balance_for_xcds(std::vector<WorkItem> &tiles)
+/* tz = rows of a full tile; a 64 x 4 tile (T_SMALL) counts 4/tz of one */
+inline void balance_for_xcds(std::vector<WorkItem> &tiles, int tz)
 {
     if (tiles.empty()) return;
     std::vector<std::vector<WorkItem>> part(kXcds);
-    auto cost = [](const WorkItem &t) { return t.d ? kStripCost : 1.0; };
+    const double small = tz > 0 ? 4.0 / tz : 1.0;
+    auto cost = [small](const WorkItem &t) {
+        return ((t.d & 3) ? kStripCost : 1.0) * ((t.d & T_SMALL) ? small : 1.0);
+    };
     size_t i = 0;
     while (i < tiles.size()) {
         size_t j = i;

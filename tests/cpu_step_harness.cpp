@@ -115,10 +115,21 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
     if (!V.active) return;
     auto *L = new rtm::TileLds<R, RZ>();
     auto *G = new rtm::TileRegs<R, RZ>[256];
+    auto *Ls = new rtm::TileLds<R, 1>();
+    auto *Gs = new rtm::TileRegs<R, 1>[256];
     std::vector<rtm::WorkItem> tiles, inner;
     rtm::build_work(tiles, inner, 0, S.X, S.Z, S.TL, S.TR, S.TU, S.TD, RZ, interior, seg_rows);
     for (const auto &t : tiles) {
         // the LDS holds stale data from the previous tile, as on the GPU
+        if (t.d & rtm::T_SMALL) {   // a 64 x 4 tile of a mixed list (k_tiles_mix)
+            switch (t.d & 3) {
+            case 0: run_tile<R, 1, 0, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
+            case 1: run_tile<R, 1, 1, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
+            case 2: run_tile<R, 1, 2, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
+            default: run_tile<R, 1, 3, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
+            }
+            continue;
+        }
         switch (t.d) {
         case 0: run_tile<R, RZ, 0, Q16>(*L, G, S, V, t.b, t.c); break;
         case 1: run_tile<R, RZ, 1, Q16>(*L, G, S, V, t.b, t.c); break;
@@ -127,6 +138,8 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
         }
     }
     for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
+    delete[] Gs;
+    delete Ls;
     delete[] G;
     delete L;
 }
@@ -167,15 +180,28 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
 }
 
 extern "C" {
+/* rtm::build_work's tile list for one grid as {sim, x0, z0, class}
+ * quadruples (capacity `cap` items); returns its length or -1. */
+int hx_work_list(int X, int Z, int TL, int TR, int TU, int TD, int rz, int interior, int seg_rows, int *out, int cap)
+{
+    std::vector<rtm::WorkItem> tiles, inner;
+    rtm::build_work(tiles, inner, 0, X, Z, TL, TR, TU, TD, rz, interior, seg_rows);
+    if ((int)tiles.size() > cap) return -1;
+    for (size_t i = 0; i < tiles.size(); i++) {
+        out[4 * i] = tiles[i].a; out[4 * i + 1] = tiles[i].b; out[4 * i + 2] = tiles[i].c; out[4 * i + 3] = tiles[i].d;
+    }
+    return (int)tiles.size();
+}
+
 /* rtm::balance_for_xcds over the tile lists of nsims same-sized grids (sim
  * ids 0..nsims-1, tile interior mode): out receives the reordered list as
  * {sim, x0, z0, class} quadruples (capacity `cap` items); returns its length
  * or -1. */
-int hx_balance(int X, int Z, int TL, int TR, int TU, int TD, int rz, int nsims, int *out, int cap)
+int hx_balance(int X, int Z, int TL, int TR, int TU, int TD, int rz, int nsims, int interior, int *out, int cap)
 {
     std::vector<rtm::WorkItem> tiles, inner;
-    for (int s = 0; s < nsims; s++) rtm::build_work(tiles, inner, s, X, Z, TL, TR, TU, TD, rz, rtm::IN_TILES, 0);
-    rtm::balance_for_xcds(tiles);
+    for (int s = 0; s < nsims; s++) rtm::build_work(tiles, inner, s, X, Z, TL, TR, TU, TD, rz, interior, 0);
+    rtm::balance_for_xcds(tiles, 4 * rz);
     if ((int)tiles.size() > cap) return -1;
     for (size_t i = 0; i < tiles.size(); i++) {
         out[4 * i] = tiles[i].a; out[4 * i + 1] = tiles[i].b; out[4 * i + 2] = tiles[i].c; out[4 * i + 3] = tiles[i].d;

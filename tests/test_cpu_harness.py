@@ -43,7 +43,7 @@ def test_forward_backward_bitexact(hx, name, dtype):
 
 @pytest.mark.parametrize("name", ["tiny", "asym", "wide"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (4, 1, 0), (2, 1, 12),
+@pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (4, 2, 0), (2, 2, 0), (4, 1, 0), (2, 1, 12),
                                              (1, 1, 4), (8, 1, 32)])
 def test_tiled_kernel_bitexact(hx, name, dtype, rz, interior, seg):
     """The step kernels' bodies run on the CPU against the oracle: the
@@ -229,44 +229,68 @@ def test_boundary_saving_reconstruction_catches_a_wrong_term(plant):
     assert max(np.linalg.norm(a - b) for a, b in zip(S_bad, S_ref)) > 1e-3 * big
 
 
-@pytest.mark.parametrize("nsims", [1, 3])
-def test_xcd_balance_keeps_every_tile_once(hx, nsims):
-    """The tile list reordered for the eight XCDs (rtm::balance_for_xcds):
-    every tile of build_work's list appears exactly once; the eight shares
-    have one length, padded with skipped entries; each tile column of a
-    simulation stays in one share; the shares' costs (strip tile 3, interior
-    1) are within one tile column's cost of the mean; inside each
-    simulation's run of a share the classes come corner, z-strip, x-strip,
-    interior, each row-major."""
+def _tiles(hx, fn, *args):
     import ctypes as C
-    for X, Z, TL, TR, TU, TD in [(1801, 451, 50, 50, 50, 50), (100, 71, 14, 20, 18, 13), (680, 280, 40, 40, 40, 40)]:
-        for rz in (1, 2, 4):
-            cap = 200000
-            buf = np.zeros(4 * cap, np.int32)
-            n = hx.hx_balance(X, Z, TL, TR, TU, TD, rz, nsims, buf.ctypes.data_as(C.c_void_p), cap)
-            assert n > 0 and n % 8 == 0
-            t = buf[:4 * n].reshape(n, 4)
+    cap = 400000
+    buf = np.zeros(4 * cap, np.int32)
+    n = fn(*args, buf.ctypes.data_as(C.c_void_p), cap)
+    assert n > 0
+    return buf[:4 * n].reshape(n, 4)
+
+
+GRIDS = [(1801, 451, 50, 50, 50, 50), (100, 71, 14, 20, 18, 13), (680, 280, 40, 40, 40, 40)]
+
+
+@pytest.mark.parametrize("rz", [2, 4])
+def test_mixed_tile_list_covers_the_stepped_region_once(hx, rz):
+    """The mixed list (full interior tiles, 64 x 4 tiles wherever a strip is
+    touched) covers every point of the stepped region exactly once, its full
+    tiles are all interior ones, and it has the same area as the uniform
+    list."""
+    for X, Z, TL, TR, TU, TD in GRIDS:
+        t = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, 2, 0)
+        cover = np.zeros((Z, X), np.int32)
+        for _, x0, z0, cls in t:
+            h = 4 if cls & 4 else 4 * rz
+            cover[z0:min(z0 + h, Z - 4), x0:min(x0 + 64, X - 4)] += 1
+        assert (cover[4:Z - 4, 4:X - 4] == 1).all()
+        assert cover.sum() == (Z - 8) * (X - 8)
+        full = t[(t[:, 3] & 4) == 0]
+        assert (full[:, 3] == 0).all() and len(full) < len(t)   # a narrow grid may have no full tile
+
+
+@pytest.mark.parametrize("nsims,interior", [(1, 0), (3, 0), (1, 2), (3, 2)])
+def test_xcd_balance_keeps_every_tile_once(hx, nsims, interior):
+    """The tile list reordered for the eight XCDs (rtm::balance_for_xcds):
+    it holds build_work's tiles each exactly once; the eight shares have one
+    length, padded with skipped entries; each tile column of a simulation
+    stays in one share; the shares' costs (strip 3, interior 1, a 64 x 4
+    tile by its size) are within one tile column's cost of the mean; inside
+    each simulation's run of a share, small tiles come before full ones and
+    the classes run corner, z-strip, x-strip, interior, each row-major."""
+    for X, Z, TL, TR, TU, TD in GRIDS:
+        for rz in ((1, 2, 4) if interior == 0 else (2, 4)):
+            t = _tiles(hx, hx.hx_balance, X, Z, TL, TR, TU, TD, rz, nsims, interior)
+            n = len(t)
+            assert n % 8 == 0
             shares = t.reshape(8, n // 8, 4)
             real = t[t[:, 0] >= 0]
             assert (t[t[:, 0] < 0][:, 1:] == 0).all()
+            ref = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, interior, 0)
+            want = sorted((s, *map(int, r[1:])) for s in range(nsims) for r in ref)
+            assert sorted(tuple(map(int, r)) for r in real) == want
             TZ = 4 * rz
-            across, down = (X - 8 + 63) // 64, (Z - 8 + TZ - 1) // TZ
-            assert len(real) == nsims * across * down
-            keys = {tuple(r[:3]) for r in real}
-            assert len(keys) == len(real)
-            assert keys == {(s, 4 + 64 * i, 4 + TZ * k) for s in range(nsims) for i in range(across) for k in range(down)}
+            cost_of = lambda r: (3 if r[3] & 3 else 1) * (4 / TZ if r[3] & 4 else 1)  # noqa: E731
             col_share = {}
             for k in range(8):
                 for r in shares[k]:
                     if r[0] >= 0:
                         assert col_share.setdefault((r[0], r[1]), k) == k
-            cost = [sum(3 if r[3] else 1 for r in shares[k] if r[0] >= 0) for k in range(8)]
-            col_cost = max(sum(3 if r[3] else 1 for r in real if r[0] == s and r[1] == x) for s, x in col_share)
-            mean = sum(cost) / 8
-            assert max(cost) - mean <= nsims * col_cost + 1e-9, cost
-            rank = {3: 0, 2: 1, 1: 2, 0: 3}
+            cost = [sum(cost_of(r) for r in shares[k] if r[0] >= 0) for k in range(8)]
+            col_cost = max(sum(cost_of(r) for r in real if r[0] == s and r[1] == x) for s, x in col_share)
+            assert max(cost) - sum(cost) / 8 <= nsims * col_cost + 1e-9, cost
             for k in range(8):
                 rows = [r for r in shares[k] if r[0] >= 0]
                 for s in range(nsims):
-                    mine = [(rank[r[3]], r[2], r[1]) for r in rows if r[0] == s]
+                    mine = [(-int(r[3]), int(r[2]), int(r[1])) for r in rows if r[0] == s]
                     assert mine == sorted(mine)

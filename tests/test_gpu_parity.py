@@ -104,11 +104,13 @@ def _wide_case():
     return cfg, model, wav, seis
 
 
-@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 0), (2, 1, 12), (1, 1, 4), (8, 1, 32), (4, 1, 16)])
+@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 0), (2, 1, 12), (1, 1, 4), (8, 1, 32), (4, 1, 16), (4, 2, 0),
+                                             (2, 2, 0)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
     """Strip tiles plus the lane-shift column kernel on the interior
-    (including a 64-column item): fields, recorded traces and images
+    (including a 64-column item), and mixed tile lists (64 x 4 tiles on the
+    strips beside full interior tiles): fields, recorded traces and images
     bit-identical to the oracle."""
     say(f"[gpu] interior={interior} rz={rz} rows={seg} {np.dtype(dtype).name}")
     cfg, model, wav, seis = _wide_case()
@@ -137,7 +139,7 @@ def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
-@pytest.mark.parametrize("interior,seg", [(0, 0), (1, 16)])
+@pytest.mark.parametrize("interior,seg", [(0, 0), (1, 16), (2, 0)])
 def test_batch_marched_bitexact(gpu_available, interior, seg):
     """A batch with a streaming interior kernel: every realization's image
     equals the oracle's, in the batch as alone."""
@@ -216,7 +218,7 @@ def _boundary_saving_expected(ts):
     return _BS_CACHE[ts]
 
 
-@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0), (5, 1)])
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0), (5, 1), (4, 2)])
 def test_boundary_saving_bitexact(gpu_available, ts, interior):
     """Boundary-saving snapshot mode: the forward pass saves only the ring
     around the image region each step and the backward pass rebuilds the
@@ -313,7 +315,7 @@ def test_batch_boundary_saving_bitexact(gpu_available):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
-@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0)])
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0), (4, 2)])
 def test_sixteen_bit_snapshots_bitexact(gpu_available, mode, ts, interior):
     """FP16 / BF16 snapshot store with per-snapshot max-abs scale: the image
     equals the oracle's backward pass over the oracle's FP32 snapshots put
