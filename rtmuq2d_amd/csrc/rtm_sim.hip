@@ -380,9 +380,10 @@ template <typename R>
 __global__ void k_quantize_interior(const R *__restrict__ u, int P, int TL, int TU, int nx, int nz,
                                     unsigned short *__restrict__ q, const unsigned *__restrict__ slots, int mode)
 {
+    // every lane of the wavefront takes part in reading the slots
+    const float scale = rtm::pow2f(14 - rtm::snap_exp(rtm::snap_max_bits(slots)));
     const int ii = blockIdx.x * blockDim.x + threadIdx.x, kk = blockIdx.y;
     if (ii >= nx || kk >= nz) return;
-    const float scale = rtm::pow2f(14 - rtm::snap_exp(rtm::snap_max_bits(slots)));
     q[(size_t)kk * nx + ii] = rtm::quant16(float(u[(size_t)(kk + TU) * P + ii + TL]), scale, mode);
 }
 

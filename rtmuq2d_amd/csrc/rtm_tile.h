@@ -220,24 +220,52 @@ RTM_HD unsigned short f32_to_bf16(float f)
 
 RTM_HD float bf16_to_f32(unsigned short b) { return bits_f32((unsigned)b << 16); }
 
+/* On the device the conversions are the hardware's (v_cvt_f16_f32,
+ * v_cvt_pk_bf16_f32, v_cvt_f32_f16): round to nearest even with subnormals
+ * kept (the kernels run with FP16/FP32 denormals on), the results of the
+ * integer restatements above, which the host uses. */
 RTM_HD unsigned short quant16(float v, float scale, int mode)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float x = v * scale;
+    return mode == Q_FP16 ? __builtin_bit_cast(unsigned short, (_Float16)x)
+                          : __builtin_bit_cast(unsigned short, (__bf16)x);
+#else
     return mode == Q_FP16 ? f32_to_f16(v * scale) : f32_to_bf16(v * scale);
+#endif
 }
 
 RTM_HD float dequant16(unsigned short q, float scale, int mode)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (mode == Q_FP16 ? (float)__builtin_bit_cast(_Float16, q) : bf16_to_f32(q)) * scale;
+#else
     return (mode == Q_FP16 ? f16_to_f32(q) : bf16_to_f32(q)) * scale;
+#endif
 }
 
+/* The max over a snapshot's kSnapSlots slots.  On the device every lane of
+ * the (fully active) wavefront loads one slot and a butterfly of lane shuffles
+ * takes the max, one memory round trip; the host loops. */
 RTM_HD unsigned snap_max_bits(const unsigned *slots)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned m = lane < kSnapSlots ? slots[lane * kSlotStride] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = (unsigned)__shfl_xor((int)m, off);
+        m = o > m ? o : m;
+    }
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)m);
+#else
     unsigned m = 0;
     for (int t = 0; t < kSnapSlots; t++) {
         const unsigned v = ldg(slots, t * kSlotStride);
         m = v > m ? v : m;
     }
     return m;
+#endif
 }
 
 template <typename R, int RZ>
@@ -601,15 +629,15 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             }
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
-            if ((Q16 ? (V.qsnap || V.smax) : (V.snap || V.sfield)) && img_col && kk >= 0 && kk < S.nz) {
+            const bool img_pt = img_col && kk >= 0 && kk < S.nz;
+            if (Q16 && V.forward) {
+                /* the snapshot max, kept on every step (the caller commits it on
+                 * store steps only); the 16-bit store of u^n follows the loop */
+                if (img_pt) umax = fmaxf(umax, fabsf(float(U)));
+            } else if ((Q16 ? V.qsnap != nullptr : (V.snap || V.sfield)) && img_pt) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
-                    if (!Q16) {
-                        if (V.snap) stg(V.snap, q, U);
-                    } else {
-                        if (V.smax) umax = fmaxf(umax, fabsf(float(U)));
-                        if (V.qsnap) stg(V.qsnap, q, quant16(float(uc), V.qscale, S.qmode));
-                    }
+                    if (V.snap) stg(V.snap, q, U);
                 } else if (S.flags & F_KAHAN) {
                     R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
                     const R s = G.sv[rr];
@@ -622,6 +650,19 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
                     stg(S.Is, q, G.Isv[rr] + s * s);
                 }
             }
+        }
+    }
+    if (Q16 && V.forward && V.qsnap && img_col) {
+        /* 16-bit store of u^n (the previous step's snapshot) from the tile's
+         * LDS copy, apart from the update loop: kept inside it, u^n and the
+         * store's operands stayed live across the loop and slowed every step */
+#pragma unroll
+        for (int rr = 0; rr < RZ; rr++) {
+            const int k = z0 + ty * RZ + rr;
+            const int kk = k - S.TU;
+            if (k >= S.Z - kHalf) break;
+            if (kk >= 0 && kk < S.nz)
+                stg(V.qsnap, kk * S.nx + (i - S.TL), quant16(float(L.u[ty * RZ + rr + 8][lc]), V.qscale, S.qmode));
         }
     }
     return umax;
