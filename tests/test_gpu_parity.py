@@ -671,13 +671,14 @@ def _server_requests():
     return out
 
 
-def test_server_each_result_same_alone_and_in_the_mix(gpu_available):
+@pytest.mark.parametrize("shape", [(4, 50, 1, 16), (4, 50, 2, 0), (2, 30, 0, 0)])
+def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape):
     """Continuous batching: requests of three grid sizes arrive while others
     are in flight, are admitted by HBM footprint (the arena holds about two at
     a time), share every launch, and each image equals the oracle's for that
     request alone (FP16 store rule applied), bit for bit; so it is also
-    independent of the mix."""
-    say("[gpu] request server: alone vs in the mix")
+    independent of the mix.  Column interior, mixed tiles and plain tiles."""
+    say(f"[gpu] request server: alone vs in the mix, shape {shape}")
     reqs = _server_requests()
     refs = []
     for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
@@ -687,7 +688,7 @@ def test_server_each_result_same_alone_and_in_the_mix(gpu_available):
     budget = int(sorted(fp)[-1] + sorted(fp)[-2] + (1 << 20))
     got = {}
     with R.Server(budget, max_slots=4, snapshot_mode=1) as srv:
-        srv.set_tiling(4, 50, 1, 16)
+        srv.set_tiling(*shape)
         ids = []
         ids.append(srv.submit(*_sub(reqs[0])))
         ids.append(srv.submit(*_sub(reqs[1])))
