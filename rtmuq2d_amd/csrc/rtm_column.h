@@ -96,10 +96,10 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         const int q = kk * S.nx + ii;
         if (V.forward) {
             if (!Q16) {
-                if (V.snap) stg(V.snap, q, vals[e]);
+                if (V.snap) stg_stream(V.snap, q, vals[e]);
             } else {
                 if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
-                if (V.qsnap) stg(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
+                if (V.qsnap) stg_stream(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
             }
         } else {
             const R s = Q16        ? R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode))

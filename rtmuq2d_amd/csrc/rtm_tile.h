@@ -63,6 +63,20 @@ RTM_HD void stg(T *p, int i, T v)
     *(RTM_GAS T *)((RTM_GAS char *)p + off) = v;
 }
 
+/* A store the pass does not read back (snapshot stores): on the device a
+ * non-temporal store, which spares the caches for the fields every step
+ * re-reads (measured 3% faster on the c1 forward step at TS=1; the matching
+ * non-temporal load of the backward pass was slower and is not used). */
+template <typename T>
+RTM_HD void stg_stream(T *p, int i, T v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, p + i);
+#else
+    p[i] = v;
+#endif
+}
+
 RTM_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 enum SimPhase : int { PH_IDLE = 0, PH_FORWARD = 1, PH_BACKWARD = 2 };
@@ -637,7 +651,7 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             } else if ((Q16 ? V.qsnap != nullptr : (V.snap || V.sfield)) && img_pt) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
-                    if (V.snap) stg(V.snap, q, U);
+                    if (V.snap) stg_stream(V.snap, q, U);
                 } else if (S.flags & F_KAHAN) {
                     R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
                     const R s = G.sv[rr];
@@ -662,7 +676,8 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             const int kk = k - S.TU;
             if (k >= S.Z - kHalf) break;
             if (kk >= 0 && kk < S.nz)
-                stg(V.qsnap, kk * S.nx + (i - S.TL), quant16(float(L.u[ty * RZ + rr + 8][lc]), V.qscale, S.qmode));
+                stg_stream(V.qsnap, kk * S.nx + (i - S.TL),
+                           quant16(float(L.u[ty * RZ + rr + 8][lc]), V.qscale, S.qmode));
         }
     }
     return umax;
