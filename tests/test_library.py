@@ -212,3 +212,34 @@ def test_driver_params_file(tmp_path):
         rc, _, err = _dry("--params", str(f))
         assert rc == 2 and err, bad
     assert _dry("--ranks", "2", "--rank", "2")[0] == 2
+
+
+def _c_blocks(path):
+    text = open(path).read()
+    return re.findall(r"```c\n(.*?)```", text, flags=re.S)
+
+
+def test_integration_guide_c_examples_compile(tmp_path):
+    """The C examples of the integration guide compile against
+    include/rtm_hip.h and link against the library: the device-resident API
+    and the request server in a function body, and, where the reference's
+    sources are present, the rtm_apply_fixed_parameters translation unit
+    against the reference's own fixed_parameters.h."""
+    blocks = _c_blocks(os.path.join(common.REPO, "INTEGRATION.md"))
+    assert len(blocks) == 3
+    fixed, api, server = blocks
+    body = api.replace('#include "rtm_hip.h"\n', "")
+    src = tmp_path / "api.c"
+    src.write_text("#include <stdio.h>\n#include \"rtm_hip.h\"\n"
+                   "int example(float *vpad, float *wavelet, float *seis, float *I, float *Is, float *image,\n"
+                   "            int shot_x, int shot_z) {\n" + body + "\n" + server + "\nreturn 0;\n}\n")
+    lib = os.path.dirname(R.LIB_PATH)
+    subprocess.run(["gcc", "-std=gnu99", "-Wall", "-Werror", "-Wno-unused-variable", "-shared", "-fPIC",
+                    "-I", os.path.join(common.REPO, "include"), str(src), "-o", str(tmp_path / "api.so"),
+                    "-L", lib, "-lrtm_hip"], check=True)
+    ref_src = "/root/reference/SRC"
+    if os.path.exists(os.path.join(ref_src, "fixed_parameters.h")):
+        unit = tmp_path / "rtm_params_from_fixed.c"
+        unit.write_text(fixed)
+        subprocess.run(["gcc", "-std=gnu99", "-Wall", "-c", "-I", ref_src, "-I", os.path.join(common.REPO, "include"),
+                        str(unit), "-o", str(tmp_path / "fixed.o")], check=True)
