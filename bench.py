@@ -41,7 +41,7 @@ BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
 # mean of a forward and a backward step at TS=1: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --profile`, each scaled by what the
 # counter reads for a 1 GiB copy in the same run (x2.0 and x1.0).
-C1_TRAFFIC_BYTES = 26434272
+C1_TRAFFIC_BYTES = 26542751
 
 
 def full_step_bytes(p, ts):
