@@ -550,12 +550,18 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         b.set_velocity(k, v)
     b.set_wavelet(w)
     b.set_seismogram(seis)
+    # launch groups: each candidate group size shape-tuned, the least time
+    # per realization-step kept; then the shape table at that group size
+    gsweep = b.autotune_group(200)
+    for t in gsweep:
+        log(f"[bench] batch group {t['group']}: {t['us_per_sim_step']:.2f} us per realization-step")
     sweep = b.autotune(200)
     for t in sweep:
         log(f"[bench] batch autotune {tiling_dict((t['rows_per_thread'], t['graph_block'], t['interior'], t['seg_rows']))}: "
             f"{t['ms_per_step'] * 1e3:.1f} us/step")
     bbest = (None, b.tiling())
-    log(f"[bench] batch using {tiling_dict(bbest[1])}")
+    gsize, ngroups = b.group()
+    log(f"[bench] batch using {tiling_dict(bbest[1])}, {ngroups} launch groups of {gsize}")
     b.zero_images()
     t0 = time.perf_counter()
     b.migrate(sx, sz)
@@ -581,6 +587,9 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "batched_over_plain": round(batch_iph / plain_iph, 3),
         "step_kernel_us": {"forward": round(f_ms * 1e3, 2), "backward": round(b_ms * 1e3, 2)},
         "tiling": tiling_dict(bbest[1]),
+        "launch_groups": {"sims_per_group": gsize, "groups": ngroups,
+                          "autotune_us_per_sim_step": {str(t["group"]): round(t["us_per_sim_step"], 3)
+                                                       for t in gsweep}},
         "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
                           us_per_step=round(t["ms_per_step"] * 1e3, 2)) for t in sweep],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

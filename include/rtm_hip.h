@@ -201,8 +201,9 @@ int rtm_plan_last_timing(rtm_plan *plan, double *fwd_step_ms, double *bwd_step_m
 /* Ensemble of nsims realizations of one geometry (the reference's
  * simulations, SRC/mpi_rtm_iso2d.c:39-51, here on one device): every
  * realization has its own velocity, wavefields, CPML state, snapshot store
- * and image; the shot gather (wavelet, traces) is shared.  All realizations
- * advance together, one tiled launch per time step in hipGraph blocks. */
+ * and image; the shot gather (wavelet, traces) is shared.  The realizations
+ * of a launch group advance together, one tiled launch per time step in
+ * hipGraph blocks. */
 typedef struct rtm_batch rtm_batch;
 
 int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_batch **out);
@@ -211,8 +212,21 @@ size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real);
 int rtm_batch_size(rtm_batch *batch);
 int rtm_batch_set_tiling(rtm_batch *batch, int rows_per_thread, int graph_block, int interior, int seg_rows);
 int rtm_batch_get_tiling(rtm_batch *batch, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows);
-/* The autotuner over all realizations of the batch (as rtm_plan_autotune). */
+/* The autotuner over the realizations of one launch group (as
+ * rtm_plan_autotune; ms_per_step is the time per step of that group). */
 int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double *ms_per_step, int *n_out);
+/* Launch groups (no reference counterpart): the realizations run in groups
+ * of sims_per_group (0 or >= the batch size: one group), each group's
+ * forward and backward passes before the next group's.  Results do not
+ * depend on the grouping.  A group whose per-step working set fits the
+ * device's last-level cache steps faster than one that does not. */
+int rtm_batch_set_group(rtm_batch *batch, int sims_per_group);
+int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
+/* Shape-tunes each group size (powers of two from 4 below the batch size,
+ * and the whole batch) and keeps the one with the least time per
+ * realization-step; row k = group_sizes[k], us_per_sim_step[k]. */
+int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_sizes, double *us_per_sim_step,
+                             int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
 int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
 int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */

@@ -119,6 +119,9 @@ def lib() -> C.CDLL:
         "rtm_plan_autotune": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
         "rtm_batch_get_tiling": (C.c_int, [plan_p, i32p, i32p, i32p, i32p]),
         "rtm_batch_autotune": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
+        "rtm_batch_set_group": (C.c_int, [plan_p, C.c_int]),
+        "rtm_batch_get_group": (C.c_int, [plan_p, i32p, i32p]),
+        "rtm_batch_autotune_group": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
         "rtm_plan_set_velocity": (C.c_int, [plan_p, f32p]),
         "rtm_plan_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
         "rtm_plan_set_seismogram": (C.c_int, [plan_p, f32p]),
@@ -399,7 +402,30 @@ class Batch:
         return _get_tiling(lib().rtm_batch_get_tiling, self.h)
 
     def autotune(self, steps: int = 200):
+        """rtm_batch_autotune: shapes timed on launch group 0 at the current
+        group size (ms_per_step is that group's)."""
         return _autotune(lib().rtm_batch_autotune, self.h, steps)
+
+    def set_group(self, sims_per_group: int) -> None:
+        """rtm_batch_set_group: realizations per launch group (0: all)."""
+        _check(lib().rtm_batch_set_group(self.h, int(sims_per_group)), "set_group")
+
+    def group(self):
+        """(realizations per launch group, number of groups)."""
+        g, n = C.c_int(), C.c_int()
+        _check(lib().rtm_batch_get_group(self.h, C.byref(g), C.byref(n)), "get_group")
+        return g.value, n.value
+
+    def autotune_group(self, steps: int = 200):
+        """rtm_batch_autotune_group: shape-tune each candidate group size and
+        keep the group with the least time per realization-step."""
+        cap = 32
+        gs = np.zeros(cap, np.int32)
+        us = np.zeros(cap, np.float64)
+        n = C.c_int()
+        _check(lib().rtm_batch_autotune_group(self.h, int(steps), cap, iptr(gs), dptr(us), C.byref(n)),
+               "autotune_group")
+        return [dict(group=int(g), us_per_sim_step=float(t)) for g, t in zip(gs[:n.value], us[:n.value])]
 
     def set_velocity(self, sim: int, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)

@@ -1,6 +1,13 @@
 // rtm_batch.hip -- an ensemble of simulations on one device, advanced in
 // lockstep by one tiled launch per time step (hipGraph blocks of steps).
 //
+// The realizations run in launch groups: each group's launches cover that
+// group's realizations only, and a group finishes its forward and backward
+// passes before the next group starts.  A group whose per-step working set
+// (fields, velocity, CPML state) fits the 256 MB last-level cache (MALL)
+// steps from cache instead of HBM; 16 realizations of the 1701x351 grid do,
+// 64 do not.  The group size is the caller's choice or the autotuner's.
+//
 // This is the on-device form of the reference's "embarrassingly parallel
 // over realizations" driver (SRC/mpi_rtm_iso2d.c:39-51: MPI ranks each run a
 // range of simulation ids with no messages): every realization keeps its own
@@ -8,6 +15,7 @@
 // gather (wavelet, traces, receiver tables, CPML profiles) is shared.  The
 // images are reduced across realizations with a Welford mean/variance in
 // FP64, in realization order.
+#include <algorithm>
 #include <cstdlib>
 
 #include "rtm_sim.h"
@@ -39,7 +47,9 @@ __global__ void k_welford(const R *const *__restrict__ imgs, int N, size_t n, do
 
 struct rtm_batch {
     std::vector<SimBuffers> sims;
-    Exec ex;
+    // one executor per launch group; group k holds sims [k*group, k*group + ex[k].nsims)
+    std::vector<Exec> ex;
+    int group = 0;                      // realizations per launch group (0: all)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int real = RTM_REAL_F32;
@@ -50,7 +60,8 @@ struct rtm_batch {
 
 static void batch_free(rtm_batch *q)
 {
-    q->ex.release();
+    for (auto &e : q->ex) e.release();
+    q->ex.clear();
     for (auto &s : q->sims) s.release();
     q->sims.clear();
     if (q->ev0) (void)hipEventDestroy(q->ev0);
@@ -58,16 +69,29 @@ static void batch_free(rtm_batch *q)
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
+static int group_size(const rtm_batch *q)
+{
+    const int N = (int)q->sims.size();
+    return q->group > 0 && q->group < N ? q->group : N;
+}
+
 static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
 {
-    std::vector<int4> tiles, inner;
-    for (int s = 0; s < (int)q->sims.size(); s++)
-        append_work(tiles, inner, s, q->sims[s].p, rz, interior, seg_rows);
-    balance_tiles_for_xcds(tiles, 4 * rz);
-    q->ex.release();
-    if (q->ex.init(q->real, (int)q->sims.size(), (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
-    q->ex.q16 = q->sims[0].qmode != rtm::Q_NONE;
-    if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
+    const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
+    for (auto &e : q->ex) e.release();
+    q->ex.clear();
+    q->ex.resize(ng);       // never resized again before the next release
+    for (int k = 0; k < ng; k++) {
+        const int s0 = k * g, n = std::min(g, N - s0);
+        std::vector<int4> tiles, inner;
+        for (int s = 0; s < n; s++)
+            append_work(tiles, inner, s, q->sims[s0 + s].p, rz, interior, seg_rows);
+        balance_tiles_for_xcds(tiles, 4 * rz);
+        Exec &ex = q->ex[k];
+        if (ex.init(q->real, n, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+        ex.q16 = q->sims[0].qmode != rtm::Q_NONE;
+        if (ex.set_tiles(tiles, q->stream) || ex.set_inner(inner, q->stream)) return -1;
+    }
     q->rz = rz;
     q->graph_block = block;
     q->interior = interior;
@@ -116,6 +140,7 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     const char *env = getenv("RTM_TILE_RZ");
     Shape sh{env && *env ? atoi(env) : 4, rtm::IN_TILES, 0};
     if (!(env && *env)) tuner_lookup(batch_key(q), &sh);
+    q->group = sh.group;
     if (batch_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
     *out = q;
     return 0;
@@ -176,13 +201,16 @@ extern "C" int rtm_batch_zero_images(rtm_batch *q)
     return 0;
 }
 
+// One pass of launch group k: every step advances that group's realizations.
+// The pass's event-timed duration is added to fwd_ms or bwd_ms.
 template <typename R>
-static int batch_pass(rtm_batch *q, int phase, int src_j)
+static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
 {
-    const int N = (int)q->sims.size();
+    Exec &ex = q->ex[k];
+    const int s0 = k * group_size(q), N = ex.cap_sims;
     std::vector<SimDesc<R>> d(N);
     for (int s = 0; s < N; s++) {
-        SimBuffers &b = q->sims[s];
+        SimBuffers &b = q->sims[s0 + s];
         b.reset_state(q->stream, phase == rtm::PH_FORWARD && b.ringmode);
         if (phase == rtm::PH_FORWARD) b.reset_snap_scales(q->stream);
         d[s] = b.desc<R>();
@@ -199,27 +227,28 @@ static int batch_pass(rtm_batch *q, int phase, int src_j)
             if (b.ringmode) d[s].src_j = src_j;   // only the reconstruction reads it in this phase
         }
     }
-    const SimBuffers &b0 = q->sims[0];
-    q->ex.set_ring_phase(b0.ringmode ? phase : 0,
-                         (int)((phase == rtm::PH_FORWARD ? b0.NR : b0.nint + b0.NR) + 255) / 256);
-    if (q->ex.upload_descs(d.data(), N, q->stream)) return -1;
-    if (q->ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
-    const int nt = q->sims[0].p.number_of_time_steps;
+    const SimBuffers &b0 = q->sims[s0];
+    ex.set_ring_phase(b0.ringmode ? phase : 0,
+                      (int)((phase == rtm::PH_FORWARD ? b0.NR : b0.nint + b0.NR) + 255) / 256);
+    if (ex.upload_descs(d.data(), N, q->stream)) return -1;
+    if (ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
+    const int nt = b0.p.number_of_time_steps;
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
-    if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+    if (ex.run(nt, q->graph_block, q->stream)) return -1;
     if (phase == rtm::PH_FORWARD)
         for (int s = 0; s < N; s++)
-            if (q->sims[s].flush_last_snapshot(q->stream)) return -1;
+            if (q->sims[s0 + s].flush_last_snapshot(q->stream)) return -1;
     HIP_TRY(hipEventRecord(q->ev1, q->stream));
     HIP_TRY(hipEventSynchronize(q->ev1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
-    (phase == rtm::PH_FORWARD ? q->fwd_ms : q->bwd_ms) = ms;
+    (phase == rtm::PH_FORWARD ? q->fwd_ms : q->bwd_ms) += ms;
     q->last_steps = nt;
     return 0;
 }
 
-// Forward then backward for every realization, one shot, images accumulated.
+// Forward then backward for every realization, one shot, images accumulated;
+// launch group by launch group.
 extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
 {
     if (!q) { rtm_set_error("rtm_batch_migrate: null batch"); return -1; }
@@ -239,11 +268,15 @@ extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
         return -1;
     }
     const int src_j = zs * b0.P + xs;
-    int rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_FORWARD, src_j)
-                                     : batch_pass<float>(q, rtm::PH_FORWARD, src_j);
-    if (rc == 0)
-        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, rtm::PH_BACKWARD, src_j)
-                                     : batch_pass<float>(q, rtm::PH_BACKWARD, src_j);
+    q->fwd_ms = q->bwd_ms = 0.0;
+    int rc = 0;
+    for (int k = 0; k < (int)q->ex.size() && rc == 0; k++) {
+        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, k, rtm::PH_FORWARD, src_j)
+                                     : batch_pass<float>(q, k, rtm::PH_FORWARD, src_j);
+        if (rc == 0)
+            rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, k, rtm::PH_BACKWARD, src_j)
+                                         : batch_pass<float>(q, k, rtm::PH_BACKWARD, src_j);
+    }
     for (auto &s : q->sims) s.have_snaps = true;
     return rc;
 }
@@ -330,19 +363,22 @@ extern "C" int rtm_batch_welford(rtm_batch *q, double *mean, double *var)
     return q->real == RTM_REAL_F64 ? batch_welford_t<double>(q, mean, var) : batch_welford_t<float>(q, mean, var);
 }
 
+// Times every candidate shape on launch group 0 (its realizations stepping
+// together, no snapshot or image work) at the current group size; keeps the
+// fastest.  *best_ms: its time per step of group 0.
 template <typename R>
-static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, double *ms, int *n_out)
+static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, double *ms, int *n_out, double *best_ms)
 {
     const auto &cand = tuner_candidates();
     double best_t = 1e30;
     Shape best = cand[0];
     int k = 0;
     std::vector<SimDesc<R>> d;
-    for (const auto &s : q->sims) d.push_back(s.desc<R>());
+    for (int s = 0; s < group_size(q); s++) d.push_back(q->sims[s].desc<R>());
     for (const Shape &c : cand) {
         if (batch_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
         double t = 0.0;
-        if (time_steps<R>(q->ex, d, steps, q->graph_block, q->stream, &t)) return -1;
+        if (time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, &t)) return -1;
         if (k < cap) {
             if (shapes) {
                 shapes[4 * k] = c.rz; shapes[4 * k + 1] = q->graph_block;
@@ -355,7 +391,9 @@ static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, doubl
     }
     if (n_out) *n_out = k < cap ? k : cap;
     if (batch_set_tiling(q, best.rz, q->graph_block, best.interior, best.seg)) return -1;
+    best.group = q->group;
     tuner_store(batch_key(q), best);
+    if (best_ms) *best_ms = best_t;
     return 0;
 }
 
@@ -364,8 +402,67 @@ extern "C" int rtm_batch_autotune(rtm_batch *q, int steps, int cap, int *shapes,
     if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune: bad argument"); return -1; }
     HIP_TRY(hipSetDevice(q->sims[0].p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
-    return q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, cap, shapes, ms_per_step, n_out)
-                                   : batch_autotune_t<float>(q, steps, cap, shapes, ms_per_step, n_out);
+    return q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, cap, shapes, ms_per_step, n_out, nullptr)
+                                   : batch_autotune_t<float>(q, steps, cap, shapes, ms_per_step, n_out, nullptr);
+}
+
+extern "C" int rtm_batch_set_group(rtm_batch *q, int sims_per_group)
+{
+    if (!q || sims_per_group < 0) { rtm_set_error("rtm_batch_set_group: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    q->group = sims_per_group >= (int)q->sims.size() ? 0 : sims_per_group;
+    return batch_set_tiling(q, q->rz, q->graph_block, q->interior, q->seg_rows);
+}
+
+extern "C" int rtm_batch_get_group(rtm_batch *q, int *sims_per_group, int *groups)
+{
+    if (!q) { rtm_set_error("rtm_batch_get_group: null batch"); return -1; }
+    if (sims_per_group) *sims_per_group = group_size(q);
+    if (groups) *groups = (int)q->ex.size();
+    return 0;
+}
+
+// Group sizes tried: every power of two from 4 below the batch size, and the
+// whole batch.  Each is shape-tuned; the group with the least time per
+// realization-step wins, with its best shape.
+extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *group_sizes, double *us_per_sim_step,
+                                        int *n_out)
+{
+    if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune_group: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    const int N = (int)q->sims.size();
+    std::vector<int> cand;
+    for (int g = 4; g < N; g *= 2) cand.push_back(g);
+    cand.push_back(N);
+    double best_t = 1e30;
+    int best_g = N, k = 0;
+    Shape best_shape{q->rz, q->interior, q->seg_rows};
+    for (int g : cand) {
+        q->group = g >= N ? 0 : g;
+        double t = 0.0;
+        const int rc = q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, 0, nullptr, nullptr, nullptr, &t)
+                                               : batch_autotune_t<float>(q, steps, 0, nullptr, nullptr, nullptr, &t);
+        if (rc) return -1;
+        const double per = t * 1e3 / group_size(q);
+        if (k < cap) {
+            if (group_sizes) group_sizes[k] = g;
+            if (us_per_sim_step) us_per_sim_step[k] = per;
+        }
+        k++;
+        if (per < best_t) {
+            best_t = per;
+            best_g = g;
+            best_shape = Shape{q->rz, q->interior, q->seg_rows};
+        }
+    }
+    if (n_out) *n_out = k < cap ? k : cap;
+    q->group = best_g >= N ? 0 : best_g;
+    if (batch_set_tiling(q, best_shape.rz, q->graph_block, best_shape.interior, best_shape.seg)) return -1;
+    best_shape.group = q->group;
+    tuner_store(batch_key(q), best_shape);
+    return 0;
 }
 
 extern "C" int rtm_batch_get_tiling(rtm_batch *q, int *rows_per_thread, int *graph_block, int *interior, int *seg_rows)

@@ -693,6 +693,7 @@ struct Exec {
 
 struct Shape {
     int rz, interior, seg;
+    int group = 0;      // batches: realizations per launch group (0: all)
 };
 
 // the shapes tried: tile heights 4..32 rows, and the column kernel beside

@@ -163,12 +163,13 @@ def test_batch_marched_bitexact(gpu_available, interior, seg):
             assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
 
 
-@pytest.mark.parametrize("rz", [2, 8])
-def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
+@pytest.mark.parametrize("rz,group", [(2, 0), (8, 0), (4, 2), (2, 4)])
+def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group):
     """A batch of realizations advanced in lockstep: each image bit-identical
-    to the oracle's for that realization (alone or in the batch), and the
+    to the oracle's for that realization (alone or in the batch, in one
+    launch group or in groups of 2 or 4 with a ragged last group), and the
     FP64 Welford mean/variance equal to numpy's over the same images."""
-    say(f"[gpu] batch ensemble rz={rz}")
+    say(f"[gpu] batch ensemble rz={rz} group={group}")
     cfg = common.REF_CONFIGS["asym"]
     g = golden("asym")
     op = common.orc_params(cfg)
@@ -182,6 +183,8 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
     refs = [O.rtm_shot(op, sx, sz, v, g["wavelet"], d) for v in vpads]
     with R.Batch(p, len(models)) as b:
         b.set_tiling(rz, 100)
+        b.set_group(group)
+        assert b.group() == ((group, (len(models) + group - 1) // group) if group else (len(models), 1))
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
         b.set_wavelet(g["wavelet"])
@@ -196,6 +199,40 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz):
     stack = np.stack([r[0].astype(np.float64) for r in refs])
     assert np.allclose(mean, stack.mean(0), rtol=1e-13, atol=1e-13 * np.abs(stack).max())
     assert np.allclose(var, stack.var(0, ddof=1), rtol=1e-10, atol=1e-12 * stack.var(0).max())
+
+
+def test_batch_group_autotune_keeps_results(gpu_available):
+    """The group autotuner tries every candidate group size, keeps one of them
+    with its best shape, and the images stay bit-identical to the oracle's;
+    a new batch of the same grids starts from the tuned group."""
+    say("[gpu] batch group autotune")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    sx, sz = common.shot_positions(cfg)[0]
+    d = g["seismograms"][0]
+    vpads = [O.extend(op, g["models"][0] * np.float32(1 + 0.04 * k)) for k in range(9)]
+    refs = [O.rtm_shot(op, sx, sz, v, g["wavelet"], d) for v in vpads[:2]]
+    with R.Batch(p, len(vpads)) as b:
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(g["wavelet"])
+        b.set_seismogram(d)
+        rows = b.autotune_group(20)
+        assert [r["group"] for r in rows] == [4, 8, 9]
+        assert all(r["us_per_sim_step"] > 0 for r in rows)
+        best = min(rows, key=lambda r: r["us_per_sim_step"])["group"]
+        gsize, ngroups = b.group()
+        assert gsize == best and ngroups == (len(vpads) + best - 1) // best
+        tiling = b.tiling()
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k in range(2):
+            I, Is = b.image(k)
+            assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
+    with R.Batch(p, len(vpads)) as b2:
+        assert b2.group() == (gsize, ngroups) and b2.tiling() == tiling
 
 
 _BS_CACHE = {}
@@ -287,10 +324,12 @@ def test_boundary_saving_records_traces(gpu_available, spacing):
             assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
-def test_batch_boundary_saving_bitexact(gpu_available):
+@pytest.mark.parametrize("group", [0, 2])
+def test_batch_boundary_saving_bitexact(gpu_available, group):
     """A batch in boundary-saving mode: each realization's image equals the
-    oracle's backward pass over its own reconstructed source field."""
-    say("[gpu] batch boundary saving")
+    oracle's backward pass over its own reconstructed source field (one
+    launch group, or groups of 2 and 1)."""
+    say(f"[gpu] batch boundary saving group={group}")
     cfg = common.REF_CONFIGS["asym"]
     g = golden("asym")
     op = common.orc_params(cfg, time_slices=4)
@@ -300,6 +339,7 @@ def test_batch_boundary_saving_bitexact(gpu_available):
     vpads = [O.extend(op, g["models"][0] * np.float32(1 + 0.05 * k)) for k in range(3)]
     with R.Batch(p, len(vpads)) as b:
         b.set_tiling(4, 100, 1, 16)
+        b.set_group(group)
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
         b.set_wavelet(g["wavelet"])
