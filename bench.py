@@ -454,9 +454,13 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
     with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
         srv.set_tiling(*shape)
         res = serve.run(srv, reqs)
-        # capacity: the same requests all waiting at t = 0
+        # capacity: the same requests all waiting at t = 0, in launch groups
+        # (the default budget) and then with every active request in one group
         ncap = min(nreq, 100)
         cap = serve.run(srv, [serve.Request(0.0, r.make, r.label) for r in reqs[:ncap]])
+        group_budget = srv.group_bytes()[0]
+        srv.set_group_bytes(0)
+        cap1 = serve.run(srv, [serve.Request(0.0, r.make, r.label) for r in reqs[:ncap]])
     cap_plain_s = float(np.sum([service[c] for c in classes[:ncap]]))
     seq = serve.sequential_latencies(arrivals, [service[c] for c in classes])
     seq_wall = max(a + l for a, l in zip(sorted(arrivals), seq))
@@ -484,9 +488,12 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
         "plain_images_per_hour": round(ncap * 3600.0 / cap_plain_s, 1),
         "served_over_plain": round(cap_plain_s / cap["wall_seconds"], 3),
         "max_active": cap["max_active"],
+        "launch_group_bytes": group_budget,
+        "one_group_served_seconds": round(cap1["wall_seconds"], 3),
     }
     log(f"[bench] c4 capacity ({ncap} requests waiting): served {cap['wall_seconds']:.2f} s vs plain sequential "
-        f"{cap_plain_s:.2f} s ({out['capacity']['served_over_plain']}x)")
+        f"{cap_plain_s:.2f} s ({out['capacity']['served_over_plain']}x); one launch group "
+        f"{cap1['wall_seconds']:.2f} s")
     log(f"[bench] c4 served {nreq} requests in {res['wall_seconds']:.1f} s: {out['images_per_hour']:.0f} "
         f"images/h, p50/p95/p99 {out['latency_s']['p50']:.2f}/{out['latency_s']['p95']:.2f}/"
         f"{out['latency_s']['p99']:.2f} s; plain sequential {out['plain_sequential']['images_per_hour']:.0f} "

@@ -274,6 +274,13 @@ int rtm_server_next_done(rtm_server *server, int *id);
 /* Copy a completed request's image ([Nx][Nz], n = Nx*Nz floats) and forget it. */
 int rtm_server_take_image(rtm_server *server, int id, float *image, size_t n);
 int rtm_server_stats(rtm_server *server, int *active, int *queued, size_t *bytes_in_use, long long *steps);
+/* Launch groups: the active requests advance in groups whose per-step
+ * working sets (16 B per padded grid point) sum to at most `bytes`, group
+ * after group, each a block of steps; 0 puts every active request in one
+ * group.  The default, 224 MiB, keeps a group within the device's 256 MB
+ * last-level cache.  Results do not depend on the grouping. */
+int rtm_server_set_group_bytes(rtm_server *server, size_t bytes);
+int rtm_server_get_group_bytes(rtm_server *server, size_t *bytes, int *groups);
 
 #ifdef __cplusplus
 }

@@ -159,6 +159,8 @@ def lib() -> C.CDLL:
         "rtm_server_next_done": (C.c_int, [plan_p, i32p]),
         "rtm_server_take_image": (C.c_int, [plan_p, C.c_int, f32p, C.c_size_t]),
         "rtm_server_stats": (C.c_int, [plan_p, i32p, i32p, P(C.c_size_t), P(C.c_longlong)]),
+        "rtm_server_set_group_bytes": (C.c_int, [plan_p, C.c_size_t]),
+        "rtm_server_get_group_bytes": (C.c_int, [plan_p, P(C.c_size_t), i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -514,6 +516,17 @@ class Server:
                    seg_rows: int = 0) -> None:
         _check(lib().rtm_server_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
                                            int(seg_rows)), "set_tiling")
+
+    def set_group_bytes(self, nbytes: int) -> None:
+        """rtm_server_set_group_bytes: per-step working-set budget of a launch
+        group (0: every active request in one group)."""
+        _check(lib().rtm_server_set_group_bytes(self.h, int(nbytes)), "set_group_bytes")
+
+    def group_bytes(self):
+        """(launch-group budget in bytes, launch groups at the last tick)."""
+        b, n = C.c_size_t(), C.c_int()
+        _check(lib().rtm_server_get_group_bytes(self.h, C.byref(b), C.byref(n)), "get_group_bytes")
+        return b.value, n.value
 
     def submit(self, p: RtmParams, vpad: np.ndarray, wavelet: np.ndarray, seis: np.ndarray, shot_x: int,
                shot_z: int) -> int:

@@ -16,6 +16,14 @@
 // alone (the same kernels on the same per-request state), so its image does
 // not depend on what else is in flight.
 //
+// The active requests advance in launch groups: greedily, in slot order,
+// groups whose per-step working sets (16 B per padded grid point: u^n,
+// u^{n-1}, C and the update) sum to at most group_bytes.  Each group has its
+// own executor (work lists, captured graph) and runs its block of steps
+// before the next group runs its own, so each launch's working set can stay
+// in the device's 256 MB last-level cache.  Grouping changes only which
+// requests share a launch, never a request's arithmetic.
+//
 // The reference runs one simulation per MPI rank, each start to finish
 // (SRC/mpi_rtm_iso2d.c:39-51); this is the same work admitted as it arrives.
 #include <deque>
@@ -27,6 +35,9 @@
 using namespace rtmd;
 
 namespace {
+
+// default launch-group budget: 16 grids of 1801 x 451 padded points
+constexpr size_t kGroupBytes = size_t(224) << 20;
 
 struct Request {
     int id = -1;
@@ -48,7 +59,8 @@ struct rtm_server {
     int max_slots = 0, block = 100;
     int rz = 4, interior = rtm::IN_TILES, seg_rows = 0;
     DeviceArena arena;
-    Exec ex;
+    size_t group_bytes = kGroupBytes;
+    std::vector<std::unique_ptr<Exec>> ex;      // one executor per launch group
     hipStream_t stream = nullptr;
     int next_id = 0;
     std::deque<std::unique_ptr<Request>> queue;
@@ -72,7 +84,8 @@ static void server_free(rtm_server *s)
     for (auto &r : s->slots)
         if (r) r->b.release();
     s->slots.clear();
-    s->ex.release();
+    for (auto &e : s->ex) e->release();
+    s->ex.clear();
     s->arena.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
@@ -99,8 +112,6 @@ extern "C" int rtm_server_create(int device, size_t hbm_bytes, int max_slots, in
         return fail();
     }
     if (s->arena.init(hbm_bytes)) return fail();
-    if (s->ex.init(RTM_REAL_F32, max_slots, 1, s->rz, 1, s->interior)) return fail();
-    s->ex.q16 = snapshot_mode != RTM_SNAP_FP32;
     *out = s;
     return 0;
 }
@@ -237,7 +248,6 @@ static int server_admit(rtm_server *s)
         r.phase = rtm::PH_FORWARD;
         r.n = 0;
         s->descs[slot] = forward_desc(r);
-        s->ex.h_n0[slot] = 0;
         s->slots[slot] = std::move(s->queue.front());
         s->queue.pop_front();
         s->lists_dirty = s->descs_dirty = true;
@@ -246,24 +256,44 @@ static int server_admit(rtm_server *s)
     return admitted;
 }
 
+// bytes a request's step streams: u^n, u^{n-1}, C read and u^{n+1} written
+static size_t step_bytes(const Request &r) { return r.b.nfield * 16; }
+
 static int server_rebuild(rtm_server *s)
 {
-    std::vector<int4> tiles, inner;
-    for (int i = 0; i < s->max_slots; i++)
-        if (s->slots[i]) append_work(tiles, inner, i, s->slots[i]->p, s->rz, s->interior, s->seg_rows);
-    balance_tiles_for_xcds(tiles, 4 * s->rz);
-    if ((int)tiles.size() > s->ex.cap_tiles || (int)inner.size() > s->ex.cap_inner || s->ex.rz != s->rz ||
-        s->ex.interior != s->interior) {
-        std::vector<int> n0 = s->ex.h_n0;
-        s->ex.release();
-        const int ct = (int)(tiles.size() * 3 / 2) + 64, ci = (int)(inner.size() * 3 / 2) + 64;
-        if (s->ex.init(RTM_REAL_F32, s->max_slots, ct, s->rz, ci, s->interior)) return -1;
-        s->ex.q16 = s->snapshot_mode != RTM_SNAP_FP32;
-        s->ex.h_n0 = n0;
-        s->descs_dirty = true;   // a fresh executor holds no descriptors or counters
+    std::vector<std::vector<int>> groups;
+    size_t sum = 0;
+    for (int i = 0; i < s->max_slots; i++) {
+        if (!s->slots[i]) continue;
+        const size_t w = step_bytes(*s->slots[i]);
+        if (groups.empty() || (s->group_bytes && sum + w > s->group_bytes)) {
+            groups.emplace_back();
+            sum = 0;
+        }
+        groups.back().push_back(i);
+        sum += w;
     }
-    if (s->ex.set_tiles(tiles, s->stream) || s->ex.set_inner(inner, s->stream)) return -1;
+    while (s->ex.size() > groups.size()) {
+        s->ex.back()->release();
+        s->ex.pop_back();
+    }
+    while (s->ex.size() < groups.size()) s->ex.emplace_back(new Exec());
+    for (size_t g = 0; g < groups.size(); g++) {
+        std::vector<int4> tiles, inner;
+        for (int i : groups[g]) append_work(tiles, inner, i, s->slots[i]->p, s->rz, s->interior, s->seg_rows);
+        balance_tiles_for_xcds(tiles, 4 * s->rz);
+        Exec &ex = *s->ex[g];
+        if (!ex.d_desc || (int)tiles.size() > ex.cap_tiles || (int)inner.size() > ex.cap_inner || ex.rz != s->rz ||
+            ex.interior != s->interior) {
+            ex.release();
+            const int ct = (int)(tiles.size() * 3 / 2) + 64, ci = (int)(inner.size() * 3 / 2) + 64;
+            if (ex.init(RTM_REAL_F32, s->max_slots, ct, s->rz, ci, s->interior)) return -1;
+            ex.q16 = s->snapshot_mode != RTM_SNAP_FP32;
+        }
+        if (ex.set_tiles(tiles, s->stream) || ex.set_inner(inner, s->stream)) return -1;
+    }
     s->lists_dirty = false;
+    s->descs_dirty = true;   // every executor gets the descriptors and counters of its new members
     return 0;
 }
 
@@ -298,11 +328,17 @@ extern "C" int rtm_server_tick(rtm_server *s)
     if (active) {
         if (s->lists_dirty && server_rebuild(s)) return -1;
         if (s->descs_dirty) {
-            if (s->ex.upload_descs(s->descs.data(), s->max_slots, s->stream)) return -1;
-            if (s->ex.set_n0(s->ex.h_n0, s->stream)) return -1;
+            // a request's step counter is the steps done in its current pass
+            std::vector<int> n0(s->max_slots, 0);
+            for (int i = 0; i < s->max_slots; i++)
+                if (s->slots[i]) n0[i] = s->slots[i]->n;
+            for (auto &e : s->ex)
+                if (e->upload_descs(s->descs.data(), s->max_slots, s->stream) || e->set_n0(n0, s->stream)) return -1;
             s->descs_dirty = false;
         }
-        if (s->ex.run(s->block, s->block, s->stream)) return -1;
+        // group after group, each a block of steps over its own members
+        for (auto &e : s->ex)
+            if (e->run(s->block, s->block, s->stream)) return -1;
         s->steps_run += s->block;
         // phase changes and completions at the block boundary
         for (int i = 0; i < s->max_slots; i++) {
@@ -316,7 +352,6 @@ extern "C" int rtm_server_tick(rtm_server *s)
                 r->phase = rtm::PH_BACKWARD;
                 r->n = 0;
                 s->descs[i] = backward_desc(*r);
-                s->ex.h_n0[i] = 0;
                 s->descs_dirty = true;
             } else if (server_finish(s, i)) {
                 return -1;
@@ -352,6 +387,22 @@ extern "C" int rtm_server_take_image(rtm_server *s, int id, float *image, size_t
     }
     rtm_set_error("rtm_server_take_image: request %d is not complete", id);
     return -1;
+}
+
+extern "C" int rtm_server_set_group_bytes(rtm_server *s, size_t bytes)
+{
+    if (!s) { rtm_set_error("rtm_server_set_group_bytes: null server"); return -1; }
+    s->group_bytes = bytes;
+    s->lists_dirty = true;
+    return 0;
+}
+
+extern "C" int rtm_server_get_group_bytes(rtm_server *s, size_t *bytes, int *groups)
+{
+    if (!s) { rtm_set_error("rtm_server_get_group_bytes: null server"); return -1; }
+    if (bytes) *bytes = s->group_bytes;
+    if (groups) *groups = (int)s->ex.size();
+    return 0;
 }
 
 extern "C" int rtm_server_stats(rtm_server *s, int *active, int *queued, size_t *bytes_in_use, long long *steps)

@@ -711,14 +711,16 @@ def _server_requests():
     return out
 
 
-@pytest.mark.parametrize("shape", [(4, 50, 1, 16), (4, 50, 2, 0), (2, 30, 0, 0)])
-def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape):
+@pytest.mark.parametrize("shape,group_bytes", [((4, 50, 1, 16), None), ((4, 50, 2, 0), None),
+                                               ((2, 30, 0, 0), None), ((4, 50, 0, 0), 1)])
+def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape, group_bytes):
     """Continuous batching: requests of three grid sizes arrive while others
     are in flight, are admitted by HBM footprint (the arena holds about two at
     a time), share every launch, and each image equals the oracle's for that
     request alone (FP16 store rule applied), bit for bit; so it is also
-    independent of the mix.  Column interior, mixed tiles and plain tiles."""
-    say(f"[gpu] request server: alone vs in the mix, shape {shape}")
+    independent of the mix.  Column interior, mixed tiles and plain tiles;
+    and a launch-group budget so small that each request is its own group."""
+    say(f"[gpu] request server: alone vs in the mix, shape {shape} group_bytes {group_bytes}")
     reqs = _server_requests()
     refs = []
     for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
@@ -729,6 +731,8 @@ def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape):
     got = {}
     with R.Server(budget, max_slots=4, snapshot_mode=1) as srv:
         srv.set_tiling(*shape)
+        if group_bytes is not None:
+            srv.set_group_bytes(group_bytes)
         ids = []
         ids.append(srv.submit(*_sub(reqs[0])))
         ids.append(srv.submit(*_sub(reqs[1])))
@@ -737,13 +741,16 @@ def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape):
         srv.tick()
         st = srv.stats()
         assert st["active"] <= 2 and st["bytes_in_use"] <= budget
+        most_groups = srv.group_bytes()[1]
         ids.append(srv.submit(*_sub(reqs[3])))
         ids.append(srv.submit(*_sub(reqs[4])))
         while srv.tick() > 0:
+            most_groups = max(most_groups, srv.group_bytes()[1])
             for rid, I in srv.done():
                 got[rid] = I
         for rid, I in srv.done():
             got[rid] = I
+    assert most_groups == (2 if group_bytes == 1 else 1)
     assert sorted(got) == sorted(ids)
     for k, rid in enumerate(ids):
         assert np.array_equal(got[rid], refs[k]), k
