@@ -219,7 +219,10 @@ int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double
  * of sims_per_group (0 or >= the batch size: one group), each group's
  * forward and backward passes before the next group's.  Results do not
  * depend on the grouping.  A group whose per-step working set fits the
- * device's last-level cache steps faster than one that does not. */
+ * device's last-level cache steps faster than one that does not.  A new
+ * batch starts with the group size its grids were last tuned to, else with
+ * as many realizations as fit 224 MiB at 16 B (FP32) or 32 B (FP64) per
+ * padded grid point. */
 int rtm_batch_set_group(rtm_batch *batch, int sims_per_group);
 int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
 /* Shape-tunes each group size (powers of two from 4 below the batch size,

@@ -139,8 +139,12 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
         if (q->sims[s].alloc(*p, real, s ? &q->sims[0] : nullptr)) return fail();
     const char *env = getenv("RTM_TILE_RZ");
     Shape sh{env && *env ? atoi(env) : 4, rtm::IN_TILES, 0};
+    // untuned: as many realizations per launch group as the cache budget
+    // holds (kStepBytesPerPoint is for 4-byte reals)
+    const size_t per = q->sims[0].nfield * (kStepBytesPerPoint / 4) * q->sims[0].esz;
+    sh.group = (int)std::max<size_t>(1, kGroupBytes / per);
     if (!(env && *env)) tuner_lookup(batch_key(q), &sh);
-    q->group = sh.group;
+    q->group = sh.group >= nsims ? 0 : sh.group;
     if (batch_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
     *out = q;
     return 0;

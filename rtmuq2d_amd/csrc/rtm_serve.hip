@@ -36,9 +36,6 @@ using namespace rtmd;
 
 namespace {
 
-// default launch-group budget: 16 grids of 1801 x 451 padded points
-constexpr size_t kGroupBytes = size_t(224) << 20;
-
 struct Request {
     int id = -1;
     rtm_params p{};
@@ -59,7 +56,7 @@ struct rtm_server {
     int max_slots = 0, block = 100;
     int rz = 4, interior = rtm::IN_TILES, seg_rows = 0;
     DeviceArena arena;
-    size_t group_bytes = kGroupBytes;
+    size_t group_bytes = kGroupBytes;   // launch-group budget (0: one group)
     std::vector<std::unique_ptr<Exec>> ex;      // one executor per launch group
     hipStream_t stream = nullptr;
     int next_id = 0;
@@ -256,8 +253,7 @@ static int server_admit(rtm_server *s)
     return admitted;
 }
 
-// bytes a request's step streams: u^n, u^{n-1}, C read and u^{n+1} written
-static size_t step_bytes(const Request &r) { return r.b.nfield * 16; }
+static size_t step_bytes(const Request &r) { return r.b.nfield * kStepBytesPerPoint; }
 
 static int server_rebuild(rtm_server *s)
 {

@@ -691,6 +691,14 @@ struct Exec {
 // autotuner: candidate kernel shapes, timing, and a per-grid-class cache
 // ---------------------------------------------------------------------------
 
+// Launch groups: the working set of one step of a grid is about 16 B per
+// padded point (u^n, u^{n-1}, C read, u^{n+1} written); a launch whose
+// grids' working sets sum to at most kGroupBytes stays within the device's
+// 256 MB last-level cache from one step to the next (16 grids of 1801 x 451
+// padded points).
+constexpr size_t kStepBytesPerPoint = 16;
+constexpr size_t kGroupBytes = size_t(224) << 20;
+
 struct Shape {
     int rz, interior, seg;
     int group = 0;      // batches: realizations per launch group (0: all)
