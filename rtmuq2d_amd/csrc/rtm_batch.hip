@@ -58,10 +58,11 @@ struct rtm_batch {
     int last_steps = 0;
 };
 
+static void batch_drop_execs(rtm_batch *q);
+
 static void batch_free(rtm_batch *q)
 {
-    for (auto &e : q->ex) e.release();
-    q->ex.clear();
+    batch_drop_execs(q);
     for (auto &s : q->sims) s.release();
     q->sims.clear();
     if (q->ev0) (void)hipEventDestroy(q->ev0);
@@ -75,12 +76,22 @@ static int group_size(const rtm_batch *q)
     return q->group > 0 && q->group < N ? q->group : N;
 }
 
-static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
+static void batch_drop_execs(rtm_batch *q)
 {
-    const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
     for (auto &e : q->ex) e.release();
     q->ex.clear();
+}
+
+// An unknown shape is refused with the current one kept; any later failure
+// leaves the batch with no executors, and every pass refuses to run until a
+// tiling call succeeds.
+static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
+{
+    if (check_shape(rz, interior)) return -1;   // the current shape stays
+    const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
+    batch_drop_execs(q);
     q->ex.resize(ng);       // never resized again before the next release
+    auto fail = [&]() { batch_drop_execs(q); return -1; };
     for (int k = 0; k < ng; k++) {
         const int s0 = k * g, n = std::min(g, N - s0);
         std::vector<int4> tiles, inner;
@@ -88,9 +99,9 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
             append_work(tiles, inner, s, q->sims[s0 + s].p, rz, interior, seg_rows);
         balance_tiles_for_xcds(tiles, 4 * rz);
         Exec &ex = q->ex[k];
-        if (ex.init(q->real, n, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
+        if (ex.init(q->real, n, (int)tiles.size(), rz, (int)inner.size(), interior)) return fail();
         ex.q16 = q->sims[0].qmode != rtm::Q_NONE;
-        if (ex.set_tiles(tiles, q->stream) || ex.set_inner(inner, q->stream)) return -1;
+        if (ex.set_tiles(tiles, q->stream) || ex.set_inner(inner, q->stream)) return fail();
     }
     q->rz = rz;
     q->graph_block = block;
@@ -269,6 +280,10 @@ extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
     if (xs < rtm::kHalf || xs >= b0.X - rtm::kHalf || zs < rtm::kHalf || zs >= b0.Z - rtm::kHalf || b0.n_out) {
         rtm_set_error("rtm_batch_migrate: source and receivers must lie in the stepped region "
                       "4 <= i < X-4, 4 <= k < Z-4 (use a plan otherwise)");
+        return -1;
+    }
+    if (q->ex.empty()) {
+        rtm_set_error("rtm_batch_migrate: the last tiling call failed; set a tiling first");
         return -1;
     }
     const int src_j = zs * b0.P + xs;

@@ -78,6 +78,7 @@ static void plan_free(rtm_plan *q)
 
 static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, int seg_rows)
 {
+    if (check_shape(rz, interior)) return -1;   // the current shape stays
     std::vector<int4> tiles, inner;
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
     balance_tiles_for_xcds(tiles, 4 * rz);

@@ -125,18 +125,7 @@ extern "C" int rtm_server_destroy(rtm_server *s)
 extern "C" int rtm_server_set_tiling(rtm_server *s, int rows_per_thread, int graph_block, int interior, int seg_rows)
 {
     if (!s || graph_block < 1) { rtm_set_error("rtm_server_set_tiling: bad argument"); return -1; }
-    if (rows_per_thread != 1 && rows_per_thread != 2 && rows_per_thread != 4 && rows_per_thread != 8) {
-        rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rows_per_thread);
-        return -1;
-    }
-    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED) {
-        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns) or 2 (mixed tiles) (got %d)", interior);
-        return -1;
-    }
-    if (interior == rtm::IN_MIXED && rows_per_thread != 2 && rows_per_thread != 4) {
-        rtm_set_error("mixed tiles take 2 or 4 rows per thread for the interior tiles (got %d)", rows_per_thread);
-        return -1;
-    }
+    if (check_shape(rows_per_thread, interior)) return -1;
     s->rz = rows_per_thread;
     s->block = graph_block;
     s->interior = interior;

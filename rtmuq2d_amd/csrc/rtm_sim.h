@@ -641,6 +641,10 @@ inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, i
 // executor: one launch per time step over a tile list, hipGraph blocks
 // ---------------------------------------------------------------------------
 
+// 0 if (rows per thread, interior kernel) is a shape the kernels exist for;
+// otherwise -1 with the reason in the error string
+int check_shape(int rz, int interior);
+
 struct Exec {
     int real = RTM_REAL_F32;
     int rz = 4;                         // rows per thread (tile height 4*rz)

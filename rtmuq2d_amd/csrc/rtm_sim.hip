@@ -481,11 +481,8 @@ void balance_tiles_for_xcds(std::vector<int4> &tiles, int tz)
 // executor
 // ---------------------------------------------------------------------------
 
-int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_inner, int interior_kernel)
+int check_shape(int rz, int interior)
 {
-    real = real_type;
-    rz = rows_per_thread;
-    interior = interior_kernel;
     if (rz != 1 && rz != 2 && rz != 4 && rz != 8) {
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
         return -1;
@@ -498,6 +495,15 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
         rtm_set_error("mixed tiles take 2 or 4 rows per thread for the interior tiles (got %d)", rz);
         return -1;
     }
+    return 0;
+}
+
+int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, int max_inner, int interior_kernel)
+{
+    real = real_type;
+    rz = rows_per_thread;
+    interior = interior_kernel;
+    if (check_shape(rz, interior)) return -1;
     cap_sims = max_sims > 0 ? max_sims : 1;
     cap_tiles = max_tiles > 0 ? max_tiles : 1;
     const size_t dsz = real == RTM_REAL_F64 ? sizeof(SimDesc<double>) : sizeof(SimDesc<float>);
@@ -673,6 +679,10 @@ int Exec::set_ring_phase(int phase, int blocks)
 int Exec::run(int steps, int block, hipStream_t s)
 {
     if (steps <= 0) return 0;
+    if (!d_desc || !d_tiles || !d_ntiles) {
+        rtm_set_error("the step executor holds no work lists (its last tiling call failed)");
+        return -1;
+    }
     int done = 0;
     if (block > 1 && steps >= block) {
         if (!gexec || gblock != block) {

@@ -235,6 +235,43 @@ def test_batch_group_autotune_keeps_results(gpu_available):
         assert b2.group() == (gsize, ngroups) and b2.tiling() == tiling
 
 
+def test_refused_tiling_keeps_the_current_shape(gpu_available):
+    """A shape no kernel exists for is refused before anything is released:
+    the plan and the batch keep their current shape and still migrate to the
+    oracle's image."""
+    say("[gpu] refused tiling")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    sx, sz = common.shot_positions(cfg)[0]
+    d = g["seismograms"][0]
+    vpad = O.extend(op, g["models"][0])
+    ref = O.rtm_shot(op, sx, sz, vpad, g["wavelet"], d)
+    with R.Plan(p) as plan, R.Batch(p, 2) as b:
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        plan.set_seismogram(d)
+        for k in range(2):
+            b.set_velocity(k, vpad)
+        b.set_wavelet(g["wavelet"])
+        b.set_seismogram(d)
+        for obj in (plan, b):
+            obj.set_tiling(2, 100, 0, 0)
+            with pytest.raises(R.RtmError, match="mixed tiles"):
+                obj.set_tiling(8, 100, 2, 0)
+            with pytest.raises(R.RtmError, match="rows per thread"):
+                obj.set_tiling(3, 100, 0, 0)
+            assert obj.tiling() == (2, 100, 0, 0)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        b.zero_images()
+        b.migrate(sx, sz)
+        images = [plan.image()] + [b.image(k) for k in range(2)]
+    for I, Is in images:
+        assert np.array_equal(I, ref[0]) and np.array_equal(Is, ref[1])
+
+
 _BS_CACHE = {}
 
 
