@@ -92,10 +92,11 @@ def c1_inputs(p, seed=2024):
     return v_true, v_mig, w
 
 
-def cpu_baseline(p, v_mig, w, seis):
+def cpu_baseline(p, v_mig, w, seis, runs=3):
     """The reference's own rtm_routine (oracle/_ref, -O3 -march=native
     OpenMP, its disk snapshots every 20 steps) on one full c1 image of the
-    same inputs, on this host's cores (OMP_NUM_THREADS)."""
+    same inputs, on this host's cores (OMP_NUM_THREADS); the median of
+    `runs` runs."""
     ref = os.path.join(REPO, "oracle", "_ref", "c1s_native", "ref_main")
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     nt_s = p.number_of_time_steps
@@ -113,12 +114,15 @@ def cpu_baseline(p, v_mig, w, seis):
             f.write(z)
         env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_SCHEDULE="static",
                    OMP_PROC_BIND="close")
-        r = subprocess.run([ref, "1", "1"], cwd=run, env=env, capture_output=True, text=True,
-                           timeout=900)
-        if r.returncode != 0:
-            log("cpu baseline failed:", r.stdout[-500:], r.stderr[-500:])
-            return None
-        secs = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("REF_SECONDS")][0]
+        times = []
+        for _ in range(runs):
+            r = subprocess.run([ref, "1", "1"], cwd=run, env=env, capture_output=True, text=True,
+                               timeout=900)
+            if r.returncode != 0:
+                log("cpu baseline failed:", r.stdout[-500:], r.stderr[-500:])
+                return None
+            times.append([float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("REF_SECONDS")][0])
+        secs = float(np.median(times))
     upd = 2.0 * nt_s * X * Z
     sec_per_image = secs * (p.number_of_time_steps / nt_s)
     return {
@@ -127,7 +131,8 @@ def cpu_baseline(p, v_mig, w, seis):
         "gupdates_per_s": round(upd / secs / 1e9, 4),
         "sample": (f"reference rtm_routine (gcc -O3 -march=native, OpenMP, disk snapshots every "
                    f"20 steps) on the {X}x{Z} padded c1 grid, {nt_s} of {p.number_of_time_steps} "
-                   f"steps ({secs:.2f} s wall)"),
+                   f"steps; median of {runs} runs ({secs:.2f} s wall)"),
+        "run_seconds": [round(t, 3) for t in times],
     }
 
 
