@@ -92,6 +92,40 @@ def c1_inputs(p, seed=2024):
     return v_true, v_mig, w
 
 
+def image_vs_reference(R, p, run, env):
+    """The c1 image of the reference's rtm_routine (the timed -march=native
+    build, and the build without contraction of a*b+c, oracle/_ref/c1s_strict)
+    against this library's rtm_routine on the same input files in `run`
+    (TS=20, the reference's compiled interval)."""
+    nx, nz = p.nx, p.nz
+    brute = os.path.join(run, "OUTPUTS", "stackedBrute_00001.bin")
+    native = np.fromfile(brute, np.float32).reshape(nx, nz)
+    strict = None
+    ref_strict = os.path.join(REPO, "oracle", "_ref", "c1s_strict", "ref_main")
+    if os.path.exists(ref_strict):
+        r = subprocess.run([ref_strict, "1", "1"], cwd=run, env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            strict = np.fromfile(brute, np.float32).reshape(nx, nz)
+    os.remove(brute)
+    cwd = os.getcwd()
+    os.chdir(run)
+    try:
+        R.set_params(p.copy().update(
+            time_slices=20, velocity_file="./INPUTS/vp_%05d.bin", wavelet_file="./INPUTS/wavelet.bin",
+            seismogram_file="./INPUTS/seismogram_%05d.zfp", stacked_file="./OUTPUTS/stackedBrute_%05d.bin",
+            stacked_laplacian_file="./OUTPUTS/stackedLaplacian_%05d.bin"))
+        R.lib().rtm_routine(1)
+    finally:
+        os.chdir(cwd)
+    gpu = np.fromfile(brute, np.float32).reshape(nx, nz)
+    out = {"time_slices": 20, "relL2_vs_native_build": rel_l2(gpu, native)}
+    if strict is not None:
+        out["relL2_vs_strict_build"] = rel_l2(gpu, strict)
+        out["bitexact_vs_strict_build"] = bool(np.array_equal(gpu, strict))
+    log(f"[bench] c1 image of this library vs the reference's: {out}")
+    return out
+
+
 def cpu_baseline(p, v_mig, w, seis, runs=3):
     """The reference's own rtm_routine (oracle/_ref, -O3 -march=native
     OpenMP, its disk snapshots every 20 steps) on one full c1 image of the
@@ -123,6 +157,7 @@ def cpu_baseline(p, v_mig, w, seis, runs=3):
                 return None
             times.append([float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("REF_SECONDS")][0])
         secs = float(np.median(times))
+        parity = image_vs_reference(R, p, run, env)
     upd = 2.0 * nt_s * X * Z
     sec_per_image = secs * (p.number_of_time_steps / nt_s)
     return {
@@ -133,6 +168,7 @@ def cpu_baseline(p, v_mig, w, seis, runs=3):
                    f"20 steps) on the {X}x{Z} padded c1 grid, {nt_s} of {p.number_of_time_steps} "
                    f"steps; median of {runs} runs ({secs:.2f} s wall)"),
         "run_seconds": [round(t, 3) for t in times],
+        "gpu_image_vs_reference": parity,
     }
 
 
