@@ -725,6 +725,35 @@ def test_c0_image_matches_reference(gpu_available):
     assert common.rel_l2(b, native) <= 1e-5
 
 
+def test_c1_full_size_image_matches_reference(gpu_available):
+    """One full c1-size image (1701x351, CPML 50, nt=4000, TS=20, 1701
+    receivers) through the drop-in rtm_routine: bit-identical to the
+    reference's strict build (checksums committed), hence within the
+    recorded relL2 of its -march=native build, which must be <= 1e-5."""
+    say("[gpu] c1-size end-to-end")
+    meta = json.load(open(os.path.join(common.GOLDEN, "c1s.json")))
+    cfg = common.REF_CONFIGS["c1s"]
+    v_true, models, wav, seis = common.make_inputs(cfg)
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    assert sha(models[0]) == meta["models"][0], "input generator drifted"
+    zs = R.zfp_compress(seis[0], cfg["zfp_tolerance"])
+    assert hashlib.sha256(zs).hexdigest() == meta["zfp"][0], "seismogram or ZFP stream drifted"
+    p = common.rtm_params(cfg)
+    with tempfile.TemporaryDirectory() as run:
+        common.write_inputs(run, cfg, models, wav, [zs])
+        cwd = os.getcwd()
+        os.chdir(run)
+        try:
+            R.set_params(p)
+            R.lib().rtm_routine(1)
+        finally:
+            os.chdir(cwd)
+        b, lap = common.read_outputs(run, cfg, 1)
+    assert sha(b) == meta["stackedBrute"][0], f"norm {np.linalg.norm(b)} vs {meta['brute_norm'][0]}"
+    assert sha(lap) == meta["stackedLaplacian"][0]
+    assert meta["strict_vs_native_relL2"] <= 1e-5
+
+
 def _server_requests():
     """Requests on three grids (tiny, asym, wide), each with its own velocity,
     shot position and seismogram, FP16 snapshot store every 4 steps."""
