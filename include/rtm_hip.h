@@ -193,6 +193,17 @@ int rtm_plan_get_image_f64(rtm_plan *plan, double *I, double *Is);
 int rtm_plan_get_fields(rtm_plan *plan, float *u_prev, float *u_last);  /* padded [X][Z] */
 int rtm_plan_get_fields_f64(rtm_plan *plan, double *u_prev, double *u_last);
 int rtm_plan_synchronize(rtm_plan *plan);
+/* Shot stacking on the device (replaces the per-shot temporary image files
+ * and the host stack of SRC/rtm_iso2d.c:115-146): rtm_plan_stack_image adds
+ * the plan's current images (I, I_s) into its stacks, in call order, which
+ * is the reference's shot order; the stacks start at zero.
+ * rtm_plan_get_stack returns the stack, divided by the self-image stack
+ * where that is non-zero and 0 elsewhere if normalize (SRC/rtm_iso2d.c:150),
+ * and its 8th-order Laplacian (laplacian_filter2D, SRC/math_functions.c:9-27),
+ * both [Nx][Nz]; either output may be NULL. */
+int rtm_plan_stack_image(rtm_plan *plan);
+int rtm_plan_zero_stack(rtm_plan *plan);
+int rtm_plan_get_stack(rtm_plan *plan, int normalize, float *stack, float *laplacian);
 /* Average time per time step (ms) of the last forward and the last backward
  * pass, from HIP events recorded on the plan's stream around the step
  * launches; steps = time steps per pass. */

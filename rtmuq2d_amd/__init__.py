@@ -135,6 +135,9 @@ def lib() -> C.CDLL:
         "rtm_plan_get_fields": (C.c_int, [plan_p, f32p, f32p]),
         "rtm_plan_get_fields_f64": (C.c_int, [plan_p, f64p, f64p]),
         "rtm_plan_synchronize": (C.c_int, [plan_p]),
+        "rtm_plan_stack_image": (C.c_int, [plan_p]),
+        "rtm_plan_zero_stack": (C.c_int, [plan_p]),
+        "rtm_plan_get_stack": (C.c_int, [plan_p, C.c_int, f32p, f32p]),
         "rtm_plan_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
         "rtm_batch_create": (C.c_int, [P(RtmParams), C.c_int, C.c_int, P(plan_p)]),
         "rtm_batch_destroy": (C.c_int, [plan_p]),
@@ -327,6 +330,20 @@ class Plan:
 
     def synchronize(self) -> None:
         _check(lib().rtm_plan_synchronize(self.h), "synchronize")
+
+    def stack_image(self) -> None:
+        """rtm_plan_stack_image: add the current images into the device stacks."""
+        _check(lib().rtm_plan_stack_image(self.h), "stack_image")
+
+    def zero_stack(self) -> None:
+        _check(lib().rtm_plan_zero_stack(self.h), "zero_stack")
+
+    def stack(self, normalize: bool = False):
+        """rtm_plan_get_stack: (stack, its Laplacian), [Nx][Nz] float32."""
+        st = np.empty((self.p.nx, self.p.nz), np.float32)
+        lp = np.empty((self.p.nx, self.p.nz), np.float32)
+        _check(lib().rtm_plan_get_stack(self.h, int(bool(normalize)), fptr(st), fptr(lp)), "get_stack")
+        return st, lp
 
     def last_timing(self):
         """(forward ms/step, backward ms/step, steps) of the last passes."""

@@ -59,6 +59,9 @@ struct rtm_plan {
     int fwd_src_j = -1;             // source index of the last forward pass (the reconstruction re-adds it)
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
+    // shot stacks (rtm_plan_stack_image), allocated at the first stacking:
+    // stack, self-image stack, and two work images for the output
+    void *stk[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 extern "C" size_t rtm_plan_footprint(const rtm_params *p, int real)
@@ -69,6 +72,8 @@ extern "C" size_t rtm_plan_footprint(const rtm_params *p, int real)
 
 static void plan_free(rtm_plan *q)
 {
+    for (void *&m : q->stk)
+        if (m) { (void)hipFree(m); m = nullptr; }
     q->ex.release();
     q->b.release();
     if (q->ev0) (void)hipEventDestroy(q->ev0);
@@ -434,6 +439,146 @@ static int get_image_t(rtm_plan *q, O *I, O *Is)
             for (int k = 0; k < nz; k++) dst[t][(size_t)i * nz + k] = O(h[(size_t)k * nx + i]);
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// shot stacking on the device (SRC/rtm_iso2d.c:115-152)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// stack += I and stack_self += Is, one shot's images, in FP of the plan
+template <typename R>
+__global__ void k_stack_add(R *__restrict__ st, R *__restrict__ ss, const R *__restrict__ I,
+                            const R *__restrict__ Is, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        st[q] = st[q] + I[q];
+        ss[q] = ss[q] + Is[q];
+    }
+}
+
+// the stack, or with normalize its illumination-normalized form stack /
+// stack_self (0 where stack_self is 0; the division the reference leaves
+// commented out, SRC/rtm_iso2d.c:150)
+template <typename R>
+__global__ void k_stack_out(R *__restrict__ out, const R *__restrict__ st, const R *__restrict__ ss, size_t n,
+                            int normalize)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        out[q] = normalize ? (ss[q] != R(0) ? st[q] / ss[q] : R(0)) : st[q];
+}
+
+// 8th-order Laplacian without coefficient or spacing over [4,Nx-4) x
+// [4,Nz-4), zero elsewhere (SRC/math_functions.c:15-24), on the device's
+// [Nz][Nx] image layout: the x neighbours are q -+ m, the z neighbours
+// q -+ m*Nx.  Each ring sums (x-, x+, z-, z+) in that order and the rings
+// are added from m = 0 up, as the reference writes it.
+template <typename R>
+__global__ void k_laplacian(R *__restrict__ lp, const R *__restrict__ c, int nx, int nz)
+{
+    const size_t n = (size_t)nx * nz, stride = (size_t)gridDim.x * blockDim.x;
+    const R w0 = R(2.0f * -2.97399944f), w1 = R(1.70507669f), w2 = R(-0.25861812f), w3 = R(0.04577745f),
+            w4 = R(-0.00523630f);
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % (size_t)nx), k = (int)(q / (size_t)nx);
+        if (i < 4 || i >= nx - 4 || k < 4 || k >= nz - 4) {
+            lp[q] = R(0);
+            continue;
+        }
+        const size_t X1 = 1, Z1 = (size_t)nx;
+        const R t1 = ((c[q - X1] + c[q + X1]) + c[q - Z1]) + c[q + Z1];
+        const R t2 = ((c[q - 2 * X1] + c[q + 2 * X1]) + c[q - 2 * Z1]) + c[q + 2 * Z1];
+        const R t3 = ((c[q - 3 * X1] + c[q + 3 * X1]) + c[q - 3 * Z1]) + c[q + 3 * Z1];
+        const R t4 = ((c[q - 4 * X1] + c[q + 4 * X1]) + c[q - 4 * Z1]) + c[q + 4 * Z1];
+        lp[q] = (((w0 * c[q] + w1 * t1) + w2 * t2) + w3 * t3) + w4 * t4;
+    }
+}
+
+int stack_alloc(rtm_plan *q)
+{
+    if (q->stk[0]) return 0;
+    const size_t bytes = q->b.nint * q->b.esz;
+    for (void *&m : q->stk) {
+        if (hipMalloc(&m, bytes) != hipSuccess) {
+            rtm_set_error("rtm_plan_stack_image: hipMalloc(%zu) failed", bytes);
+            return -1;
+        }
+    }
+    return 0;
+}
+
+template <typename R>
+int zero_stack_t(rtm_plan *q)
+{
+    for (int t = 0; t < 2; t++) {
+        hipLaunchKernelGGL(k_fill_zero<R>, dim3(1024), dim3(256), 0, q->stream, (R *)q->stk[t], q->b.nint);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+template <typename R>
+int stack_image_t(rtm_plan *q)
+{
+    const bool fresh = !q->stk[0];
+    if (stack_alloc(q)) return -1;
+    if (fresh && zero_stack_t<R>(q)) return -1;
+    hipLaunchKernelGGL(k_stack_add<R>, dim3(1024), dim3(256), 0, q->stream, (R *)q->stk[0], (R *)q->stk[1],
+                       (const R *)q->b.I, (const R *)q->b.Is, q->b.nint);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename R>
+int get_stack_t(rtm_plan *q, int normalize, float *stack, float *lap)
+{
+    const int nx = q->b.p.nx, nz = q->b.p.nz;
+    if (stack_alloc(q)) return -1;
+    R *out = (R *)q->stk[2], *lp = (R *)q->stk[3];
+    hipLaunchKernelGGL(k_stack_out<R>, dim3(1024), dim3(256), 0, q->stream, out, (const R *)q->stk[0],
+                       (const R *)q->stk[1], q->b.nint, normalize);
+    hipLaunchKernelGGL(k_laplacian<R>, dim3(1024), dim3(256), 0, q->stream, lp, (const R *)out, nx, nz);
+    HIP_TRY(hipGetLastError());
+    std::vector<R> h(q->b.nint);
+    void *src[2] = {out, lp};
+    float *dst[2] = {stack, lap};
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    for (int t = 0; t < 2; t++) {
+        if (!dst[t]) continue;
+        HIP_TRY(hipMemcpy(h.data(), src[t], sizeof(R) * q->b.nint, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nx; i++)
+            for (int k = 0; k < nz; k++) dst[t][(size_t)i * nz + k] = float(h[(size_t)k * nx + i]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int rtm_plan_stack_image(rtm_plan *q)
+{
+    if (!q) { rtm_set_error("rtm_plan_stack_image: null plan"); return -1; }
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    return q->b.real == RTM_REAL_F64 ? stack_image_t<double>(q) : stack_image_t<float>(q);
+}
+
+extern "C" int rtm_plan_zero_stack(rtm_plan *q)
+{
+    if (!q) { rtm_set_error("rtm_plan_zero_stack: null plan"); return -1; }
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    if (stack_alloc(q)) return -1;
+    return q->b.real == RTM_REAL_F64 ? zero_stack_t<double>(q) : zero_stack_t<float>(q);
+}
+
+extern "C" int rtm_plan_get_stack(rtm_plan *q, int normalize, float *stack, float *laplacian)
+{
+    if (!q) { rtm_set_error("rtm_plan_get_stack: null plan"); return -1; }
+    HIP_TRY(hipSetDevice(q->b.p.device));
+    if (!q->stk[0] && rtm_plan_zero_stack(q)) return -1;   // nothing stacked: zero images
+    return q->b.real == RTM_REAL_F64 ? get_stack_t<double>(q, normalize, stack, laplacian)
+                                     : get_stack_t<float>(q, normalize, stack, laplacian);
 }
 
 extern "C" int rtm_plan_get_image(rtm_plan *q, float *I, float *Is)

@@ -531,9 +531,10 @@ int zfp2d_lossyDecompression(char *zfp_data_02, float *array, int dimension1, in
 /* One simulation: read velocity and wavelet, pad, then for each shot run the
  * forward pass, decode the shot's seismogram, run the backward pass and stack
  * the shot image; write the raw stack and its Laplacian.  The device plan is
- * kept for all shots, and per-shot images stay in memory instead of the
- * reference's temporary files; the stack is summed in shot order as the
- * reference does (SRC/rtm_iso2d.c:115-129). */
+ * kept for all shots; the shot images are stacked on the device instead of
+ * going through the reference's temporary files, summed in shot order as the
+ * reference does (SRC/rtm_iso2d.c:115-129), and the illumination division
+ * and the Laplacian run there too. */
 void rtm_routine(int simulationId)
 {
     const rtm_params *p = rtm_global_params();
@@ -546,10 +547,8 @@ void rtm_routine(int simulationId)
     float *vpad = calloc((size_t)X * Z, sizeof(float));
     float *seis = calloc((size_t)p->number_of_receivers * p->number_of_time_steps, sizeof(float));
     float *wav = calloc((size_t)(p->sampling_wavelet > 0 ? p->sampling_wavelet : 1), sizeof(float));
-    float *stack = calloc(nimg, sizeof(float)), *stack_self = calloc(nimg, sizeof(float));
-    float *shot_img = calloc(nimg, sizeof(float)), *shot_self = calloc(nimg, sizeof(float));
-    float *lap = calloc(nimg, sizeof(float));
-    if (!model || !vpad || !seis || !wav || !stack || !stack_self || !shot_img || !shot_self || !lap) {
+    float *stack = calloc(nimg, sizeof(float)), *lap = calloc(nimg, sizeof(float));
+    if (!model || !vpad || !seis || !wav || !stack || !lap) {
         printf("\nError: out of host memory in rtm_routine\n");
         exit(1);
     }
@@ -574,23 +573,17 @@ void rtm_routine(int simulationId)
         die_on(rtm_plan_set_seismogram(plan, seis), "rtm_routine");
         die_on(rtm_plan_zero_image(plan), "rtm_routine");
         die_on(rtm_plan_backward(plan), "rtm_routine");
-        die_on(rtm_plan_get_image(plan, shot_img, shot_self), "rtm_routine");
-        for (size_t t = 0; t < nimg; t++) {
-            stack[t] = stack[t] + shot_img[t];
-            stack_self[t] = stack_self[t] + shot_self[t];
-        }
+        die_on(rtm_plan_stack_image(plan), "rtm_routine");
     }
     printf("\n\n");
+    /* SRC/rtm_iso2d.c:150 with unlit points left at 0, then SRC/rtm_iso2d.c:155-165 */
+    die_on(rtm_plan_get_stack(plan, p->normalize_illumination, stack, lap), "rtm_routine");
     rtm_plan_destroy(plan);
-    if (p->normalize_illumination)   /* SRC/rtm_iso2d.c:150, with unlit points left at 0 */
-        for (size_t t = 0; t < nimg; t++) stack[t] = stack_self[t] != 0.0f ? stack[t] / stack_self[t] : 0.0f;
 
     snprintf(path, sizeof path, p->stacked_file, simulationId);
     recording_file(path, stack, p->nx, p->nz);
-    laplacian_with(p, lap, stack);
     snprintf(path, sizeof path, p->stacked_laplacian_file, simulationId);
     recording_file(path, lap, p->nx, p->nz);
 
-    free(model); free(vpad); free(seis); free(wav); free(stack); free(stack_self);
-    free(shot_img); free(shot_self); free(lap);
+    free(model); free(vpad); free(seis); free(wav); free(stack); free(lap);
 }

@@ -537,6 +537,48 @@ def test_rtm_routine_illumination_normalization(gpu_available):
     assert np.array_equal(lap, O.laplacian(op, expect))
 
 
+def test_plan_stack_on_device(gpu_available):
+    """Two shots stacked on the device: the stack is the FP32 sum of the shot
+    images in shot order, the normalized stack divides by the self-image
+    stack (0 where that is 0), and the Laplacian equals the host
+    laplacian_filter2D of the same stack, bit for bit."""
+    say("[gpu] device stack")
+    cfg = common.REF_CONFIGS["tiny"]
+    g = golden("tiny")
+    p = common.rtm_params(cfg)
+    op = common.orc_params(cfg)
+    vpad = O.extend(op, g["models"][0])
+    shots = common.shot_positions(cfg)
+    R.set_params(p)
+    with R.Plan(p) as plan:
+        plan.set_velocity(vpad)
+        plan.set_wavelet(g["wavelet"])
+        zero, zero_lap = plan.stack()
+        assert not zero.any() and not zero_lap.any()
+        imgs = []
+        for s, (sx, sz) in enumerate(shots):
+            plan.set_seismogram(g["seismograms"][s])
+            plan.zero_image()
+            plan.migrate(sx, sz)
+            imgs.append(plan.image())
+            plan.stack_image()
+        st, lp = plan.stack()
+        stn, lpn = plan.stack(normalize=True)
+    ref = np.zeros_like(imgs[0][0])
+    ref_self = np.zeros_like(imgs[0][1])
+    for I, Is in imgs:
+        ref = (ref + I).astype(np.float32)
+        ref_self = (ref_self + Is).astype(np.float32)
+    assert np.array_equal(st, ref)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        refn = np.where(ref_self != 0, ref / ref_self, 0).astype(np.float32)
+    assert np.array_equal(stn, refn)
+    for a, b in ((st, lp), (stn, lpn)):
+        host = np.zeros_like(a)
+        R.lib().laplacian_filter2D(R.fptr(host), R.fptr(np.ascontiguousarray(a)))
+        assert np.array_equal(b, host)
+
+
 @pytest.mark.parametrize("mode", ["ranks", "devices"])
 def test_driver_runs_the_reference_simulations(gpu_available, mode):
     """rtmuq2d_hip, the MPI-free process driver, run on the reference's files
