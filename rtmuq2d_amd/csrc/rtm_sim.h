@@ -311,7 +311,9 @@ __device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it,
     X.hq = rtm::clampi(lane == 63 ? x0 + 128 : x0 - 4, 0, P - 4) >> 2;
     X.ok = lane < (rtm::col_half(it.x) ? 32 : 64);
     X.srow = rtm::col_src_row(S, V);
-    const bool img = Q16 ? (V.qsnap || V.smax) : V.snap != nullptr;
+    // rows of the image region carry the snapshot store (forward) or the
+    // imaging (backward, from the snapshot store or the rebuilt source field)
+    const bool img = Q16 ? (V.qsnap || V.smax) : (V.snap || V.sfield);
     const int ex_lo = img ? S.TU : 0, ex_hi = img ? S.TU + S.nz : 0;
     float umax = 0.0f;
     rtm::Pair<R> w[16];

@@ -361,6 +361,32 @@ def test_boundary_saving_records_traces(gpu_available, spacing):
             assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
+@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 16), (2, 1, 0), (4, 2, 0)])
+def test_boundary_saving_with_column_interior_bitexact(gpu_available, rz, interior, seg):
+    """Boundary saving on a grid wide enough for column items (a 128-column
+    and a 64-column item): the column kernel images against the rebuilt
+    source field as the tiles do, and the image equals the oracle's backward
+    pass over the reconstructed source field, bit for bit."""
+    say(f"[gpu] boundary saving, column interior rz={rz} interior={interior} seg={seg}")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=R.SNAP_BOUNDARY)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    S_bs = common.boundary_saving_snapshots(op, sx, sz, vpad, wav)
+    I, Is, _, _ = O.backward(op, vpad, seis, S_bs)
+    with R.Plan(p) as plan:
+        plan.set_tiling(rz, 100, interior, seg)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I2, Is2 = plan.image()
+    assert np.array_equal(I2, I), f"image relL2={common.rel_l2(I2, I)}"
+    assert np.array_equal(Is2, Is)
+
+
 @pytest.mark.parametrize("group", [0, 2])
 def test_batch_boundary_saving_bitexact(gpu_available, group):
     """A batch in boundary-saving mode: each realization's image equals the
