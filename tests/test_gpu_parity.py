@@ -895,6 +895,31 @@ def _sub(req):
     return p, vpad, wav, seis, sx, sz
 
 
+def test_server_fp32_store_matches_the_oracle(gpu_available):
+    """A server with the FP32 snapshot store (the reference's scheme): every
+    image equals the oracle's for that request alone, bit for bit, with all
+    requests admitted at once and each in a launch group of its own."""
+    say("[gpu] request server, FP32 store")
+    reqs = _server_requests()[:3]
+    refs = []
+    for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
+        S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+        refs.append(O.backward(op, vpad, seis, S)[0])
+    got = {}
+    with R.Server(sum(R.request_footprint(r[2].copy().update(snapshot_mode=0)) for r in reqs) + (8 << 20),
+                  max_slots=4, snapshot_mode=0) as srv:
+        srv.set_group_bytes(1)
+        ids = [srv.submit(*_sub(r)) for r in reqs]
+        while srv.tick() > 0:
+            for rid, I in srv.done():
+                got[rid] = I
+        for rid, I in srv.done():
+            got[rid] = I
+    assert sorted(got) == sorted(ids)
+    for k, rid in enumerate(ids):
+        assert np.array_equal(got[rid], refs[k]), k
+
+
 def test_server_rejects_requests_it_can_never_run(gpu_available):
     """Requests larger than the arena, with the source outside the stepped
     region, or with another snapshot store are refused at submission."""
