@@ -236,9 +236,11 @@ int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double
  * padded grid point. */
 int rtm_batch_set_group(rtm_batch *batch, int sims_per_group);
 int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
-/* Shape-tunes each group size (powers of two from 4 below the batch size,
- * and the whole batch) and keeps the one with the least time per
- * realization-step; row k = group_sizes[k], us_per_sim_step[k]. */
+/* Shape-tunes each candidate group size (1/2 to 2 times the size that fills
+ * the 224 MiB budget, and the whole batch), timing forward steps that write
+ * the snapshot stores (overwritten by the next migration), and keeps the one
+ * with the least time per realization-step; row k = group_sizes[k],
+ * us_per_sim_step[k]. */
 int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                              int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */

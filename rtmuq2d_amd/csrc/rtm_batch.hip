@@ -383,10 +383,12 @@ extern "C" int rtm_batch_welford(rtm_batch *q, double *mean, double *var)
 }
 
 // Times every candidate shape on launch group 0 (its realizations stepping
-// together, no snapshot or image work) at the current group size; keeps the
-// fastest.  *best_ms: its time per step of group 0.
+// together; with `snaps`, writing their snapshot stores as a forward pass
+// does) at the current group size; keeps the fastest.  *best_ms: its time
+// per step of group 0.
 template <typename R>
-static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, double *ms, int *n_out, double *best_ms)
+static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, double *ms, int *n_out, double *best_ms,
+                            bool snaps = false)
 {
     const auto &cand = tuner_candidates();
     double best_t = 1e30;
@@ -394,10 +396,13 @@ static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, doubl
     int k = 0;
     std::vector<SimDesc<R>> d;
     for (int s = 0; s < group_size(q); s++) d.push_back(q->sims[s].desc<R>());
+    // the snapshot stores are rewritten by the next forward pass; boundary
+    // saving keeps none
+    const int flags = snaps && !q->sims[0].ringmode ? rtm::F_SNAP : 0;
     for (const Shape &c : cand) {
         if (batch_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
         double t = 0.0;
-        if (time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, &t)) return -1;
+        if (time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, &t, flags)) return -1;
         if (k < cap) {
             if (shapes) {
                 shapes[4 * k] = c.rz; shapes[4 * k + 1] = q->graph_block;
@@ -442,8 +447,12 @@ extern "C" int rtm_batch_get_group(rtm_batch *q, int *sims_per_group, int *group
     return 0;
 }
 
-// Group sizes tried: every power of two from 4 below the batch size, and the
-// whole batch.  Each is shape-tuned; the group with the least time per
+// Group sizes tried: the size whose per-step working set fills the cache
+// budget (kGroupBytes at 16 B per padded point), h, scaled by 1/2, 5/8, 3/4,
+// 7/8, 1, 5/4, 3/2 and 2, and the whole batch.  Each is shape-tuned with its
+// snapshot stores written, as in a forward pass (a migration's stores and
+// image updates also take cache room, so the best group is smaller than one
+// timed on the fields alone); the group with the least time per
 // realization-step wins, with its best shape.
 extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                                         int *n_out)
@@ -452,8 +461,13 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     HIP_TRY(hipSetDevice(q->sims[0].p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
     const int N = (int)q->sims.size();
+    const size_t per = q->sims[0].nfield * (kStepBytesPerPoint / 4) * q->sims[0].esz;
+    const int h = (int)std::max<size_t>(1, kGroupBytes / per);
     std::vector<int> cand;
-    for (int g = 4; g < N; g *= 2) cand.push_back(g);
+    for (int f8 : {4, 5, 6, 7, 8, 10, 12, 16}) {
+        const int g = std::max(1, (h * f8 + 7) / 8);
+        if (g < N && (cand.empty() || cand.back() != g)) cand.push_back(g);
+    }
     cand.push_back(N);
     double best_t = 1e30;
     int best_g = N, k = 0;
@@ -461,8 +475,8 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     for (int g : cand) {
         q->group = g >= N ? 0 : g;
         double t = 0.0;
-        const int rc = q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, 0, nullptr, nullptr, nullptr, &t)
-                                               : batch_autotune_t<float>(q, steps, 0, nullptr, nullptr, nullptr, &t);
+        const int rc = q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, 0, nullptr, nullptr, nullptr, &t, true)
+                                               : batch_autotune_t<float>(q, steps, 0, nullptr, nullptr, nullptr, &t, true);
         if (rc) return -1;
         const double per = t * 1e3 / group_size(q);
         if (k < cap) {

@@ -719,11 +719,13 @@ std::string tuner_key(int real, bool q16, const std::vector<const rtm_params *> 
 bool tuner_lookup(const std::string &key, Shape *out);
 void tuner_store(const std::string &key, const Shape &s);
 
-// ms per step of `steps` forward steps of `descs` (from rest, no source, no
-// stores) with the executor's current work lists; descriptors and step
-// counters are left as uploaded here, so the caller re-uploads its own
+// ms per step of `steps` forward steps of `descs` (from rest, no source;
+// stores only if `flags` asks for them) with the executor's current work
+// lists; descriptors and step counters are left as uploaded here, so the
+// caller re-uploads its own
 template <typename R>
-int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms);
+int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms,
+               int flags = 0);
 
 }  // namespace rtmd
 

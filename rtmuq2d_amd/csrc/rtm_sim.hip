@@ -757,11 +757,11 @@ void tuner_store(const std::string &key, const Shape &s)
 }
 
 template <typename R>
-int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms)
+int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hipStream_t s, double *ms, int flags)
 {
     for (auto &d : descs) {
         d.phase = rtm::PH_FORWARD;
-        d.flags = 0;
+        d.flags = flags;
         d.src_j = -1;
     }
     if (ex.upload_descs(descs.data(), (int)descs.size(), s)) return -1;
@@ -785,7 +785,7 @@ int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hi
     return rc;
 }
 
-template int time_steps<float>(Exec &, std::vector<SimDesc<float>>, int, int, hipStream_t, double *);
-template int time_steps<double>(Exec &, std::vector<SimDesc<double>>, int, int, hipStream_t, double *);
+template int time_steps<float>(Exec &, std::vector<SimDesc<float>>, int, int, hipStream_t, double *, int);
+template int time_steps<double>(Exec &, std::vector<SimDesc<double>>, int, int, hipStream_t, double *, int);
 
 }  // namespace rtmd

@@ -220,7 +220,8 @@ def test_batch_group_autotune_keeps_results(gpu_available):
         b.set_wavelet(g["wavelet"])
         b.set_seismogram(d)
         rows = b.autotune_group(20)
-        assert [r["group"] for r in rows] == [4, 8, 9]
+        groups = [r["group"] for r in rows]
+        assert groups[-1] == 9 and groups == sorted(set(groups)) and groups[0] >= 1
         assert all(r["us_per_sim_step"] > 0 for r in rows)
         best = min(rows, key=lambda r: r["us_per_sim_step"])["group"]
         gsize, ngroups = b.group()
