@@ -262,7 +262,7 @@ def test_refused_tiling_keeps_the_current_shape(gpu_available):
             with pytest.raises(R.RtmError, match="mixed tiles"):
                 obj.set_tiling(8, 100, 2, 0)
             with pytest.raises(R.RtmError, match="rows per thread"):
-                obj.set_tiling(3, 100, 0, 0)
+                obj.set_tiling(5, 100, 0, 0)
             assert obj.tiling() == (2, 100, 0, 0)
         plan.zero_image()
         plan.migrate(sx, sz)
