@@ -190,7 +190,7 @@ def main() -> None:
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed workload for counter profiles (see run_profile)")
-    ap.add_argument("--profile-sims", type=int, default=16, help="realizations in the profile batch")
+    ap.add_argument("--profile-sims", type=int, default=13, help="realizations in the profile batch (a c2 launch group)")
     ap.add_argument("--profile-nt", type=int, default=400, help="time steps of the profile workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -314,14 +314,15 @@ def main() -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_profile(R, nt=400, nsims=16):
+def run_profile(R, nt=400, nsims=13):
     """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
     pass per counter group over the same dispatches):
       1. a 1 GiB device-to-device copy (torch), the calibration of the
          FETCH_SIZE / WRITE_SIZE counters (1 GiB read, 1 GiB written);
       2. the c1 plain workload for `nt` steps, tiles of 64x8, graph blocks;
-      3. 16 realizations of c1 for `nt` steps (TS=20), tiles of 64x16 only,
-         then strip tiles beside the column kernel."""
+      3. `nsims` realizations of c1 (a c2 launch group) for `nt` steps
+         (TS=20), tiles of 64x16 only, then strip tiles beside the column
+         kernel."""
     import torch
     x = torch.rand(1 << 28, device="cuda")
     y = torch.empty_like(x)
