@@ -238,9 +238,9 @@ int rtm_batch_set_group(rtm_batch *batch, int sims_per_group);
 int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
 /* Shape-tunes each candidate group size (1/2 to 2 times the size that fills
  * the 224 MiB budget, and the whole batch), timing forward steps that write
- * the snapshot stores (overwritten by the next migration), and keeps the one
- * with the least time per realization-step; row k = group_sizes[k],
- * us_per_sim_step[k]. */
+ * the snapshot stores (overwritten by the next migration), and keeps the
+ * smallest group within 2% of the least time per realization-step; row k =
+ * group_sizes[k], us_per_sim_step[k]. */
 int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                              int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */

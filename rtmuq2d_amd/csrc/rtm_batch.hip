@@ -452,8 +452,8 @@ extern "C" int rtm_batch_get_group(rtm_batch *q, int *sims_per_group, int *group
 // 7/8, 1, 5/4, 3/2 and 2, and the whole batch.  Each is shape-tuned with its
 // snapshot stores written, as in a forward pass (a migration's stores and
 // image updates also take cache room, so the best group is smaller than one
-// timed on the fields alone); the group with the least time per
-// realization-step wins, with its best shape.
+// timed on the fields alone); the smallest group within 2% of the least
+// time per realization-step wins, with its best shape.
 extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                                         int *n_out)
 {
@@ -472,6 +472,12 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     double best_t = 1e30;
     int best_g = N, k = 0;
     Shape best_shape{q->rz, q->interior, q->seg_rows};
+    struct Timed {
+        double per;
+        int g;
+        Shape shape;
+    };
+    std::vector<Timed> timed;   // in increasing group size
     for (int g : cand) {
         q->group = g >= N ? 0 : g;
         double t = 0.0;
@@ -484,13 +490,19 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
             if (us_per_sim_step) us_per_sim_step[k] = per;
         }
         k++;
-        if (per < best_t) {
-            best_t = per;
-            best_g = g;
-            best_shape = Shape{q->rz, q->interior, q->seg_rows};
-        }
+        timed.push_back({per, g, Shape{q->rz, q->interior, q->seg_rows}});
+        best_t = std::min(best_t, per);
     }
     if (n_out) *n_out = k < cap ? k : cap;
+    // the smallest group within 2% of the fastest: the timing leaves out the
+    // backward pass's image updates, which take cache room too, and a full
+    // migration of c2 measured 119K images/h at 13 against 110K at 16
+    for (const Timed &t : timed)
+        if (t.per <= best_t * 1.02) {
+            best_g = t.g;
+            best_shape = t.shape;
+            break;
+        }
     q->group = best_g >= N ? 0 : best_g;
     if (batch_set_tiling(q, best_shape.rz, q->graph_block, best_shape.interior, best_shape.seg)) return -1;
     best_shape.group = q->group;

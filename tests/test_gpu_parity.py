@@ -202,8 +202,8 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group):
 
 
 def test_batch_group_autotune_keeps_results(gpu_available):
-    """The group autotuner tries every candidate group size, keeps one of them
-    with its best shape, and the images stay bit-identical to the oracle's;
+    """The group autotuner tries every candidate group size, keeps the smallest
+    within 2% of the fastest with its best shape, and the images stay bit-identical to the oracle's;
     a new batch of the same grids starts from the tuned group."""
     say("[gpu] batch group autotune")
     cfg = common.REF_CONFIGS["asym"]
@@ -223,7 +223,8 @@ def test_batch_group_autotune_keeps_results(gpu_available):
         groups = [r["group"] for r in rows]
         assert groups[-1] == 9 and groups == sorted(set(groups)) and groups[0] >= 1
         assert all(r["us_per_sim_step"] > 0 for r in rows)
-        best = min(rows, key=lambda r: r["us_per_sim_step"])["group"]
+        fastest = min(r["us_per_sim_step"] for r in rows)
+        best = min(r["group"] for r in rows if r["us_per_sim_step"] <= fastest * 1.02)
         gsize, ngroups = b.group()
         assert gsize == best and ngroups == (len(vpads) + best - 1) // best
         tiling = b.tiling()
