@@ -126,6 +126,25 @@ def image_vs_reference(R, p, run, env):
     return out
 
 
+def host_cpu():
+    """Model, logical CPUs and sockets of this host (/proc/cpuinfo)."""
+    model, sockets, logical = "", set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and not model:
+                    model = v
+                elif k == "physical id":
+                    sockets.add(v)
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": logical or os.cpu_count(), "sockets": len(sockets) or None}
+
+
 def cpu_baseline(p, v_mig, w, seis, runs=3):
     """The reference's own rtm_routine (oracle/_ref, -O3 -march=native
     OpenMP, its disk snapshots every 20 steps) on one full c1 image of the
@@ -169,6 +188,7 @@ def cpu_baseline(p, v_mig, w, seis, runs=3):
                    f"steps; median of {runs} runs ({secs:.2f} s wall)"),
         "run_seconds": [round(t, 3) for t in times],
         "gpu_image_vs_reference": parity,
+        "host": host_cpu(),
     }
 
 
