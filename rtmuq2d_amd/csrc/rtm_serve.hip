@@ -246,6 +246,11 @@ static size_t step_bytes(const Request &r) { return r.b.nfield * kStepBytesPerPo
 
 static int server_rebuild(rtm_server *s)
 {
+    // the last tick's graph launches may still run and read the executors'
+    // lists and graphs: an executor is released or re-initialized below only
+    // after they end (a graph destroyed in flight, or lists freed under
+    // running kernels, is undefined)
+    HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<std::vector<int>> groups;
     size_t sum = 0;
     for (int i = 0; i < s->max_slots; i++) {
