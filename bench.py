@@ -515,7 +515,7 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
             for i in range(nreq)]
     with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
         srv.set_tiling(*shape)
-        res = serve.run(srv, reqs)
+        res = serve.run(srv, reqs, progress=lambda n, t: log(f"[bench] c4 {n} requests served at {t:.1f} s"))
         # capacity: the same requests all waiting at t = 0, in launch groups
         # (the default budget) and then with every active request in one group
         ncap = min(nreq, 100)

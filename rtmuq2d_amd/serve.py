@@ -38,10 +38,13 @@ def percentiles(lat) -> Dict[str, float]:
             "p99": float(np.percentile(a, 99)), "mean": float(a.mean()), "max": float(a.max())}
 
 
-def run(server, requests: List[Request], on_done: Optional[Callable] = None, log=None) -> Dict:
+def run(server, requests: List[Request], on_done: Optional[Callable] = None, log=None,
+        progress: Optional[Callable] = None, progress_every: int = 100) -> Dict:
     """Drive ``server`` through ``requests`` (sorted by arrival).  Returns
     per-request latency (completion - arrival, seconds), the wall time and
-    the image throughput.  ``on_done(index, image)`` sees every result."""
+    the image throughput.  ``on_done(index, image)`` sees every result;
+    ``progress(done, seconds)`` is called after every ``progress_every``
+    completions."""
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
@@ -66,6 +69,9 @@ def run(server, requests: List[Request], on_done: Optional[Callable] = None, log
                 i = id2idx.pop(rid)
                 done_at[i] = now()
                 pending -= 1
+                ndone = len(requests) - pending - (len(order) - nxt)
+                if progress and ndone % progress_every == 0:
+                    progress(ndone, now())
                 if on_done:
                     on_done(i, img)
                 if log:
