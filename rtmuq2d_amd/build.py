@@ -32,6 +32,9 @@ HIP_FLAGS = [
     # keeps the FP32 step bit-identical to the reference's arithmetic
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the kernel arguments arrive in SGPRs (gfx950 kernarg preload) instead
+    # of through the first dependent scalar load of every workgroup
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 C_FLAGS = ["-std=gnu99", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
