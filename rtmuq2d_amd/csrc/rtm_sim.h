@@ -111,7 +111,10 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red,
 // Q16: every sim of the launch keeps a 16-bit snapshot store (the store
 // kind is a property of the whole plan or batch, so the FP32-store kernels
 // carry none of that code).
-template <typename R, int RZ, bool Q16>
+// ONE: the launch steps a single simulation (a plan), so its descriptor's
+// address is known before the tile entry arrives and the descriptor loads
+// need not wait for it.
+template <typename R, int RZ, bool Q16, bool ONE = false>
 __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                const int *__restrict__ n0,
                                                const int4 *__restrict__ tiles,
@@ -128,8 +131,9 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     if (tile.x < 0) return;   // padding of an XCD's share (balance_tiles_for_xcds)
     // read through the restrict pointer: uniform fields become scalar loads
     // that no store of the step can invalidate
-    const SimDesc<R> &S = sims[tile.x];
-    const int n = n0[tile.x] + s;
+    const int sim = ONE ? 0 : tile.x;
+    const SimDesc<R> &S = sims[sim];
+    const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
     run_tile_cls<R, RZ, Q16>(L, red, S, n, tile, tx, ty);

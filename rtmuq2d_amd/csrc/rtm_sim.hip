@@ -627,6 +627,14 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
             hipLaunchKernelGGL((k_tiles_mix<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
         else
             hipLaunchKernelGGL((k_tiles_mix<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+    } else if (e.ntiles && e.cap_sims == 1) {
+        const int grid = 8 * ((e.cap_tiles + 7) / 8);
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
     } else if (e.ntiles) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
