@@ -205,6 +205,9 @@ def main() -> None:
     ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
     ap.add_argument("--serve-tiling", default="4,100,0,0",
                     help="request server kernel shape: rows_per_thread,graph_block,interior,seg_rows")
+    ap.add_argument("--tiling", default=None,
+                    help="fixed c1/c3 kernel shape rows_per_thread,graph_block,interior,seg_rows instead of "
+                         "the autotuner's pick (for traced runs, whose per-dispatch overhead changes the pick)")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--batch", type=int, default=64, help="realizations in the batched run (0: skip)")
     ap.add_argument("--batch-ts", type=int, default=20, help="snapshot interval of the batched run")
@@ -255,7 +258,11 @@ def main() -> None:
         plan.migrate(p.first_location_shot, p.depth_shot)
 
     # the library's autotuner picks the kernel shape (timed on this grid)
-    tuning = plan.autotune(400)
+    tuning = []
+    if args.tiling:
+        plan.set_tiling(*(int(v) for v in args.tiling.split(",")))
+    else:
+        tuning = plan.autotune(400)
     for t in tuning:
         log(f"[bench] autotune {tiling_dict((t['rows_per_thread'], t['graph_block'], t['interior'], t['seg_rows']))}: "
             f"{t['ms_per_step'] * 1e3:.2f} us/step")
