@@ -21,6 +21,8 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import subprocess
@@ -92,6 +94,23 @@ def c1_inputs(p, seed=2024):
     return v_true, v_mig, w
 
 
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send what native code prints to fd 1 (rtm_routine's progress banner,
+    as the reference prints it) to stderr, so that stdout carries only the
+    result line."""
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def image_vs_reference(R, p, run, env):
     """The c1 image of the reference's rtm_routine (the timed -march=native
     build, and the build without contraction of a*b+c, oracle/_ref/c1s_strict)
@@ -114,7 +133,8 @@ def image_vs_reference(R, p, run, env):
             time_slices=20, velocity_file="./INPUTS/vp_%05d.bin", wavelet_file="./INPUTS/wavelet.bin",
             seismogram_file="./INPUTS/seismogram_%05d.zfp", stacked_file="./OUTPUTS/stackedBrute_%05d.bin",
             stacked_laplacian_file="./OUTPUTS/stackedLaplacian_%05d.bin"))
-        R.lib().rtm_routine(1)
+        with stdout_to_stderr():
+            R.lib().rtm_routine(1)
     finally:
         os.chdir(cwd)
     gpu = np.fromfile(brute, np.float32).reshape(nx, nz)
