@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
         ("first_receiver", C.c_int), ("depth_receiver", C.c_int),
         ("spacing_receiver", C.c_int), ("n_receivers", C.c_int),
         ("time_slices", C.c_int),
+        ("image_kahan", C.c_int),      # 1: Kahan-compensated image sums (an option of this build)
     ]
 
     @property
@@ -43,7 +44,8 @@ def from_rtm(p) -> OrcParams:
     return OrcParams(p.nx, p.nz, p.thickness_left, p.thickness_right, p.thickness_up,
                      p.thickness_down, p.spacing_grid, p.sampling, p.number_of_time_steps,
                      p.sampling_wavelet, p.first_location_receiver, p.depth_receiver,
-                     p.spacing_receiver, p.number_of_receivers, p.time_slices)
+                     p.spacing_receiver, p.number_of_receivers, p.time_slices,
+                     int(getattr(p, "image_kahan", 0)))
 
 
 _lib = None

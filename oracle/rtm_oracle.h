@@ -33,6 +33,9 @@ typedef struct orc_params {
     int spacing_receiver;
     int n_receivers;
     int time_slices;            /* TIME_SLICES                    forward_modeling.h:4     */
+    int image_kahan;            /* 0: I += S*R as the reference (image_conditions.c:29-30);
+                                   1: the same sums Kahan-compensated (an option of this
+                                   build, not of the reference) */
 } orc_params;
 
 #endif

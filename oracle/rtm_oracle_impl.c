@@ -16,7 +16,9 @@
  *     function and end to end (rtm_routine);
  *   - closed forms: constant fields, the 1.99764 quadratic factor, CPML
  *     profile formulas evaluated independently in numpy, delta tests for
- *     source/receiver indexing, I = sum S*R for constant fields.
+ *     source/receiver indexing, I = sum S*R for constant fields; the Kahan
+ *     image option (no counterpart in the reference) against exactly
+ *     rounded sums of constant products, where plain FP32 summation drifts.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -257,17 +259,37 @@ void FN(orc_image)(const orc_params *p, REAL *I, REAL *Is, const REAL *S, const 
         }
 }
 
-/* Same, with the source wavefield given as an interior-only [Nx][Nz] array. */
-static void FN(image_interior)(const orc_params *p, REAL *I, REAL *Is, const REAL *Sint, const REAL *R)
+/* Kahan-compensated acc += v: the option p->image_kahan of this build (the
+ * reference sums plainly).  The compensation c carries the low-order part
+ * lost by the previous addition; every operation is one FP rounding
+ * (-ffp-contract=off), in this order. */
+static inline void FN(kahan_acc)(REAL *acc, REAL *c, REAL v)
+{
+    const REAL y = v - *c;
+    const REAL t = *acc + y;
+    *c = (t - *acc) - y;
+    *acc = t;
+}
+
+/* Same, with the source wavefield given as an interior-only [Nx][Nz] array;
+ * with compensations Ic, Isc (non-NULL) the sums are Kahan-compensated. */
+static void FN(image_interior)(const orc_params *p, REAL *I, REAL *Is, const REAL *Sint, const REAL *R,
+                               REAL *Ic, REAL *Isc)
 {
     const int Z = dimZ(p);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < p->nx; i++)
         for (int k = 0; k < p->nz; k++) {
             const int j = (i + p->t_left) * Z + (k + p->t_up);
-            const REAL s = Sint[i * p->nz + k];
-            I[i * p->nz + k] = I[i * p->nz + k] + s * R[j];
-            if (Is) Is[i * p->nz + k] = Is[i * p->nz + k] + s * s;
+            const int q = i * p->nz + k;
+            const REAL s = Sint[q];
+            if (Ic) {
+                FN(kahan_acc)(&I[q], &Ic[q], s * R[j]);
+                if (Is) FN(kahan_acc)(&Is[q], &Isc[q], s * s);
+            } else {
+                I[q] = I[q] + s * R[j];
+                if (Is) Is[q] = Is[q] + s * s;
+            }
         }
 }
 
@@ -359,6 +381,13 @@ int FN(orc_backward)(const orc_params *p, const float *vpad, const float *seis,
     FN(state_init)(&s, p, vpad);
     const int zr = p->t_up + p->depth_receiver;
     const size_t nint = (size_t)p->nx * p->nz;
+    /* Kahan option: compensations start at zero with each pass */
+    REAL *Ic = p->image_kahan ? calloc(nint, sizeof(REAL)) : NULL;
+    REAL *Isc = p->image_kahan ? calloc(nint, sizeof(REAL)) : NULL;
+    if (p->image_kahan && (!Ic || !Isc)) {
+        free(Ic); free(Isc); FN(state_free)(&s);
+        return -2;
+    }
     int c = p->nt / TS - 1;
     for (int n = 0; n < p->nt; n++) {
         const int tau = p->nt - n - 1;
@@ -368,13 +397,15 @@ int FN(orc_backward)(const orc_params *p, const float *vpad, const float *seis,
             s.U[(p->t_left + pos) * Z + zr] = (REAL)seis[(size_t)r * p->nt + tau];
         }
         if (tau % TS == 0) {
-            if (snaps) FN(image_interior)(p, I, Is, snaps + (size_t)c * nint, s.U);
+            if (snaps) FN(image_interior)(p, I, Is, snaps + (size_t)c * nint, s.U, Ic, Isc);
             c--;
         }
         REAL *t = s.U; s.U = s.u; s.u = t;
     }
     if (last_u) memcpy(last_u, s.U, sizeof(REAL) * (size_t)X * Z);
     if (last_U) memcpy(last_U, s.u, sizeof(REAL) * (size_t)X * Z);
+    free(Ic);
+    free(Isc);
     FN(state_free)(&s);
     return 0;
 }

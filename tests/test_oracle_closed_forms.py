@@ -244,3 +244,48 @@ def _brute_rtm(op, sx, sz, v, w, seis):
     snaps, _, _ = run("fwd")
     _, I, Is = run("bwd", snaps)
     return I, Is
+
+
+def test_kahan_image_option_closed_form():
+    """The oracle's Kahan image option (not in the reference; the device's
+    compensated image is checked against it): with the snapshot of the first
+    imaged step at 1 and every other snapshot at 2^-13, the exact self-image
+    is 1 + (n-1) 2^-26.  Each 2^-26 is below half an FP32 ulp of 1, so plain
+    summation stays at exactly 1, while compensated summation lands on the
+    FP32 value nearest the exact sum (1 + 125 * 2^-23 for n = 1000).  A
+    compensation dropped, applied with the wrong sign or updated in another
+    order gives 1 or a value several ulps off."""
+    n = 1000
+    op = _op(nt=n, ts=1)
+    rng = np.random.default_rng(9)
+    vpad = O.extend(op, rng.uniform(1500, 2500, (op.nx, op.nz)).astype(np.float32))
+    seis = np.zeros((op.n_receivers, op.nt), np.float32)
+    snaps = np.full((n, op.nx, op.nz), np.float32(2.0 ** -13), np.float32)
+    snaps[n - 1] = 1.0                       # imaged first: the backward pass starts at tau = nt-1
+    _, Is_plain, _, _ = O.backward(op, vpad, seis, snaps)
+    op.image_kahan = 1
+    I_k, Is_k, _, _ = O.backward(op, vpad, seis, snaps)
+    exact = 1.0 + (n - 1) * 2.0 ** -26
+    nearest = np.float32(1.0 + round((exact - 1.0) * 2 ** 23) * 2.0 ** -23)
+    assert np.all(Is_plain == np.float32(1.0))
+    assert np.all(Is_k == nearest), (float(Is_k.min()), float(Is_k.max()), float(nearest))
+    assert not I_k.any()                     # zero traces: the receiver field stays zero
+
+
+def test_kahan_image_option_tracks_fp64():
+    """On a modelled shot the compensated FP32 image is at least as close to
+    the FP64 oracle's as the plain one, and differs from it only by rounding
+    (the imaging products are the same)."""
+    op = _op(nt=400, ts=1, nrec=20, spacing=2)
+    rng = np.random.default_rng(10)
+    vpad = O.extend(op, rng.uniform(1500, 2500, (op.nx, op.nz)).astype(np.float32))
+    w = rng.standard_normal(op.n_wavelet).astype(np.float32) * 1e-2
+    S, d, _, _ = O.forward(op, 20, 6, vpad, w)
+    S64, _, _, _ = O.forward(op, 20, 6, vpad, w, dtype=np.float64)
+    I, Is, _, _ = O.backward(op, vpad, d, S)
+    I64, Is64, _, _ = O.backward(op, vpad, d, S64)
+    op.image_kahan = 1
+    Ik, Isk, _, _ = O.backward(op, vpad, d, S)
+    assert common.rel_l2(Ik, I64) <= common.rel_l2(I, I64) * 1.001
+    assert common.rel_l2(Isk, Is64) <= common.rel_l2(Is, Is64) * 1.001
+    assert common.rel_l2(Ik, I) < 1e-5 and not np.array_equal(Ik, I)
