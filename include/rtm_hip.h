@@ -119,11 +119,34 @@ void normalized_cross_correlation(float *imageCondition, float *selfImageConditi
                                   float *sourceWavefield, float *receiverWavefield);
 
 /* SRC/compression_tools.h:6,9 -- headerless ZFP 2-D fixed-accuracy streams
- * (host, ZFP 0.5.5).  Decompression returns 0 on success, 1 on failure. */
+ * (host; the ZFP 0.5.5 stream format, written and read by this library's own
+ * codec below, without the ZFP library).  Decompression returns 0 on
+ * success, 1 on failure (also when the file ends before the field does). */
 int zfp2d_lossyCompression(void **compression, float *array, int dimension1, int dimension2,
                            float tolerance);
 int zfp2d_lossyDecompression(char *zfp_data_02, float *array, int dimension1, int dimension2,
                              float tolerance);
+
+/* The codec behind them, in memory (no reference counterpart: the reference
+ * calls the ZFP library, ZFP_LIBRARY_0.5.5/include/zfp.h).  Streams of
+ * a[x + nx*y] as zfp_field_2d(a, zfp_type_float, nx, ny) with 64-bit stream
+ * words; the three lossy modes of zfp_stream_set_accuracy / _precision /
+ * _rate (rate without write-random-access alignment). */
+typedef struct rtm_zfp_mode {
+    unsigned minbits, maxbits, maxprec;
+    int minexp;
+} rtm_zfp_mode;
+double rtm_zfp_mode_accuracy(rtm_zfp_mode *m, double tol);       /* returns the tolerance used */
+unsigned rtm_zfp_mode_precision(rtm_zfp_mode *m, unsigned prec);  /* returns the precision used */
+double rtm_zfp_mode_rate(rtm_zfp_mode *m, double rate);           /* returns the rate used */
+/* zfp_stream_maximum_size for such a field (bytes) */
+size_t rtm_zfp2f_max_bytes(const rtm_zfp_mode *m, unsigned nx, unsigned ny);
+/* stream bytes (a multiple of 8), or 0 if cap is too small; any thread
+ * count gives the same bytes */
+size_t rtm_zfp2f_encode(const rtm_zfp_mode *m, const float *a, unsigned nx, unsigned ny, void *out, size_t cap,
+                        int threads);
+/* bytes consumed (whole words), or 0 if the stream ends before the field */
+size_t rtm_zfp2f_decode(const rtm_zfp_mode *m, const void *in, size_t bytes, unsigned nx, unsigned ny, float *a);
 
 /* SRC/pi_file.h:5,8 -- raw float32 files; exit(1) on any error, as the reference. */
 int reading_file(char *path, float *matrix, int dim1, int dim2);

@@ -24,9 +24,12 @@ Two things are built here:
        ``-O3 -fopenmp -xHost`` with gcc standing in for icc; used for the CPU
        baseline timing and the relL2 <= 1e-5 end-to-end check.
 
-   The ZFP 0.5.5 library the reference links is built by the package build
-   (``rtmuq2d_amd/build.py``) from the vendored source and linked from
-   ``rtmuq2d_amd/lib``.
+   The reference links ZFP 0.5.5; it is compiled here from the reference's
+   vendored source (``ZFP_LIBRARY_0.5.5/src``, read in place, with the
+   reference's settings: 64-bit stream words, OpenMP, ``-O3 -DNDEBUG``) into
+   ``oracle/_ref/zfp/libzfp.so``.  That library is a checker too: the tests
+   compare this build's own ZFP stream codec (``rtmuq2d_amd/csrc/rtm_zfp.c``)
+   with it byte for byte.  The product neither builds nor loads it.
 """
 from __future__ import annotations
 
@@ -39,9 +42,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 REF_ROOT = "/root/reference"
 REF_SRC = os.path.join(REF_ROOT, "SRC")
-REF_ZFP_INC = os.path.join(REF_ROOT, "ZFP_LIBRARY_0.5.5", "include")
+REF_ZFP = os.path.join(REF_ROOT, "ZFP_LIBRARY_0.5.5")
+REF_ZFP_INC = os.path.join(REF_ZFP, "include")
 REF_OUT = os.path.join(HERE, "_ref")
-ZFP_LIB_DIR = os.path.join(REPO, "rtmuq2d_amd", "lib")
+ZFP_LIB_DIR = os.path.join(REF_OUT, "zfp")
+ZFP_LIB = os.path.join(ZFP_LIB_DIR, "libzfp.so")
 
 # Reference translation units on the RTM path (SRC/, all but the MPI main).
 REF_UNITS = [
@@ -145,9 +150,8 @@ def build_ref(name: str, flavour: str = "strict") -> str:
     """Compile the reference for configuration `name`; returns its output dir."""
     if not ref_available():
         raise FileNotFoundError(REF_SRC)
-    zfp = os.path.join(ZFP_LIB_DIR, "libzfp.so")
-    if not os.path.exists(zfp):
-        raise FileNotFoundError(f"{zfp} (build the package first)")
+    if not os.path.exists(ZFP_LIB):
+        build_ref_zfp()
     cfg = REF_CONFIGS[name]
     odir = os.path.join(REF_OUT, f"{name}_{flavour}")
     os.makedirs(odir, exist_ok=True)
@@ -172,8 +176,27 @@ def build_ref(name: str, flavour: str = "strict") -> str:
     return odir
 
 
+def build_ref_zfp() -> str:
+    """ZFP 0.5.5 compiled from the reference's vendored source into
+    oracle/_ref/zfp/libzfp.so (the prebuilt binaries the reference ships
+    are never used)."""
+    if not ref_available():
+        raise FileNotFoundError(REF_SRC)
+    src = os.path.join(REF_ZFP, "src")
+    os.makedirs(ZFP_LIB_DIR, exist_ok=True)
+    units = ["bitstream.c", "zfp.c"] + [f"{k}{d}{t}.c" for k in ("decode", "encode") for d in "1234" for t in "ilfd"]
+    objs = []
+    for u in units:
+        o = os.path.join(ZFP_LIB_DIR, u.replace(".c", ".o"))
+        _run(["gcc", "-std=gnu99", "-O3", "-DNDEBUG", "-fPIC", "-fopenmp", "-w", "-I", REF_ZFP_INC,
+              "-c", os.path.join(src, u), "-o", o])
+        objs.append(o)
+    _run(["gcc", "-shared", "-fopenmp", *objs, "-o", ZFP_LIB, "-lm"])
+    return ZFP_LIB
+
+
 def build_all_refs() -> list:
-    out = []
+    out = [build_ref_zfp()]
     for name in REF_CONFIGS:
         out.append(build_ref(name, "strict"))
     out.append(build_ref("c0", "native"))

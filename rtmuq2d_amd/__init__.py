@@ -17,12 +17,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "librtm_hip.so")
-ZFP_PATH = os.path.join(LIB_DIR, "libzfp.so")
 PATH_LEN = 512
 
 SNAP_FP32, SNAP_FP16, SNAP_BF16, SNAP_BOUNDARY = 0, 1, 2, 3
 REAL_F32, REAL_F64 = 0, 1
 INTERIOR_TILES, INTERIOR_COLUMNS, INTERIOR_MIXED = 0, 1, 2
+
+
+class ZfpMode(C.Structure):
+    """Mirror of ``rtm_zfp_mode`` (include/rtm_hip.h)."""
+    _fields_ = [("minbits", C.c_uint), ("maxbits", C.c_uint), ("maxprec", C.c_uint), ("minexp", C.c_int)]
 
 
 class RtmParams(C.Structure):
@@ -107,6 +111,13 @@ def lib() -> C.CDLL:
         "normalized_cross_correlation": (None, [f32p] * 4),
         "zfp2d_lossyCompression": (C.c_int, [P(C.c_void_p), f32p, C.c_int, C.c_int, C.c_float]),
         "zfp2d_lossyDecompression": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int, C.c_float]),
+        "rtm_zfp_mode_accuracy": (C.c_double, [P(ZfpMode), C.c_double]),
+        "rtm_zfp_mode_precision": (C.c_uint, [P(ZfpMode), C.c_uint]),
+        "rtm_zfp_mode_rate": (C.c_double, [P(ZfpMode), C.c_double]),
+        "rtm_zfp2f_max_bytes": (C.c_size_t, [P(ZfpMode), C.c_uint, C.c_uint]),
+        "rtm_zfp2f_encode": (C.c_size_t, [P(ZfpMode), f32p, C.c_uint, C.c_uint, C.c_void_p, C.c_size_t,
+                                          C.c_int]),
+        "rtm_zfp2f_decode": (C.c_size_t, [P(ZfpMode), C.c_void_p, C.c_size_t, C.c_uint, C.c_uint, f32p]),
         "reading_file": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
         "recording_file": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
         "extend_matrix": (None, [f32p, f32p]),
@@ -382,6 +393,50 @@ def zfp_decompress_file(path: str, d1: int, d2: int, tol: float) -> np.ndarray:
     if rc != 0:
         raise RtmError(f"zfp2d_lossyDecompression({path}) failed")
     return a
+
+
+def zfp_mode(mode: str, value) -> ZfpMode:
+    """A ZFP lossy mode: ("accuracy", tol), ("precision", bit planes) or
+    ("rate", bits per value)."""
+    m = ZfpMode()
+    if mode == "accuracy":
+        lib().rtm_zfp_mode_accuracy(C.byref(m), float(value))
+    elif mode == "precision":
+        lib().rtm_zfp_mode_precision(C.byref(m), int(value))
+    elif mode == "rate":
+        lib().rtm_zfp_mode_rate(C.byref(m), float(value))
+    else:
+        raise ValueError(mode)
+    return m
+
+
+def zfp_encode(a: np.ndarray, mode: str, value, threads: int = 1) -> bytes:
+    """ZFP 0.5.5 stream of a[ny][nx] float32 (x fastest) in the given mode,
+    by this library's codec (rtm_zfp2f_encode)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    ny, nx = a.shape
+    m = zfp_mode(mode, value)
+    cap = lib().rtm_zfp2f_max_bytes(C.byref(m), nx, ny)
+    buf = C.create_string_buffer(cap)
+    n = lib().rtm_zfp2f_encode(C.byref(m), fptr(a), nx, ny, C.cast(buf, C.c_void_p), cap, int(threads))
+    if n == 0:
+        raise RtmError("rtm_zfp2f_encode: buffer too small")
+    return buf.raw[:n]
+
+
+def zfp_decode(stream: bytes, nx: int, ny: int, mode: str, value) -> np.ndarray:
+    """The inverse of zfp_encode; raises if the stream ends before the field."""
+    m = zfp_mode(mode, value)
+    out = np.zeros((ny, nx), np.float32)
+    buf = C.create_string_buffer(bytes(stream), len(stream))
+    n = lib().rtm_zfp2f_decode(C.byref(m), C.cast(buf, C.c_void_p), len(stream), nx, ny, fptr(out))
+    if n == 0:
+        raise RtmError("rtm_zfp2f_decode: the stream ends before the field")
+    return out
+
+
+def zfp_max_bytes(nx: int, ny: int, mode: str, value) -> int:
+    return int(lib().rtm_zfp2f_max_bytes(C.byref(zfp_mode(mode, value)), nx, ny))
 
 
 class Batch:

@@ -1,13 +1,10 @@
 """Build recipe for the product libraries (run by __graft_entry__.build()).
 
-* ``rtmuq2d_amd/lib/libzfp.so`` -- ZFP 0.5.5, the codec rtmUq2D uses for its
-  seismograms (SRC/compression_tools.c:78-135).  It is compiled from the
-  vendored source in the reference tree (``ZFP_LIBRARY_0.5.5/src``) with the
-  reference's own build settings (64-bit stream words, OpenMP on, ``-O3
-  -DNDEBUG -fPIC -fopenmp -std=gnu99``).  No prebuilt binary is used.  When
-  the reference tree is absent the previously built library is kept.
 * ``rtmuq2d_amd/lib/librtm_hip.so`` -- the C-ABI library: HIP kernels for
-  gfx950 plus the host C layer that mirrors rtm_iso2d's entry points.
+  gfx950 plus the host C layer that mirrors rtm_iso2d's entry points,
+  including this library's own codec for the ZFP 0.5.5 seismogram streams
+  (``csrc/rtm_zfp.c``; no ZFP library is built or loaded).
+* ``rtmuq2d_amd/lib/rtmuq2d_hip`` -- the process driver.
 """
 from __future__ import annotations
 
@@ -22,7 +19,6 @@ REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib")
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
-ZFP_ROOT = "/root/reference/ZFP_LIBRARY_0.5.5"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -47,28 +43,6 @@ def _run(cmd):
         sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
         raise RuntimeError(f"build step failed: {os.path.basename(cmd[0])}")
     return r
-
-
-def build_zfp() -> str | None:
-    out = os.path.join(LIB, "libzfp.so")
-    src = os.path.join(ZFP_ROOT, "src")
-    if not os.path.isdir(src):
-        return out if os.path.exists(out) else None
-    os.makedirs(LIB, exist_ok=True)
-    bdir = os.path.join(BUILD, "zfp")
-    os.makedirs(bdir, exist_ok=True)
-    units = ["bitstream.c", "zfp.c"]
-    for d in "1234":
-        for t in "ilfd":
-            units += [f"decode{d}{t}.c", f"encode{d}{t}.c"]
-    objs = []
-    for u in units:
-        o = os.path.join(bdir, u.replace(".c", ".o"))
-        _run(["gcc", "-std=gnu99", "-O3", "-DNDEBUG", "-fPIC", "-fopenmp", "-w",
-              "-I", os.path.join(ZFP_ROOT, "include"), "-c", os.path.join(src, u), "-o", o])
-        objs.append(o)
-    _run(["gcc", "-shared", "-fopenmp", *objs, "-o", out, "-lm"])
-    return out
 
 
 RESOURCES = os.path.join(LIB, "kernel_resources.json")
@@ -114,8 +88,7 @@ def build_hip() -> str:
         objs.append(o)
     out = os.path.join(LIB, "librtm_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", *objs, "-o", tmp,
-          "-ldl", "-lm"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", *objs, "-o", tmp, "-lm"])
     os.replace(tmp, out)
     return out
 
@@ -135,7 +108,7 @@ def build_driver() -> str:
 
 
 def build() -> dict:
-    return {"zfp": build_zfp(), "rtm": build_hip(), "driver": build_driver()}
+    return {"rtm": build_hip(), "driver": build_driver()}
 
 
 if __name__ == "__main__":

@@ -9,13 +9,16 @@
  * codec and the per-simulation driver rtm_routine.
  */
 #define _GNU_SOURCE
-#include <dlfcn.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "rtm_internal.h"
+#include "rtm_zfp.h"
 
 /* ------------------------------------------------------------------------ */
 /* parameters                                                                */
@@ -386,141 +389,61 @@ void laplacian_filter2D(float *lp, float *inputMatrix)
 }
 
 /* ------------------------------------------------------------------------ */
-/* ZFP 0.5.5 (host), resolved at first use from libzfp.so next to this      */
-/* library (or $RTM_ZFP_LIBRARY)                                             */
+/* ZFP 0.5.5 streams (host): this library's codec, rtm_zfp.c                 */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-    void *h;
-    void *(*field_2d)(void *, int, unsigned, unsigned);
-    void *(*stream_open_z)(void *);
-    double (*set_accuracy)(void *, double);
-    size_t (*max_size)(const void *, const void *);
-    void *(*bs_open)(void *, size_t);
-    void (*set_bit_stream)(void *, void *);
-    void (*rewind)(void *);
-    size_t (*compress)(void *, const void *);
-    size_t (*decompress)(void *, void *);
-    void (*field_free)(void *);
-    void (*stream_close_z)(void *);
-    void (*bs_close)(void *);
-    int (*set_execution)(void *, int);
-    int (*set_omp_threads)(void *, unsigned);
-} zfp_api;
-
-static zfp_api g_zfp;
-enum { ZFP_TYPE_FLOAT = 3, ZFP_EXEC_OMP = 1 };   /* zfp.h:123,168 */
-
-static int zfp_load(void)
+static int zfp_threads(void)
 {
-    if (g_zfp.h) return 0;
-    char path[4096] = {0};
-    const char *env = getenv("RTM_ZFP_LIBRARY");
-    if (env && *env) {
-        snprintf(path, sizeof path, "%s", env);
-    } else {
-        Dl_info info;
-        if (dladdr((void *)&zfp_load, &info) && info.dli_fname) {
-            snprintf(path, sizeof path, "%s", info.dli_fname);
-            char *slash = strrchr(path, '/');
-            if (slash) snprintf(slash + 1, sizeof path - (size_t)(slash + 1 - path), "libzfp.so");
-            else snprintf(path, sizeof path, "libzfp.so");
-        } else {
-            snprintf(path, sizeof path, "libzfp.so");
-        }
-    }
-    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
-    if (!h) { rtm_set_error("cannot load ZFP library %s: %s", path, dlerror()); return -1; }
-    zfp_api z;
-    memset(&z, 0, sizeof z);
-    z.h = h;
-#define SYM(field, name) *(void **)(&z.field) = dlsym(h, name)
-    SYM(field_2d, "zfp_field_2d");
-    SYM(stream_open_z, "zfp_stream_open");
-    SYM(set_accuracy, "zfp_stream_set_accuracy");
-    SYM(max_size, "zfp_stream_maximum_size");
-    SYM(bs_open, "stream_open");
-    SYM(set_bit_stream, "zfp_stream_set_bit_stream");
-    SYM(rewind, "zfp_stream_rewind");
-    SYM(compress, "zfp_compress");
-    SYM(decompress, "zfp_decompress");
-    SYM(field_free, "zfp_field_free");
-    SYM(stream_close_z, "zfp_stream_close");
-    SYM(bs_close, "stream_close");
-    SYM(set_execution, "zfp_stream_set_execution");
-    SYM(set_omp_threads, "zfp_stream_set_omp_threads");
-#undef SYM
-    if (!z.field_2d || !z.stream_open_z || !z.set_accuracy || !z.max_size || !z.bs_open ||
-        !z.set_bit_stream || !z.rewind || !z.compress || !z.decompress || !z.field_free ||
-        !z.stream_close_z || !z.bs_close) {
-        rtm_set_error("ZFP library %s lacks required symbols", path);
-        dlclose(h);
-        return -1;
-    }
-    g_zfp = z;
-    return 0;
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* Headerless fixed-accuracy stream of the 2-D field (nx = dimension1 fastest,
- * ny = dimension2) over the flat array, as SRC/compression_tools.c:10-74
- * (OpenMP execution; chunk streams are concatenated, so the bytes equal the
- * serial encoder's).  Returns the stream size; *compression receives a
- * malloc'ed buffer. */
+ * ny = dimension2) over the flat array, as SRC/compression_tools.c:10-74 asks
+ * ZFP for (OpenMP execution: rows of blocks are encoded in parallel and their
+ * bits concatenated, the serial encoder's bytes).  Returns the stream size;
+ * *compression receives a malloc'ed buffer of the maximum stream size. */
 int zfp2d_lossyCompression(void **compression, float *array, int dimension1, int dimension2,
                            float tolerance)
 {
     *compression = NULL;
-    if (zfp_load()) { fprintf(stderr, "%s\n", rtm_last_error()); return 0; }
-    void *field = g_zfp.field_2d(array, ZFP_TYPE_FLOAT, (unsigned)dimension1, (unsigned)dimension2);
-    void *zs = g_zfp.stream_open_z(NULL);
-    g_zfp.set_accuracy(zs, tolerance);
-    const size_t cap = g_zfp.max_size(zs, field);
+    if (dimension1 <= 0 || dimension2 <= 0) return 0;
+    rtm_zfp_mode m;
+    rtm_zfp_mode_accuracy(&m, tolerance);
+    const size_t cap = rtm_zfp2f_max_bytes(&m, (unsigned)dimension1, (unsigned)dimension2);
     void *buf = malloc(cap);
     size_t size = 0;
-    if (buf) {
-        void *bs = g_zfp.bs_open(buf, cap);
-        g_zfp.set_bit_stream(zs, bs);
-        g_zfp.rewind(zs);
-        if (g_zfp.set_execution && g_zfp.set_execution(zs, ZFP_EXEC_OMP) && g_zfp.set_omp_threads)
-            g_zfp.set_omp_threads(zs, 0);
-        size = g_zfp.compress(zs, field);
-        g_zfp.bs_close(bs);
-    }
-    g_zfp.field_free(field);
-    g_zfp.stream_close_z(zs);
+    if (buf) size = rtm_zfp2f_encode(&m, array, (unsigned)dimension1, (unsigned)dimension2, buf, cap, zfp_threads());
     *compression = buf;
     return (int)size;
 }
 
 /* Decode the headerless stream in `zfp_data_02` into array (same field
  * convention and tolerance as the encoder; the mode is not stored in the
- * stream), SRC/compression_tools.c:78-135.  0 = success, 1 = failure. */
+ * stream), SRC/compression_tools.c:78-135: at most the maximum stream size
+ * is read.  0 = success, 1 = failure ("Decompression Failed" on stderr, as
+ * the reference). */
 int zfp2d_lossyDecompression(char *zfp_data_02, float *array, int dimension1, int dimension2,
                              float tolerance)
 {
-    if (zfp_load()) { fprintf(stderr, "%s\n", rtm_last_error()); return 1; }
-    void *field = g_zfp.field_2d(array, ZFP_TYPE_FLOAT, (unsigned)dimension1, (unsigned)dimension2);
-    void *zs = g_zfp.stream_open_z(NULL);
-    g_zfp.set_accuracy(zs, tolerance);
-    const size_t cap = g_zfp.max_size(zs, field);
-    void *buf = calloc(cap ? cap : 1, 1);
     int status = 1;
-    FILE *f = buf ? fopen(zfp_data_02, "rb") : NULL;
-    if (f) {
-        const size_t got = fread(buf, 1, cap, f);
-        fclose(f);
-        (void)got;
-        void *bs = g_zfp.bs_open(buf, cap);
-        g_zfp.set_bit_stream(zs, bs);
-        g_zfp.rewind(zs);
-        const size_t used = g_zfp.decompress(zs, field);
-        g_zfp.bs_close(bs);
-        status = used ? 0 : 1;
+    if (dimension1 > 0 && dimension2 > 0) {
+        rtm_zfp_mode m;
+        rtm_zfp_mode_accuracy(&m, tolerance);
+        const size_t cap = rtm_zfp2f_max_bytes(&m, (unsigned)dimension1, (unsigned)dimension2);
+        void *buf = malloc(cap ? cap : 1);
+        FILE *f = buf ? fopen(zfp_data_02, "rb") : NULL;
+        if (f) {
+            const size_t got = fread(buf, 1, cap, f);
+            fclose(f);
+            status = rtm_zfp2f_decode(&m, buf, got, (unsigned)dimension1, (unsigned)dimension2, array) ? 0 : 1;
+        }
+        free(buf);
     }
     if (status) fprintf(stderr, "Decompression Failed\n");
-    free(buf);
-    g_zfp.field_free(field);
-    g_zfp.stream_close_z(zs);
     return status;
 }
 
