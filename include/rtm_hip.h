@@ -304,15 +304,28 @@ int rtm_server_submit(rtm_server *server, const rtm_params *p, const float *vpad
 /* Admit queued requests that fit (arrival order), advance every active
  * request by one block of time steps in shared launches, switch finished
  * forward passes to their backward pass and retire finished requests.
- * Returns the number of requests still queued or active (< 0 on error; a
- * queued request that cannot be placed even in the empty arena is dropped
- * with that error, and the next tick carries on with the rest). */
+ * Returns the number of requests still queued or active (< 0 only for a
+ * null server).  Failures retire the requests they touch, not the server:
+ * a device error in one launch group's step fails that group's requests; a
+ * failed upload, or a request that does not fit even the empty arena, fails
+ * that request; each is reported by rtm_server_take_image, and the other
+ * requests and later ticks carry on. */
 int rtm_server_tick(rtm_server *server);
-/* 1 and *id = the oldest completed request not yet taken, or 0 if none. */
+/* 1 and *id = the oldest retired (completed or failed) request not yet
+ * taken, or 0 if none. */
 int rtm_server_next_done(rtm_server *server, int *id);
-/* Copy a completed request's image ([Nx][Nz], n = Nx*Nz floats) and forget it. */
+/* Copy a completed request's image ([Nx][Nz], n = Nx*Nz floats) and forget
+ * it: 0; -2 (and forgotten) if the request failed, the reason in
+ * rtm_last_error(); -1 if it is not retired or n differs. */
 int rtm_server_take_image(rtm_server *server, int id, float *image, size_t n);
 int rtm_server_stats(rtm_server *server, int *active, int *queued, size_t *bytes_in_use, long long *steps);
+/* Requests failed since the server was created. */
+int rtm_server_failures(rtm_server *server);
+/* Threads: rtm_server_submit, _next_done and _take_image may be called from
+ * any thread at any time, concurrently with a tick (a submission copies its
+ * inputs and queues them; it never waits for a tick's launches).  Ticks,
+ * stats and the configuration calls serialise on one lock.  Destroy only
+ * once no other thread calls into the server. */
 /* Launch groups: the active requests advance in groups whose per-step
  * working sets (16 B per padded grid point) sum to at most `bytes`, group
  * after group, each a block of steps; 0 puts every active request in one

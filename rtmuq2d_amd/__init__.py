@@ -175,6 +175,7 @@ def lib() -> C.CDLL:
         "rtm_server_stats": (C.c_int, [plan_p, i32p, i32p, P(C.c_size_t), P(C.c_longlong)]),
         "rtm_server_set_group_bytes": (C.c_int, [plan_p, C.c_size_t]),
         "rtm_server_get_group_bytes": (C.c_int, [plan_p, P(C.c_size_t), i32p]),
+        "rtm_server_failures": (C.c_int, [plan_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -566,6 +567,7 @@ class Server:
         self.h = h
         self.snapshot_mode = snapshot_mode
         self.params = {}
+        self.failed = {}        # request id -> reason, for requests that failed
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -622,15 +624,25 @@ class Server:
         return n
 
     def done(self):
-        """Completed requests not yet collected: [(id, image [Nx][Nz])]."""
+        """Retired requests not yet collected: [(id, image [Nx][Nz])], with
+        None for a request that failed (its reason in self.failed[id])."""
         out = []
         rid = C.c_int()
         while lib().rtm_server_next_done(self.h, C.byref(rid)) == 1:
             nx, nz = self.params.pop(rid.value)
             I = np.empty((nx, nz), np.float32)
-            _check(lib().rtm_server_take_image(self.h, rid.value, fptr(I), I.size), "take_image")
+            rc = lib().rtm_server_take_image(self.h, rid.value, fptr(I), I.size)
+            if rc == -2:
+                self.failed[rid.value] = lib().rtm_last_error().decode()
+                out.append((rid.value, None))
+                continue
+            _check(rc, "take_image")
             out.append((rid.value, I))
         return out
+
+    def failures(self) -> int:
+        """Requests failed since the server was created."""
+        return int(lib().rtm_server_failures(self.h))
 
     def stats(self):
         a, q = C.c_int(), C.c_int()

@@ -41,10 +41,12 @@ def percentiles(lat) -> Dict[str, float]:
 def run(server, requests: List[Request], on_done: Optional[Callable] = None, log=None,
         progress: Optional[Callable] = None, progress_every: int = 100) -> Dict:
     """Drive ``server`` through ``requests`` (sorted by arrival).  Returns
-    per-request latency (completion - arrival, seconds), the wall time and
-    the image throughput.  ``on_done(index, image)`` sees every result;
-    ``progress(done, seconds)`` is called after every ``progress_every``
-    completions."""
+    per-request latency (completion - arrival, seconds) of the completed
+    requests, the wall time, the image throughput and the failed requests
+    (index -> reason: the server retires a failed request and carries on).
+    ``on_done(index, image)`` sees every result (image None for a failed
+    request); ``progress(done, seconds)`` is called after every
+    ``progress_every`` retirements."""
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
@@ -52,6 +54,7 @@ def run(server, requests: List[Request], on_done: Optional[Callable] = None, log
     pending = 0
     id2idx: Dict[int, int] = {}
     done_at = [None] * len(requests)
+    failed: Dict[int, str] = {}
     max_active = max_queued = 0
     while nxt < len(order) or pending:
         while nxt < len(order) and requests[order[nxt]].arrival <= now():
@@ -68,6 +71,8 @@ def run(server, requests: List[Request], on_done: Optional[Callable] = None, log
             for rid, img in server.done():
                 i = id2idx.pop(rid)
                 done_at[i] = now()
+                if img is None:
+                    failed[i] = getattr(server, "failed", {}).get(rid, "failed")
                 pending -= 1
                 ndone = len(requests) - pending - (len(order) - nxt)
                 if progress and ndone % progress_every == 0:
@@ -80,10 +85,12 @@ def run(server, requests: List[Request], on_done: Optional[Callable] = None, log
         elif nxt < len(order):
             time.sleep(max(0.0, min(requests[order[nxt]].arrival - now(), 0.05)))
     wall = now()
-    lat = [done_at[i] - requests[i].arrival for i in range(len(requests))]
-    return {"latency_s": percentiles(lat), "wall_seconds": wall,
-            "images_per_hour": len(requests) * 3600.0 / wall,
-            "max_active": max_active, "max_queued": max_queued, "latencies": lat}
+    ok = [i for i in range(len(requests)) if i not in failed]
+    lat = [done_at[i] - requests[i].arrival for i in ok]
+    return {"latency_s": percentiles(lat) if lat else {}, "wall_seconds": wall,
+            "images_per_hour": len(ok) * 3600.0 / wall,
+            "max_active": max_active, "max_queued": max_queued, "latencies": lat,
+            "failed": failed}
 
 
 def sequential_latencies(arrivals, service) -> List[float]:

@@ -52,3 +52,36 @@ def test_serve_loop_collects_every_request_and_reports_progress():
     assert marks == [100, 200]
     lat = res["latencies"]
     assert len(lat) == n and min(lat) >= 0.0 and res["wall_seconds"] >= max(lat)
+
+
+class _FailingServer(_FakeServer):
+    """Fails every third request (as the request server retires a request
+    whose launch group failed: image None, the reason kept by the server)."""
+
+    def __init__(self):
+        super().__init__()
+        self.failed = {}
+
+    def done(self):
+        out = []
+        for rid in self.finished:
+            if rid % 3 == 0:
+                self.failed[rid] = f"request {rid} failed: launch group"
+                out.append((rid, None))
+            else:
+                out.append((rid, np.zeros(1, np.float32)))
+        self.finished = []
+        return out
+
+
+def test_serve_loop_reports_failed_requests_and_serves_the_rest():
+    n = 30
+    reqs = [serve.Request(0.0, (lambda: (None, None, None, None, 0, 0)), "x") for _ in range(n)]
+    images = {}
+    srv = _FailingServer()
+    res = serve.run(srv, reqs, on_done=lambda i, img: images.__setitem__(i, img))
+    assert sorted(images) == list(range(n))
+    assert sorted(res["failed"]) == [i for i in range(n) if i % 3 == 0]
+    assert all(images[i] is None for i in res["failed"])
+    assert all("launch group" in why for why in res["failed"].values())
+    assert len(res["latencies"]) == n - len(res["failed"])
