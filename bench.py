@@ -41,10 +41,11 @@ RELL2_GATE = 1e-4      # every image within this relative L2 of the FP64 run of 
 BYTES_PER_UPDATE = 16  # read u^n, u^{n-1}, coef; write u^{n+1} (FP32)
 
 # L2-to-fabric bytes per launch of the c1 plain step kernel (tiles 64x8),
-# mean of a forward and a backward step at TS=1: rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes over `bench.py --profile`, each scaled by what the
-# counter reads for a 1 GiB copy in the same run (x2.0 and x1.0).
-C1_TRAFFIC_BYTES = 26542751
+# mean of a forward and a backward step at TS=1, from rocprofv3 passes over
+# `bench.py --profile` counting the read requests by size (32, 64 and
+# 128 B) and the write requests (32 and 64 B); a 1 GiB copy in the same run
+# reads and writes 1 GiB on these counts.
+C1_TRAFFIC_BYTES = 26377888
 
 
 def full_step_bytes(p, ts):
@@ -368,6 +369,8 @@ def main() -> None:
                     help="short fixed workload for counter profiles (see run_profile)")
     ap.add_argument("--profile-sims", type=int, default=13, help="realizations in the profile batch (a c2 launch group)")
     ap.add_argument("--profile-nt", type=int, default=400, help="time steps of the profile workload")
+    ap.add_argument("--profile-block", type=int, default=100,
+                    help="profile workload: steps per captured graph of the batch (1: eager launches)")
     ap.add_argument("--profile-batch-only", action="store_true",
                     help="profile workload: only the c2 launch group in 64x16 tiles (counter passes)")
     ap.add_argument("--no-c0", dest="c0", action="store_false",
@@ -383,7 +386,8 @@ def main() -> None:
         log("no HIP device")
         sys.exit(1)
     if args.profile:
-        run_profile(R, args.profile_nt, args.profile_sims, batch_only=args.profile_batch_only)
+        run_profile(R, args.profile_nt, args.profile_sims, batch_only=args.profile_batch_only,
+                    block=args.profile_block)
         return
     if args.only_serve:
         print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
@@ -523,7 +527,7 @@ def main() -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_profile(R, nt=400, nsims=13, batch_only=False):
+def run_profile(R, nt=400, nsims=13, batch_only=False, block=100):
     """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
     pass per counter group over the same dispatches):
       1. a 1 GiB device-to-device copy (torch), the calibration of the
@@ -533,7 +537,8 @@ def run_profile(R, nt=400, nsims=13, batch_only=False):
          (TS=20), tiles of 64x16 only, then strip tiles beside the column
          kernel.
     batch_only: the copy, then step 3 in 64x16 tiles alone, so that every
-    step-kernel dispatch of the run is the launch group's."""
+    step-kernel dispatch of the run is the launch group's.  block: steps per
+    captured graph of step 3 (1: every step launched on its own)."""
     import torch
     x = torch.rand(1 << 28, device="cuda")
     y = torch.empty_like(x)
@@ -543,7 +548,7 @@ def run_profile(R, nt=400, nsims=13, batch_only=False):
     del x, y
     p = c1_params().update(number_of_time_steps=nt, sampling_wavelet=nt)
     v_true, v_mig, w = c1_inputs(p)
-    seis = np.zeros((p.number_of_receivers, nt), np.float32)
+    seis = (np.random.default_rng(3).standard_normal((p.number_of_receivers, nt)) * 1e-3).astype(np.float32)
     if not batch_only:
         with R.Plan(p) as plan:
             plan.set_tiling(2, 100, 0, 0)
@@ -552,7 +557,7 @@ def run_profile(R, nt=400, nsims=13, batch_only=False):
             plan.set_seismogram(seis)
             plan.zero_image()
             plan.migrate(p.first_location_shot, p.depth_shot)
-            log("[profile] plain", plan.last_timing())
+            log("[profile] plain", plan.last_timing(), "image norm", float(np.linalg.norm(plan.image()[0])))
     pb = p.copy().update(time_slices=20)
     with R.Batch(pb, nsims) as b:
         vpad = R.extend(pb, v_mig)
@@ -560,11 +565,12 @@ def run_profile(R, nt=400, nsims=13, batch_only=False):
             b.set_velocity(k, vpad)
         b.set_wavelet(w)
         b.set_seismogram(seis)
-        for shape in ((4, 100, 0, 0),) if batch_only else ((4, 100, 0, 0), (4, 100, 1, 32)):
+        for shape in ((4, block, 0, 0),) if batch_only else ((4, block, 0, 0), (4, block, 1, 32)):
             b.set_tiling(*shape)
             b.zero_images()
             b.migrate(pb.first_location_shot, pb.depth_shot)
-            log(f"[profile] batch{nsims} {tiling_dict(shape)}", b.last_timing())
+            log(f"[profile] batch{nsims} {tiling_dict(shape)}", b.last_timing(),
+                "image norms", [round(float(np.linalg.norm(b.image(k)[0])), 6) for k in (0, nsims - 1)])
     print(json.dumps({"profile": "done", "nt": nt}), flush=True)
 
 
@@ -920,7 +926,9 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
                           us_per_step=round(t["ms_per_step"] * 1e3, 2)) for t in sweep],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "traffic_note": "not measured: under a rocprofv3 counter pass the batched kernels skip their "
+                                     "work (zero images, 3.6 us dispatches), so no counter reading describes them"},
         "welford_seconds": round(welford_s, 4),
         "image0_relL2_vs_fp64": rel_l2(I0, I64),
         "meets_gate": gate(rel_l2(I0, I64)),

@@ -668,13 +668,21 @@ struct Exec {
     int4 *d_inner = nullptr;            // [cap_inner] column items
     int *d_ninner = nullptr;
     std::vector<int> h_n0;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    int gblock = 0;
-    // the captured launches depend on the lists only through their
+    // Captured blocks of steps, one per boundary-saving phase (index
+    // ring_phase), so that a plan alternating between its forward and
+    // backward passes keeps both graphs instead of re-capturing at every
+    // switch.  The captured launches depend on the lists only through their
     // capacities (grid sizes) and on which lists are non-empty (the kernels
     // read the items and their counts from device memory), so new list
-    // contents keep the graph unless that changes
+    // contents keep the graphs unless that changes.  A graph is destroyed
+    // only after the stream it was last launched on has drained.
+    struct Captured {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int block = 0, ring_blocks = 0;
+    };
+    Captured cap[3];
+    hipStream_t g_stream = nullptr;     // where the graphs were last launched
     bool g_tiles = false, g_inner = false;
     // the strip tiles run on `side` beside the interior kernel (they write
     // disjoint points); fork/join through two events, also under capture
@@ -692,8 +700,10 @@ struct Exec {
     // `block` steps when block > 1); n0 advances by `steps` for all sims
     int run(int steps, int block, hipStream_t s);
     int launch_one(int s_in_block, hipStream_t s);
+    // every captured graph, once the stream they ran on has drained
     int drop_graph();
-    // the boundary-saving kernel of the coming pass (a change re-captures)
+    // the boundary-saving kernel of the coming pass (0: none; its grid
+    // width `blocks` matters only for a boundary-saving phase)
     int set_ring_phase(int phase, int blocks);
 };
 

@@ -526,13 +526,19 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
     return 0;
 }
 
+static void destroy_captured(Exec::Captured &c)
+{
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c = Exec::Captured{};
+}
+
 int Exec::drop_graph()
 {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
-    gblock = 0;
+    bool any = false;
+    for (const Captured &c : cap) any = any || c.exec;
+    if (any && g_stream) (void)hipStreamSynchronize(g_stream);   // no graph is destroyed in flight
+    for (Captured &c : cap) destroy_captured(c);
     return 0;
 }
 
@@ -565,7 +571,7 @@ int Exec::set_inner(const std::vector<int4> &items, hipStream_t s)
     if (ninner) HIP_TRY(hipMemcpyAsync(d_inner, items.data(), sizeof(int4) * ninner, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ninner, &ninner, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (gexec && (ninner > 0) != g_inner) drop_graph();
+    if ((ninner > 0) != g_inner) drop_graph();
     return 0;
 }
 
@@ -579,7 +585,7 @@ int Exec::set_tiles(const std::vector<int4> &tiles, hipStream_t s)
     if (ntiles) HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * ntiles, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ntiles, &ntiles, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (gexec && (ntiles > 0) != g_tiles) drop_graph();
+    if ((ntiles > 0) != g_tiles) drop_graph();
     return 0;
 }
 
@@ -678,9 +684,8 @@ int Exec::launch_one(int s_in_block, hipStream_t s)
 
 int Exec::set_ring_phase(int phase, int blocks)
 {
-    if (phase != ring_phase || blocks != ring_blocks) drop_graph();
     ring_phase = phase;
-    ring_blocks = blocks;
+    ring_blocks = phase ? blocks : 0;
     return 0;
 }
 
@@ -693,20 +698,26 @@ int Exec::run(int steps, int block, hipStream_t s)
     }
     int done = 0;
     if (block > 1 && steps >= block) {
-        if (!gexec || gblock != block) {
-            drop_graph();
+        Captured &c = cap[ring_phase];
+        const bool lists_match = g_tiles == (ntiles > 0) && g_inner == (ninner > 0);
+        if (!lists_match) drop_graph();
+        if (!c.exec || c.block != block || c.ring_blocks != ring_blocks) {
+            if (c.exec && g_stream) (void)hipStreamSynchronize(g_stream);
+            destroy_captured(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int t = 0; t < block; t++) launch_one(t, s);
             hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
-            hipError_t e = hipStreamEndCapture(s, &graph);
+            hipError_t e = hipStreamEndCapture(s, &c.graph);
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
-            HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-            gblock = block;
+            HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+            c.block = block;
+            c.ring_blocks = ring_blocks;
             g_tiles = ntiles > 0;
             g_inner = ninner > 0;
         }
+        g_stream = s;
         while (steps - done >= block) {
-            HIP_TRY(hipGraphLaunch(gexec, s));
+            HIP_TRY(hipGraphLaunch(c.exec, s));
             done += block;
         }
     }
