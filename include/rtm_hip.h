@@ -276,6 +276,15 @@ int rtm_batch_get_image_f64(rtm_batch *batch, int sim, double *I, double *Is);
 /* mean and unbiased variance of the realizations' images, FP64 Welford in
  * realization order, [Nx][Nz] */
 int rtm_batch_welford(rtm_batch *batch, double *mean, double *var);
+/* rtm_batch_welford with no device allocation: no reference counterpart;
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results, bit for bit.  Its
+ * partial results live in a fixed device array of the library (32 MiB +
+ * 32 KiB per device, held until the process ends): at most 4096
+ * realizations and 1,048,576 image points (Nx*Nz), larger batches are
+ * refused with an error.  All calls on one device share that array, so
+ * calls that overlap on one device race. */
+int rtm_batch_welford_noalloc(rtm_batch *batch, double *mean, double *var);
 int rtm_batch_last_timing(rtm_batch *batch, double *fwd_step_ms, double *bwd_step_ms, int *steps);
 
 /* ---------------------------------------------------------------------------

@@ -163,6 +163,7 @@ def lib() -> C.CDLL:
         "rtm_batch_get_image": (C.c_int, [plan_p, C.c_int, f32p, f32p]),
         "rtm_batch_get_image_f64": (C.c_int, [plan_p, C.c_int, f64p, f64p]),
         "rtm_batch_welford": (C.c_int, [plan_p, f64p, f64p]),
+        "rtm_batch_welford_noalloc": (C.c_int, [plan_p, f64p, f64p]),
         "rtm_batch_last_timing": (C.c_int, [plan_p, P(C.c_double), P(C.c_double), i32p]),
         "rtm_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, P(plan_p)]),
         "rtm_server_destroy": (C.c_int, [plan_p]),
@@ -536,6 +537,14 @@ class Batch:
         shape = (self.p.nx, self.p.nz)
         m, v = np.empty(shape, np.float64), np.empty(shape, np.float64)
         _check(lib().rtm_batch_welford(self.h, dptr(m), dptr(v)), "welford")
+        return m, v
+
+    def welford_noalloc(self):
+        """welford() through rtm_batch_welford_noalloc: no device allocation
+        (a fixed scratch array of the library), the same results."""
+        shape = (self.p.nx, self.p.nz)
+        m, v = np.empty(shape, np.float64), np.empty(shape, np.float64)
+        _check(lib().rtm_batch_welford_noalloc(self.h, dptr(m), dptr(v)), "welford_noalloc")
         return m, v
 
     def last_timing(self):

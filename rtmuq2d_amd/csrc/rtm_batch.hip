@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cstdlib>
 
+#include <mutex>
+
 #include "rtm_sim.h"
 
 using namespace rtmd;
@@ -41,6 +43,71 @@ __global__ void k_welford(const R *const *__restrict__ imgs, int N, size_t n, do
         mean[q] = m;
         m2[q] = s;
     }
+}
+
+// [Nz][Nx] FP64 mean and M2 -> the caller's [Nx][Nz] mean and variance
+// (M2 / (N-1), 0 when N = 1), 32 x 32 tiles through LDS
+__global__ void __launch_bounds__(256) k_welford_out(const double *__restrict__ mean, const double *__restrict__ m2,
+                                                     int N, int nx, int nz, double *__restrict__ out_mean,
+                                                     double *__restrict__ out_var)
+{
+    __shared__ double tm[32][33], tv[32][33];
+    const int i0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int k = k0 + r, i = i0 + tx;
+        if (k < nz && i < nx) {
+            const size_t q = (size_t)k * nx + i;
+            tm[r][tx] = mean[q];
+            tv[r][tx] = N > 1 ? m2[q] / (double)(N - 1) : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, k = k0 + tx;
+        if (i < nx && k < nz) {
+            out_mean[(size_t)i * nz + k] = tm[tx][r];
+            out_var[(size_t)i * nz + k] = tv[tx][r];
+        }
+    }
+}
+
+// The scratch of rtm_batch_welford_noalloc: a fixed array of this file,
+// one copy per device, held until the process ends.  Layout: the
+// realizations' image pointers (kNaMaxSims), then FP64 mean and M2 in the
+// images' [Nz][Nx] order, then mean and variance in [Nx][Nz] order
+// (kNaMaxPoints each).  Every entry a call reads, that call writes first,
+// so nothing needs clearing.
+constexpr int kNaMaxSims = 4096;
+constexpr size_t kNaMaxPoints = size_t(1) << 20;
+constexpr size_t kNaPtrBytes = sizeof(void *) * kNaMaxSims;
+constexpr size_t kNaScratchBytes = kNaPtrBytes + 4 * sizeof(double) * kNaMaxPoints;
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char *g_na_scratch[kNaMaxDevices];   // the array's address on each device, once looked up
+
+// the scratch's device address on `device` (current): looked up with
+// hipGetSymbolAddress on the device's first call; the host-side symbol is
+// never used as a device address
+unsigned char *na_scratch(int device)
+{
+    if (device < 0 || device >= kNaMaxDevices) {
+        rtm_set_error("rtm_batch_welford_noalloc: device %d out of range", device);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> l(g_na_mu);
+    if (!g_na_scratch[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            rtm_set_error("rtm_batch_welford_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_na_scratch[device] = (unsigned char *)ptr;
+    }
+    return g_na_scratch[device];
 }
 
 }  // namespace
@@ -380,6 +447,51 @@ extern "C" int rtm_batch_welford(rtm_batch *q, double *mean, double *var)
     if (!q || !mean) { rtm_set_error("rtm_batch_welford: null argument"); return -1; }
     HIP_TRY(hipSetDevice(q->sims[0].p.device));
     return q->real == RTM_REAL_F64 ? batch_welford_t<double>(q, mean, var) : batch_welford_t<float>(q, mean, var);
+}
+
+// rtm_batch_welford without allocating: the same reduction with its partial
+// results in the file's scratch array, the layout change and the division
+// by N-1 on the device, and the results copied straight into the caller's
+// arrays.
+template <typename R>
+static int batch_welford_noalloc_t(rtm_batch *q, unsigned char *base, double *mean, double *var)
+{
+    const int N = (int)q->sims.size();
+    const SimBuffers &b0 = q->sims[0];
+    const size_t n = b0.nint;
+    const void *tab[kNaMaxSims];
+    for (int s = 0; s < N; s++) tab[s] = q->sims[s].I;
+    const R **d_ptrs = (const R **)base;
+    double *d_mean = (double *)(base + kNaPtrBytes), *d_m2 = d_mean + kNaMaxPoints;
+    double *d_omean = d_m2 + kNaMaxPoints, *d_ovar = d_omean + kNaMaxPoints;
+    HIP_TRY(hipMemcpyAsync(d_ptrs, tab, sizeof(void *) * N, hipMemcpyHostToDevice, q->stream));
+    hipLaunchKernelGGL(k_welford<R>, dim3(1024), dim3(256), 0, q->stream, d_ptrs, N, n, d_mean, d_m2);
+    const int nx = b0.p.nx, nz = b0.p.nz;
+    hipLaunchKernelGGL(k_welford_out, dim3((nx + 31) / 32, (nz + 31) / 32), dim3(32, 8), 0, q->stream, d_mean, d_m2,
+                       N, nx, nz, d_omean, d_ovar);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(mean, d_omean, sizeof(double) * n, hipMemcpyDeviceToHost, q->stream));
+    if (var) HIP_TRY(hipMemcpyAsync(var, d_ovar, sizeof(double) * n, hipMemcpyDeviceToHost, q->stream));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return 0;
+}
+
+extern "C" int rtm_batch_welford_noalloc(rtm_batch *q, double *mean, double *var)
+{
+    if (!q || !mean) { rtm_set_error("rtm_batch_welford_noalloc: null argument"); return -1; }
+    const int N = (int)q->sims.size();
+    const size_t n = q->sims[0].nint;
+    if (N > kNaMaxSims || n > kNaMaxPoints) {
+        rtm_set_error("rtm_batch_welford_noalloc: %d realizations of %zu image points exceed its scratch "
+                      "(at most %d realizations and %zu image points)", N, n, kNaMaxSims, kNaMaxPoints);
+        return -1;
+    }
+    const int device = q->sims[0].p.device;
+    HIP_TRY(hipSetDevice(device));
+    unsigned char *base = na_scratch(device);
+    if (!base) return -1;
+    return q->real == RTM_REAL_F64 ? batch_welford_noalloc_t<double>(q, base, mean, var)
+                                   : batch_welford_noalloc_t<float>(q, base, mean, var);
 }
 
 // Times every candidate shape on launch group 0 (its realizations stepping
