@@ -357,6 +357,8 @@ def main() -> None:
     ap.add_argument("--serve-rate", type=float, default=1.0,
                     help="arrival rate as a multiple of the one-at-a-time plain rate")
     ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
+    ap.add_argument("--serve-store", choices=("i16", "fp16"), default="i16",
+                    help="request server's snapshot store: int16 blocks or FP16 with a per-snapshot scale")
     ap.add_argument("--serve-tiling", default="4,100,0,0",
                     help="request server kernel shape: rows_per_thread,graph_block,interior,seg_rows")
     ap.add_argument("--tiling", default=None,
@@ -391,7 +393,8 @@ def main() -> None:
         return
     if args.only_serve:
         print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
-                                     tuple(int(v) for v in args.serve_tiling.split(",")))), flush=True)
+                                     tuple(int(v) for v in args.serve_tiling.split(",")),
+                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16)), flush=True)
         return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
@@ -519,7 +522,8 @@ def main() -> None:
         out["precision"] = run_precision(R, p, v_mig, w, seis, best[1])
     if args.serve > 0:
         out["serving"] = run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
-                                     tuple(int(v) for v in args.serve_tiling.split(",")))
+                                     tuple(int(v) for v in args.serve_tiling.split(",")),
+                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16)
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
@@ -587,7 +591,7 @@ def run_precision(R, p1, v_mig, w, seis, shape):
     sx, sz = p.first_location_shot, p.depth_shot
     vpad = R.extend(p, v_mig)
 
-    def image(q, real=R.REAL_F32):
+    def image(q, real=R.REAL_F32, shape=shape):
         with R.Plan(q, real) as plan:
             if real == R.REAL_F32:
                 plan.set_tiling(*shape)
@@ -614,13 +618,16 @@ def run_precision(R, p1, v_mig, w, seis, shape):
     X = p.nx + p.thickness_left + p.thickness_right
     Z = p.nz + p.thickness_up + p.thickness_down
     for name, mode, kahan in (("fp32", R.SNAP_FP32, 1), ("fp16_scaled", R.SNAP_FP16, 1),
-                              ("bf16_scaled", R.SNAP_BF16, 1), ("fp32_no_kahan", R.SNAP_FP32, 0),
-                              ("boundary_saving", R.SNAP_BOUNDARY, 1)):
+                              ("bf16_scaled", R.SNAP_BF16, 1), ("int16_block", R.SNAP_I16, 1),
+                              ("fp32_no_kahan", R.SNAP_FP32, 0), ("boundary_saving", R.SNAP_BOUNDARY, 1)):
         q = p.copy().update(snapshot_mode=mode, image_kahan=kahan)
-        I, dt = image(q)
+        I, dt = image(q, shape=shape if mode != R.SNAP_I16 or shape[2] == 0 else (shape[0], shape[1], 0, 0))
         err = np.abs(I.astype(np.float64) - I64)
         if mode == R.SNAP_BOUNDARY:     # ring saves + the two source-field buffers (pitch 64)
             store = nt * ring * 4 + 2 * Z * ((X + 63) // 64 * 64) * 4
+        elif mode == R.SNAP_I16:        # codes + a byte per row and 64-column block
+            nqb = ((p.thickness_left + p.nx - 5) >> 6) - ((p.thickness_left - 4) >> 6) + 1
+            store = (nt // 4) * (nint * 2 + p.nz * nqb)
         else:
             store = (nt // 4) * nint * (2 if mode in (R.SNAP_FP16, R.SNAP_BF16) else 4)
         v = {"variant": name, "kahan_image": bool(kahan), "seconds_per_image": round(dt, 4),
@@ -645,23 +652,25 @@ SERVE_CLASSES = [
 ]
 
 
-def serve_params(R, cls):
+def serve_params(R, cls, store=None):
     label, nx, nz, h, dt, nt, T, f0 = cls
+    store = R.SNAP_FP16 if store is None else store
     return R.make_params(
         nx=nx, nz=nz, thickness_left=T, thickness_right=T, thickness_up=T, thickness_down=T,
         spacing_grid=h, sampling=dt, number_of_time_steps=nt, sampling_wavelet=nt,
         first_location_shot=nx // 2, depth_shot=2, spacing_shot=1, number_of_shots=1,
         first_location_receiver=0, depth_receiver=2, spacing_receiver=2, number_of_receivers=nx,
-        number_of_simulations=1, zfp_tolerance=1e-4, time_slices=4, snapshot_mode=R.SNAP_FP16, image_kahan=1)
+        number_of_simulations=1, zfp_tolerance=1e-4, time_slices=4, snapshot_mode=store, image_kahan=1)
 
 
-def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
+def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11, store=None):
     """Config c4: a Poisson stream of `nreq` migration requests, grid class
     drawn uniformly from SERVE_CLASSES, each with its own seeded folded /
     faulted velocity realization and its own source position along the
     surface, its own traces (a window of rows, at a per-request offset, of
     a seeded synthetic gather of its class with extra rows: distinct per
-    request, and no copy), FP16 snapshot store every 4 steps, Kahan image.
+    request, and no copy), a 16-bit snapshot store every 4 steps (`store`:
+    the int16 block store by default, or FP16), Kahan image.
     Two requests of each class, the first to arrive, are checked afterwards:
     the served image against the FP64 run of the same request (FP64
     stepping and snapshots) and against the same request run alone in a
@@ -672,11 +681,12 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
     arrival order.  `shape` is the server's kernel shape: one shape serves
     every grid in a launch."""
     from rtmuq2d_amd import serve, synth
+    store = R.SNAP_I16 if store is None else store
     rng = np.random.default_rng(seed)
     extra = 256                              # rows of offset a request's window can take
     ctx = []
     for cls in SERVE_CLASSES:
-        p = serve_params(R, cls)
+        p = serve_params(R, cls, store)
         wav = synth.ricker(cls[7], p.sampling, p.sampling_wavelet)
         g = np.random.default_rng(seed + cls[1])
         base = (g.standard_normal((p.number_of_receivers + extra, p.number_of_time_steps),
@@ -733,7 +743,7 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
         if i in checked:
             served[i] = img
 
-    with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=R.SNAP_FP16) as srv:
+    with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=store) as srv:
         srv.set_tiling(*shape)
         res = serve.run(srv, reqs, on_done=keep,
                         progress=lambda n, t: log(f"[bench] c4 {n} requests served at {t:.1f} s"))
@@ -750,7 +760,8 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11):
     out = {
         "workload": (f"c4 serving: {nreq} Poisson requests at {rate:.2f}/s ({rate_factor}x the one-at-a-time "
                      "plain rate), classes " + ", ".join(c[0] for c in SERVE_CLASSES)
-                     + "; FP16 snapshots every 4 steps, Kahan image"),
+                     + f"; {STORE_NAMES[store]} snapshots every 4 steps, Kahan image"),
+        "snapshot_store": STORE_NAMES[store],
         "requests": nreq, "arrival_rate_per_s": round(rate, 3), "server_tiling": tiling_dict(shape),
         "class_counts": [int((classes == k).sum()) for k in range(len(SERVE_CLASSES))],
         "latency_s": {k: round(v, 4) for k, v in res["latency_s"].items()},
@@ -792,7 +803,8 @@ def serve_accuracy(R, items):
     """Each served image of `items` [(index, label, inputs, image)] against
     the FP64 plan of the same request (FP64 stepping, FP64 snapshots at the
     same interval, FP64 image) and against the FP32 plan of it alone with the
-    server's FP16 store (the same arithmetic: equal bit for bit)."""
+    server's store (the same arithmetic: equal bit for bit); and, beside it,
+    the relL2 of the same request alone with the FP16 store."""
     out = []
     for i, label, (p, vpad, wav, seis, sx, sz), img in items:
         rec = {"request": i, "class": label}
@@ -807,19 +819,27 @@ def serve_accuracy(R, items):
             p64.zero_image()
             p64.migrate(sx, sz)
             I64, _ = p64.image(f64=True)
-        with R.Plan(p) as alone:
-            alone.set_velocity(vpad)
-            alone.set_wavelet(wav)
-            alone.set_seismogram(seis)
-            alone.zero_image()
-            alone.migrate(sx, sz)
-            I1, _ = alone.image()
+        alone = {}
+        for mode in {p.snapshot_mode, R.SNAP_FP16}:
+            with R.Plan(p.copy().update(snapshot_mode=mode)) as plan:
+                plan.set_velocity(vpad)
+                plan.set_wavelet(wav)
+                plan.set_seismogram(seis)
+                plan.zero_image()
+                plan.migrate(sx, sz)
+                alone[mode] = plan.image()[0]
         rel = rel_l2(img, I64)
-        rec.update(relL2_vs_fp64=rel, meets_gate=gate(rel), equals_plan_alone=bool(np.array_equal(img, I1)))
+        rec.update(relL2_vs_fp64=rel, meets_gate=gate(rel),
+                   equals_plan_alone=bool(np.array_equal(img, alone[p.snapshot_mode])),
+                   fp16_store_relL2_vs_fp64=rel_l2(alone[R.SNAP_FP16], I64))
         log(f"[bench] c4 request {i} ({label}): relL2 vs FP64 {rel:.3e}, equals the plan alone: "
             f"{rec['equals_plan_alone']}")
         out.append(rec)
     return out
+
+
+STORE_NAMES = {0: "FP32", 1: "FP16 (per-snapshot scale)", 2: "BF16 (per-snapshot scale)", 3: "boundary saving",
+               4: "int16 blocks (a power-of-two scale per 64-point run)"}
 
 
 def rel_l2(a, b):

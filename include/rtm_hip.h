@@ -37,7 +37,9 @@ enum rtm_snapshot_mode {
     RTM_SNAP_FP32 = 0,      /* snapshot as computed                        */
     RTM_SNAP_FP16 = 1,      /* FP16 with a per-snapshot max-abs scale      */
     RTM_SNAP_BF16 = 2,      /* BF16 with a per-snapshot max-abs scale      */
-    RTM_SNAP_BOUNDARY = 3   /* boundary saving: 4-point ring, reconstruct  */
+    RTM_SNAP_BOUNDARY = 3,  /* boundary saving: 4-point ring, reconstruct  */
+    RTM_SNAP_I16 = 4        /* int16 with a power-of-two scale per 64-point
+                               run of an image row (tile shapes only)     */
 };
 
 enum rtm_real { RTM_REAL_F32 = 0, RTM_REAL_F64 = 1 };

@@ -154,7 +154,8 @@ static void batch_drop_execs(rtm_batch *q)
 // tiling call succeeds.
 static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
 {
-    if (check_shape(rz, interior)) return -1;   // the current shape stays
+    // the current shape stays when the new one is refused
+    if (check_shape(rz, interior) || check_store_shape(q->sims[0].p.snapshot_mode, interior)) return -1;
     const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
     batch_drop_execs(q);
     q->ex.resize(ng);       // never resized again before the next release
@@ -167,7 +168,8 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
         balance_tiles_for_xcds(tiles, 4 * rz);
         Exec &ex = q->ex[k];
         if (ex.init(q->real, n, (int)tiles.size(), rz, (int)inner.size(), interior)) return fail();
-        ex.q16 = q->sims[0].qmode != rtm::Q_NONE;
+        ex.q16 = q->sims[0].qmode == rtm::Q_FP16 || q->sims[0].qmode == rtm::Q_BF16;
+        ex.i16 = q->sims[0].qmode == rtm::Q_I16;
         if (ex.set_tiles(tiles, q->stream) || ex.set_inner(inner, q->stream)) return fail();
     }
     q->rz = rz;
@@ -181,7 +183,7 @@ static std::string batch_key(const rtm_batch *q)
 {
     std::vector<const rtm_params *> g;
     for (const auto &s : q->sims) g.push_back(&s.p);
-    return tuner_key(q->real, !q->sims.empty() && q->sims[0].qmode != rtm::Q_NONE, g);
+    return tuner_key(q->real, q->sims.empty() ? 0 : q->sims[0].qmode, g);
 }
 
 extern "C" size_t rtm_batch_footprint(const rtm_params *p, int nsims, int real)
@@ -199,7 +201,7 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     *out = nullptr;
     if (rtm_params_check(p)) return -1;
     if (real != RTM_REAL_F32 && real != RTM_REAL_F64) { rtm_set_error("rtm_batch_create: bad real type"); return -1; }
-    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BOUNDARY) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_I16) {
         rtm_set_error("rtm_batch_create: snapshot mode %d not available", p->snapshot_mode);
         return -1;
     }
@@ -512,6 +514,7 @@ static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, doubl
     // saving keeps none
     const int flags = snaps && !q->sims[0].ringmode ? rtm::F_SNAP : 0;
     for (const Shape &c : cand) {
+        if (check_store_shape(q->sims[0].p.snapshot_mode, c.interior)) continue;   // not a shape of this store
         if (batch_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
         double t = 0.0;
         if (time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, &t, flags)) return -1;

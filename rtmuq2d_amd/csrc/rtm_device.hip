@@ -83,13 +83,15 @@ static void plan_free(rtm_plan *q)
 
 static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, int seg_rows)
 {
-    if (check_shape(rz, interior)) return -1;   // the current shape stays
+    // the current shape stays when the new one is refused
+    if (check_shape(rz, interior) || check_store_shape(q->b.p.snapshot_mode, interior)) return -1;
     std::vector<int4> tiles, inner;
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
     balance_tiles_for_xcds(tiles, 4 * rz);
     q->ex.release();
     if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
-    q->ex.q16 = q->b.qmode != rtm::Q_NONE;
+    q->ex.q16 = q->b.qmode == rtm::Q_FP16 || q->b.qmode == rtm::Q_BF16;
+    q->ex.i16 = q->b.qmode == rtm::Q_I16;
     if (q->ex.set_tiles(tiles, q->stream) || q->ex.set_inner(inner, q->stream)) return -1;
     q->rz = rz;
     q->graph_block = graph_block;
@@ -107,7 +109,7 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
         rtm_set_error("rtm_plan_create: unknown real type %d", real);
         return -1;
     }
-    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BOUNDARY) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_I16) {
         rtm_set_error("rtm_plan_create: snapshot mode %d is not available", p->snapshot_mode);
         return -1;
     }
@@ -124,7 +126,7 @@ extern "C" int rtm_plan_create(const rtm_params *p, int real, rtm_plan **out)
     // tiles of 64 x 16 ($RTM_TILE_RZ overrides the tile height)
     const char *env = getenv("RTM_TILE_RZ");
     Shape sh{env && *env ? atoi(env) : 4, rtm::IN_TILES, 0};
-    if (!(env && *env)) tuner_lookup(tuner_key(real, q->b.qmode != rtm::Q_NONE, {&q->b.p}), &sh);
+    if (!(env && *env)) tuner_lookup(tuner_key(real, q->b.qmode, {&q->b.p}), &sh);
     if (plan_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
     *out = q;
     return 0;
@@ -648,6 +650,7 @@ static int plan_autotune_t(rtm_plan *q, int steps, int cap, int *shapes, double 
     Shape best = cand[0];
     int k = 0;
     for (const Shape &c : cand) {
+        if (check_store_shape(q->b.p.snapshot_mode, c.interior)) continue;   // not a shape of this store
         if (plan_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
         double t = 0.0;
         if (time_steps<R>(q->ex, {q->b.desc<R>()}, steps, q->graph_block, q->stream, &t)) return -1;
@@ -663,7 +666,7 @@ static int plan_autotune_t(rtm_plan *q, int steps, int cap, int *shapes, double 
     }
     if (n_out) *n_out = k < cap ? k : cap;
     if (plan_set_tiling(q, best.rz, q->graph_block, best.interior, best.seg)) return -1;
-    tuner_store(tuner_key(q->b.real, q->b.qmode != rtm::Q_NONE, {&q->b.p}), best);
+    tuner_store(tuner_key(q->b.real, q->b.qmode, {&q->b.p}), best);
     return 0;
 }
 

@@ -98,7 +98,7 @@ int rtm_params_check(const rtm_params *p)
         rtm_set_error("params: negative sampling_wavelet or number_of_receivers");
         return -1;
     }
-    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_BOUNDARY) {
+    if (p->snapshot_mode < RTM_SNAP_FP32 || p->snapshot_mode > RTM_SNAP_I16) {
         rtm_set_error("params: unknown snapshot_mode %d", p->snapshot_mode);
         return -1;
     }

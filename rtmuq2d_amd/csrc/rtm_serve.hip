@@ -125,7 +125,8 @@ struct HipBackend {
             e.release();
             const int ct = (int)(tiles.size() * 3 / 2) + 64, ci = (int)(inner.size() * 3 / 2) + 64;
             if (e.init(RTM_REAL_F32, max_slots, ct, rz, ci, interior)) return fail("executor");
-            e.q16 = snapshot_mode != RTM_SNAP_FP32;
+            e.q16 = snapshot_mode == RTM_SNAP_FP16 || snapshot_mode == RTM_SNAP_BF16;
+            e.i16 = snapshot_mode == RTM_SNAP_I16;
         }
         if (e.set_tiles(tiles, stream) || e.set_inner(inner, stream)) return fail("work lists");
         return 0;
@@ -227,7 +228,7 @@ extern "C" int rtm_server_create(int device, size_t hbm_bytes, int max_slots, in
 {
     if (!out || max_slots < 1 || hbm_bytes < (1u << 20)) { rtm_set_error("rtm_server_create: bad argument"); return -1; }
     *out = nullptr;
-    if (snapshot_mode < RTM_SNAP_FP32 || snapshot_mode > RTM_SNAP_BF16) {
+    if (snapshot_mode < RTM_SNAP_FP32 || snapshot_mode > RTM_SNAP_I16 || snapshot_mode == RTM_SNAP_BOUNDARY) {
         rtm_set_error("rtm_server_create: snapshot mode %d not available", snapshot_mode);
         return -1;
     }
@@ -258,7 +259,7 @@ extern "C" int rtm_server_destroy(rtm_server *s)
 extern "C" int rtm_server_set_tiling(rtm_server *s, int rows_per_thread, int graph_block, int interior, int seg_rows)
 {
     if (!s || graph_block < 1) { rtm_set_error("rtm_server_set_tiling: bad argument"); return -1; }
-    if (check_shape(rows_per_thread, interior)) return -1;
+    if (check_shape(rows_per_thread, interior) || check_store_shape(s->be.snapshot_mode, interior)) return -1;
     std::lock_guard<std::mutex> l(s->core.tick_mu);
     s->be.rz = rows_per_thread;
     s->core.block = graph_block;

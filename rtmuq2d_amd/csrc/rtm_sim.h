@@ -78,33 +78,39 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
     }
 }
 
-template <typename R, int RZ, int CLS, bool Q16>
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n, int x0,
                                          int z0, int tx, int ty)
 {
-    const rtm::StepView<R> V = rtm::step_view_q<R, Q16>(S, n);
+    const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16>(S, n);
     if (!V.active) return;
     rtm::TileRegs<R, RZ> G;
-    rtm::tile_fetch<R, RZ, CLS, Q16>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_fetch<R, RZ, CLS, Q16, I16>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
         rtm::tile_psi<R, RZ, CLS>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS, Q16>(L, G, S, V, x0, z0, tx, ty);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16>(L, G, S, V, x0, z0, tx, ty);
     if (Q16 && V.smax) commit_snap_max_wg(V.smax, m, red);
+    if (I16 && V.forward && V.qsnap) {   /* uniform: a store step */
+        float rmax[RZ];
+#pragma unroll
+        for (int rr = 0; rr < RZ; rr++) rmax[rr] = wave_max(fabsf(float(G.uq[rr])));   /* the row's block */
+        rtm::tile_store_i16<R, RZ>(G, S, V, rmax, x0, z0, tx, ty);
+    }
 }
 
-template <typename R, int RZ, bool Q16>
+template <typename R, int RZ, bool Q16, bool I16 = false>
 __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n,
                                              int4 tile, int tx, int ty)
 {
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3, Q16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 0: run_tile<R, RZ, 0, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
     }
 }
 
@@ -114,7 +120,8 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red,
 // ONE: the launch steps a single simulation (a plan), so its descriptor's
 // address is known before the tile entry arrives and the descriptor loads
 // need not wait for it.
-template <typename R, int RZ, bool Q16, bool ONE = false>
+// I16: every sim of the launch keeps an int16 block store (FP32 stepping).
+template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false>
 __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                const int *__restrict__ n0,
                                                const int4 *__restrict__ tiles,
@@ -136,7 +143,7 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    run_tile_cls<R, RZ, Q16>(L, red, S, n, tile, tx, ty);
+    run_tile_cls<R, RZ, Q16, I16>(L, red, S, n, tile, tx, ty);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run
@@ -585,6 +592,8 @@ struct SimBuffers {
     int qmode = rtm::Q_NONE;            // 16-bit snapshot store (modes FP16, BF16)
     unsigned short *q16 = nullptr;      // [nsnap][nz][nx]
     unsigned *smax = nullptr;           // [nsnap] max |u| bits per snapshot
+    signed char *qexp = nullptr;        // int16 block store: [nsnap][nz][nqb] block exponents
+    int qb0 = 0, nqb = 0;               // its first block and blocks per image row
     bool ringmode = false;              // boundary saving (snapshot mode BOUNDARY)
     void *sbuf[2] = {nullptr, nullptr}; // the source wavefield's buffers
     void *ring = nullptr;               // [nt][NR] ring values of every forward step
@@ -650,12 +659,16 @@ inline void append_tiles(std::vector<int4> &out, int sim, const rtm_params &p, i
 // 0 if (rows per thread, interior kernel) is a shape the kernels exist for;
 // otherwise -1 with the reason in the error string
 int check_shape(int rz, int interior);
+// 0 if the snapshot store (enum rtm_snapshot_mode) runs with that interior
+// kernel: the int16 block store runs in tile shapes only
+int check_store_shape(int snapshot_mode, int interior);
 
 struct Exec {
     int real = RTM_REAL_F32;
     int rz = 4;                         // rows per thread (tile height 4*rz)
     int interior = rtm::IN_TILES;       // kernel of the interior items
-    bool q16 = false;                   // the sims keep 16-bit snapshot stores
+    bool q16 = false;                   // the sims keep 16-bit snapshot stores (FP16 / BF16)
+    bool i16 = false;                   // the sims keep int16 block stores (tile shapes only)
     // boundary saving: the pass's extra kernel (rtm::PH_FORWARD: ring save,
     // rtm::PH_BACKWARD: reconstruction, 0: none) and its grid width
     int ring_phase = 0, ring_blocks = 0;
@@ -729,7 +742,7 @@ struct Shape {
 const std::vector<Shape> &tuner_candidates();
 
 // cache key of a launch: element type, store kind and every sim's grid
-std::string tuner_key(int real, bool q16, const std::vector<const rtm_params *> &grids);
+std::string tuner_key(int real, int qmode, const std::vector<const rtm_params *> &grids);
 bool tuner_lookup(const std::string &key, Shape *out);
 void tuner_store(const std::string &key, const Shape &s);
 

@@ -88,6 +88,8 @@ size_t sim_bytes(const rtm_params &p, size_t esz)
     b += 2 * nt * nrec * 4;                         // input and recorded traces
     if (p.snapshot_mode == RTM_SNAP_FP16 || p.snapshot_mode == RTM_SNAP_BF16)
         b += nsnap * nint * 2 + nsnap * rtm::kSnapWords * 4;   // 16-bit snapshot store + max slots
+    else if (p.snapshot_mode == RTM_SNAP_I16)
+        b += nsnap * nint * 2 + nsnap * p.nz * rtm::i16_blocks(p.thickness_left, p.nx);   // codes + block exponents
     else if (p.snapshot_mode == RTM_SNAP_BOUNDARY)
         b += 2 * nfield * esz + nt * (size_t)(8 * (p.nx + 8) + 8 * p.nz) * esz;   // source field + ring saves
     else
@@ -136,7 +138,10 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
     if (!rc) rc = get(&csiX, fb);
     if (!rc) rc = get(&csiZ, fb);
     qmode = p.snapshot_mode == RTM_SNAP_FP16 ? rtm::Q_FP16
-            : p.snapshot_mode == RTM_SNAP_BF16 ? rtm::Q_BF16 : rtm::Q_NONE;
+            : p.snapshot_mode == RTM_SNAP_BF16 ? rtm::Q_BF16
+            : p.snapshot_mode == RTM_SNAP_I16  ? rtm::Q_I16 : rtm::Q_NONE;
+    qb0 = rtm::i16_first_block(p.thickness_left);
+    nqb = rtm::i16_blocks(p.thickness_left, p.nx);
     ringmode = p.snapshot_mode == RTM_SNAP_BOUNDARY;
     NR = 8 * (p.nx + 8) + 8 * p.nz;
     if (!rc && qmode != rtm::Q_NONE && real == RTM_REAL_F64) {
@@ -155,6 +160,9 @@ int SimBuffers::alloc(const rtm_params &pp, int real_type, const SimBuffers *sha
         if (!rc) rc = get(&ring, nt * (size_t)NR * esz);
     } else if (qmode == rtm::Q_NONE) {
         if (!rc) rc = get(&snaps, nsnap * nint * esz);
+    } else if (qmode == rtm::Q_I16) {
+        if (!rc) rc = get((void **)&q16, nsnap * nint * 2);
+        if (!rc) rc = get((void **)&qexp, nsnap * (size_t)p.nz * nqb);
     } else {
         if (!rc) rc = get((void **)&q16, nsnap * nint * 2);
         if (!rc) rc = get((void **)&smax, nsnap * rtm::kSnapWords * 4);
@@ -215,6 +223,7 @@ void SimBuffers::release()
     snaps = I = Is = Ic = Isc = nullptr;
     q16 = nullptr;
     smax = nullptr;
+    qexp = nullptr;
     sbuf[0] = sbuf[1] = ring = nullptr;
 }
 
@@ -390,7 +399,8 @@ __global__ void k_quantize_interior(const R *__restrict__ u, int P, int TL, int 
 int SimBuffers::flush_last_snapshot(hipStream_t s)
 {
     const int nt = p.number_of_time_steps, TS = p.time_slices;
-    if (qmode == rtm::Q_NONE || (nt - 1) % TS != 0) return 0;
+    // the int16 block store writes each snapshot in the step that computes it
+    if (qmode == rtm::Q_NONE || qmode == rtm::Q_I16 || (nt - 1) % TS != 0) return 0;
     const int c = (nt - 1) / TS;
     const dim3 grid((p.nx + 255) / 256, p.nz);
     hipLaunchKernelGGL(k_quantize_interior<float>, grid, dim3(256), 0, s, (const float *)u[nt & 1], P,
@@ -446,6 +456,9 @@ SimDesc<R> SimBuffers::desc() const
     d.qmode = qmode;
     d.q16 = q16;
     d.smax = smax;
+    d.qexp = qexp;
+    d.qb0 = qb0;
+    d.nqb = nqb;
     d.sbuf[0] = (R *)sbuf[0];
     d.sbuf[1] = (R *)sbuf[1];
     d.ring = (R *)ring;
@@ -667,9 +680,42 @@ static void launch_ring(const Exec &e, int s_in_block, hipStream_t s)
         hipLaunchKernelGGL(k_recon<R>, grid, dim3(256), 0, s, d, e.d_n0, s_in_block);
 }
 
+// the int16 block store's launch: tile lists only (check_store_shape)
+static void launch_tiles_i16(const Exec &e, int s_in_block, hipStream_t s)
+{
+    if (!e.ntiles) return;
+    const SimDesc<float> *d = (const SimDesc<float> *)e.d_desc;
+    const int grid = 8 * ((e.cap_tiles + 7) / 8);
+    if (e.cap_sims == 1) {
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
+    } else {
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
+    }
+}
+
+int check_store_shape(int snapshot_mode, int interior)
+{
+    if (snapshot_mode == RTM_SNAP_I16 && interior != rtm::IN_TILES) {
+        rtm_set_error("the int16 block snapshot store runs in tile shapes only (interior kernel 0, got %d)", interior);
+        return -1;
+    }
+    return 0;
+}
+
 int Exec::launch_one(int s_in_block, hipStream_t s)
 {
     if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
+    else if (i16) launch_tiles_i16(*this, s_in_block, s);
     else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
     else launch_tiles<float, false>(*this, s_in_block, s);
     // boundary saving: the ring save / reconstruction follows the step on the
@@ -748,9 +794,9 @@ std::mutex g_tune_mu;
 std::map<std::string, Shape> g_tune;
 }
 
-std::string tuner_key(int real, bool q16, const std::vector<const rtm_params *> &grids)
+std::string tuner_key(int real, int qmode, const std::vector<const rtm_params *> &grids)
 {
-    std::string k = std::to_string(real) + (q16 ? "q" : "f");
+    std::string k = std::to_string(real) + "q" + std::to_string(qmode);
     for (const rtm_params *p : grids) {
         char b[96];
         snprintf(b, sizeof b, "|%d,%d,%d,%d,%d,%d", p->nx, p->nz, p->thickness_left, p->thickness_right,

@@ -123,6 +123,12 @@ struct SimDesc {
     R *sbuf[2];
     R *ring;
     int NR;
+    /* int16 block store (Q_I16): codes in q16, and per snapshot, image row
+     * and block of 64 padded columns the block's binary exponent,
+     * [nt/TS][nz][nqb]; block b covers padded columns kHalf + 64 b ..
+     * kHalf + 64 b + 63, and qb0 is the first block holding image columns */
+    signed char *qexp;
+    int qb0, nqb;
 };
 
 /* The ring of boundary saving: every point within 4 of the image region
@@ -160,7 +166,7 @@ RTM_HD void ring_point(const SimDesc<R> &S, int t, int &i, int &k)
  * value lands in [2^14, 2^15), well inside FP16's range, and the power-of-two
  * scale and its inverse are exact, so the only error is the rounding.  The
  * conversions are integer code shared by the device and the CPU harness. */
-enum QMode : int { Q_NONE = 0, Q_FP16 = 1, Q_BF16 = 2 };
+enum QMode : int { Q_NONE = 0, Q_FP16 = 1, Q_BF16 = 2, Q_I16 = 3 };
 
 /* A snapshot's max |u| is accumulated with atomic max over kSnapSlots
  * counters, kSlotStride words (256 bytes) apart so that concurrent
@@ -258,6 +264,36 @@ RTM_HD float dequant16(unsigned short q, float scale, int mode)
 #endif
 }
 
+/* The int16 block store.  Each image row of a snapshot is cut into blocks
+ * of 64 padded columns aligned with the step kernels' tiles (every tile and
+ * column item starts at kHalf + 64 t), so a block is one wavefront row of a
+ * tile.  A block with max |u| = f 2^E (0.5 <= f < 1) stores each value as
+ * the int16 nearest u 2^(15-E) (ties to even, at most 32767 in magnitude)
+ * and E (clamped to [-100, 100]); it reads back as code 2^(E-15).  Powers of
+ * two scale exactly, so the rounding is the only error: at most 2^-16 of the
+ * block's max, against FP16's 2^-11 of each value, in the same 2 bytes per
+ * point plus a byte per block. */
+constexpr int kQBlock = 64;
+
+RTM_HD int i16_block(int i) { return (i - kHalf) >> 6; }   /* block of padded column i >= kHalf */
+RTM_HD int i16_first_block(int TL) { return i16_block(TL); }
+RTM_HD int i16_blocks(int TL, int nx) { return i16_block(TL + nx - 1) - i16_block(TL) + 1; }
+
+RTM_HD int i16_exp(float maxabs)
+{
+    const int e = (int)((f32_bits(maxabs) >> 23) & 0xff) - 126;   /* maxabs < 2^e for normal values */
+    return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
+RTM_HD short quant_i16(float v, float scale)
+{
+    float r = rintf(v * scale);   /* the current (nearest-even) rounding mode on device and host */
+    r = r > 32767.0f ? 32767.0f : (r < -32767.0f ? -32767.0f : r);
+    return (short)r;
+}
+
+RTM_HD float dequant_i16(short q, int e) { return (float)q * pow2f(e - 15); }
+
 /* The max over a snapshot's kSnapSlots slots.  On the device every lane of
  * the (fully active) wavefront loads one slot and a butterfly of lane shuffles
  * takes the max, one memory round trip; the host loops. */
@@ -310,12 +346,13 @@ struct StepView {
     float qscale;
     unsigned *smax;         /* forward, 16-bit store: this step's snapshot max |U| */
     const R *sfield;        /* backward, boundary saving: the reconstructed source field (padded) */
+    signed char *qexp;      /* int16 block store: the block exponents of the snapshot in qsnap */
 };
 
 /* Q16: the sim keeps a 16-bit snapshot store (SimDesc::qmode != 0).  The
  * kernels dispatch on it per sim, so the FP32-store code carries none of the
  * 16-bit fields. */
-template <typename R, bool Q16>
+template <typename R, bool Q16, bool I16 = false>
 RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 {
     StepView<R> v{};
@@ -335,7 +372,11 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
         v.src_w = v.src ? R(ldg(S.wav, n)) : R(0);
         v.rec_row_out = (S.flags & F_RECORD) ? S.seis_rec + (size_t)n * S.nrec : nullptr;
         const bool store = (S.flags & F_SNAP) && n % S.TS == 0;
-        if (!Q16) {
+        if (I16) {   /* stored in the step that computes the snapshot */
+            const int c = n / S.TS;
+            v.qsnap = store ? S.q16 + (size_t)c * nint : nullptr;
+            v.qexp = store ? S.qexp + (size_t)c * S.nz * S.nqb : nullptr;
+        } else if (!Q16) {
             v.snap = store ? S.snaps + (size_t)(n / S.TS) * nint : nullptr;
         } else {
             v.smax = store ? S.smax + (size_t)(n / S.TS) * kSnapWords : nullptr;
@@ -349,7 +390,11 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
         const int tau = S.nt - 1 - n;
         v.seis_row = S.seis_t + (size_t)tau * S.nrec;
         const bool img = (S.flags & F_IMAGE) && tau % S.TS == 0;
-        if (!Q16) {
+        if (I16) {
+            const int c = tau / S.TS;
+            v.qsnap = img ? S.q16 + (size_t)c * nint : nullptr;
+            v.qexp = img ? S.qexp + (size_t)c * S.nz * S.nqb : nullptr;
+        } else if (!Q16) {
             /* both fields assigned from selected values: a store through a
              * selected field address would put the whole view in scratch */
             const bool ring = (S.flags & F_RING) != 0;
@@ -369,7 +414,9 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 template <typename R>
 RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
 {
-    return S.qmode ? step_view_q<R, true>(S, n) : step_view_q<R, false>(S, n);
+    return S.qmode == Q_I16 ? step_view_q<R, false, true>(S, n)
+           : S.qmode        ? step_view_q<R, true>(S, n)
+                            : step_view_q<R, false>(S, n);
 }
 
 template <typename R>
@@ -416,11 +463,12 @@ struct TileRegs {
     R Icv[RZ], Iscv[RZ];        /* Kahan compensations                  */
     int recr;                   /* receiver at the own column, or -1    */
     R recv;                     /* its trace sample (backward)          */
+    R uq[RZ];                   /* int16 block store: U at own image points, else 0 */
 };
 
 /* Phase 1a: every global load of the step.  tx in [0,64), ty in [0,4) (the
  * wavefront index: uniform within a wavefront).  CLS is the tile class. */
-template <typename R, int RZ, int CLS, bool Q16 = false>
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false>
 RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
                        int tx, int ty)
 {
@@ -482,13 +530,15 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
     }
-    if (!V.forward && (Q16 ? V.qsnap != nullptr : (V.snap || V.sfield))) {
+    if (!V.forward && ((Q16 || I16) ? V.qsnap != nullptr : (V.snap || V.sfield))) {
         const int ii = clampi(i - S.TL, 0, S.nx - 1);
+        const int qb = clampi(i16_block(x0) - S.qb0, 0, S.nqb - 1);   /* the tile row's block (I16) */
 #pragma unroll
         for (int rr = 0; rr < RZ; rr++) {
             const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
             const int q = kk * S.nx + ii;
-            if (Q16) G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            if (I16) G.sv[rr] = R(dequant_i16((short)ldg(V.qsnap, q), ldg(V.qexp, kk * S.nqb + qb)));
+            else if (Q16) G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
             else if (V.sfield) G.sv[rr] = ldg(V.sfield, clampi(z0 + ty * RZ + rr, 0, S.Z - 1) * S.P + ic);
             else G.sv[rr] = ldg(V.snap, q);
             G.Iv[rr] = ldg(S.I, q);
@@ -569,8 +619,8 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-template <typename R, int RZ, int CLS, bool Q16 = false>
-RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S,
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false>
+RTM_HD float tile_points(const TileLds<R, RZ> &L, TileRegs<R, RZ> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4;
@@ -585,6 +635,10 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
     for (int t = 0; t < RZ + 8; t++) col[t] = L.u[ty * RZ + 4 + t][lc];
     const bool inL = i < S.TL - kHalf, inR = i >= S.X - S.TR + kHalf;
     const bool img_col = i - S.TL >= 0 && i - S.TL < S.nx;
+    if (I16) {
+#pragma unroll
+        for (int rr = 0; rr < RZ; rr++) G.uq[rr] = R(0);
+    }
 #pragma unroll
     for (int rr = 0; rr < RZ; rr++) {
         const int kl = ty * RZ + rr;         /* tile-relative row (uniform) */
@@ -644,11 +698,14 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
             const bool img_pt = img_col && kk >= 0 && kk < S.nz;
-            if (Q16 && V.forward) {
+            if (I16 && V.forward) {
+                /* the int16 block store follows the update (tile_store_i16) */
+                if (img_pt) G.uq[rr] = U;
+            } else if (Q16 && V.forward) {
                 /* the snapshot max, kept on every step (the caller commits it on
                  * store steps only); the 16-bit store of u^n follows the loop */
                 if (img_pt) umax = fmaxf(umax, fabsf(float(U)));
-            } else if ((Q16 ? V.qsnap != nullptr : (V.snap || V.sfield)) && img_pt) {
+            } else if (((Q16 || I16) ? V.qsnap != nullptr : (V.snap || V.sfield)) && img_pt) {
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
                     if (V.snap) stg_stream(V.snap, q, U);
@@ -681,6 +738,29 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, cons
         }
     }
     return umax;
+}
+
+/* Forward, int16 block store: the thread's image points of this step's
+ * snapshot.  rmax[rr] is the max |uq[rr]| over the wavefront (one block:
+ * the tile row's 64 columns); lane 0 writes the block's exponent. */
+template <typename R, int RZ>
+RTM_HD void tile_store_i16(const TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, const float *rmax,
+                           int x0, int z0, int tx, int ty)
+{
+    const int i = x0 + tx;
+    const bool img_col = i - S.TL >= 0 && i - S.TL < S.nx && i < S.X - kHalf;
+    const bool blk_img = x0 + kQBlock > S.TL && x0 < S.TL + S.nx;   /* the block holds image columns */
+    const int qb = i16_block(x0) - S.qb0;
+#pragma unroll
+    for (int rr = 0; rr < RZ; rr++) {
+        const int k = z0 + ty * RZ + rr, kk = k - S.TU;
+        if (k >= S.Z - kHalf) break;
+        if (kk < 0 || kk >= S.nz) continue;   /* uniform */
+        const int e = i16_exp(rmax[rr]);
+        if (img_col)
+            stg_stream(V.qsnap, kk * S.nx + (i - S.TL), (unsigned short)quant_i16(float(G.uq[rr]), pow2f(15 - e)));
+        if (tx == 0 && blk_img) stg(V.qexp, kk * S.nqb + qb, (signed char)e);
+    }
 }
 
 }  // namespace rtm

@@ -165,6 +165,37 @@ def quantize_snapshots(S, mode: int):
     return deq, codes, mx
 
 
+def quantize_i16_blocks(S, TL: int):
+    """The int16 block store's rule, restated independently: in every image
+    row (z) of snapshot c, the image columns are grouped by the block
+    (TL + ii - 4) // 64 of their padded column; a block with max |S| =
+    f 2^E (0.5 <= f < 1, E from the float32 bits, clamped to [-100, 100])
+    stores rint(S 2^(15-E)) (nearest even, clipped to +-32767) as int16 and
+    E, and reads back code 2^(E-15).  S is the oracle's [c][nx][nz]; returns
+    the read-back float32 snapshots [c][nx][nz], the codes [c][nz][nx]
+    (uint16 bits) and the exponents [c][nz][blocks] (int8)."""
+    S = np.asarray(S, np.float32)
+    nc, nx, nz = S.shape
+    blk = (TL + np.arange(nx) - 4) // 64
+    b0 = blk[0]
+    nb = blk[-1] - b0 + 1
+    deq = np.empty_like(S)
+    codes = np.empty((nc, nz, nx), np.int16)
+    exps = np.empty((nc, nz, nb), np.int8)
+    for b in range(nb):
+        cols = np.nonzero(blk == b0 + b)[0]
+        seg = S[:, cols, :]                                       # [c][cols][nz]
+        m = np.abs(seg).max(axis=1).astype(np.float32)            # [c][nz]
+        e = ((m.view(np.uint32) >> 23) & 0xFF).astype(np.int32) - 126
+        e = np.clip(e, -100, 100)
+        scale = np.exp2(15 - e).astype(np.float32)[:, None, :]
+        q = np.clip(np.rint((seg * scale).astype(np.float32)), -32767, 32767).astype(np.int16)
+        deq[:, cols, :] = (q.astype(np.float32) * np.exp2(e - 15).astype(np.float32)[:, None, :])
+        codes[:, :, cols] = q.transpose(0, 2, 1)
+        exps[:, :, b] = e
+    return deq, codes.view(np.uint16), exps
+
+
 # ---- boundary saving: the reconstructed source field, from the oracle ----
 
 _LAP_W = np.float32([-2.97399944, 1.70507669, -0.25861812, 0.04577745, -0.00523630])

@@ -2,6 +2,7 @@
 // CPU so the per-point fused update can be checked against the oracle
 // without a GPU.  Built by tests/test_cpu_harness.py with
 // g++ -O2 -ffp-contract=off; test infrastructure only.
+#include <cmath>
 #include <cstddef>
 #include <vector>
 
@@ -42,13 +43,13 @@ static void commit_max(const rtm::StepView<R> &V, float m)
 
 // Emulation of k_tiles for one simulation at step n: every tile, phase by
 // phase, all 256 threads of a phase before the next (the barriers).
-template <typename R, int RZ, int CLS, bool Q16>
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false>
 static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm::SimDesc<R> &S,
                      const rtm::StepView<R> &V, int x0, int z0)
 {
     for (int ty = 0; ty < 4; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            rtm::tile_fetch<R, RZ, CLS, Q16>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            rtm::tile_fetch<R, RZ, CLS, Q16, I16>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             rtm::tile_stage<R, RZ, CLS>(L, G[ty * 64 + tx], tx, ty);
         }
     if (CLS)
@@ -57,10 +58,24 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
     float m = 0.0f;
     for (int ty = 0; ty < 4; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            const float t = rtm::tile_points<R, RZ, CLS, Q16>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            const float t = rtm::tile_points<R, RZ, CLS, Q16, I16>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             m = t > m ? t : m;
         }
     commit_max(V, m);
+    if (I16 && V.forward && V.qsnap) {
+        // the kernel's wavefront max of each row's |U| (the block), then the store
+        for (int ty = 0; ty < 4; ty++) {
+            float rmax[RZ];
+            for (int rr = 0; rr < RZ; rr++) {
+                rmax[rr] = 0.0f;
+                for (int tx = 0; tx < 64; tx++) {
+                    const float a = std::fabs(float(G[ty * 64 + tx].uq[rr]));
+                    rmax[rr] = a > rmax[rr] ? a : rmax[rr];
+                }
+            }
+            for (int tx = 0; tx < 64; tx++) rtm::tile_store_i16<R, RZ>(G[ty * 64 + tx], S, V, rmax, x0, z0, tx, ty);
+        }
+    }
 }
 
 // Emulation of k_columns for one item: every row, all 64 lanes, with the
@@ -108,7 +123,7 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
 
 // One step of one simulation over the work items of the library's builder
 // (rtm::build_work): tiles, plus column interior items.
-template <typename R, int RZ, bool Q16>
+template <typename R, int RZ, bool Q16, bool I16 = false>
 static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
 {
     const rtm::StepView<R> V = rtm::step_view(S, n);
@@ -131,10 +146,10 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
             continue;
         }
         switch (t.d) {
-        case 0: run_tile<R, RZ, 0, Q16>(*L, G, S, V, t.b, t.c); break;
-        case 1: run_tile<R, RZ, 1, Q16>(*L, G, S, V, t.b, t.c); break;
-        case 2: run_tile<R, RZ, 2, Q16>(*L, G, S, V, t.b, t.c); break;
-        default: run_tile<R, RZ, 3, Q16>(*L, G, S, V, t.b, t.c); break;
+        case 0: run_tile<R, RZ, 0, Q16, I16>(*L, G, S, V, t.b, t.c); break;
+        case 1: run_tile<R, RZ, 1, Q16, I16>(*L, G, S, V, t.b, t.c); break;
+        case 2: run_tile<R, RZ, 2, Q16, I16>(*L, G, S, V, t.b, t.c); break;
+        default: run_tile<R, RZ, 3, Q16, I16>(*L, G, S, V, t.b, t.c); break;
         }
     }
     for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
@@ -147,7 +162,8 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
 template <typename R, int RZ>
 static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
 {
-    if (S.qmode) run_tiles_q<R, RZ, true>(S, n, interior, seg_rows);
+    if (S.qmode == rtm::Q_I16) run_tiles_q<R, RZ, false, true>(S, n, interior, seg_rows);
+    else if (S.qmode) run_tiles_q<R, RZ, true>(S, n, interior, seg_rows);
     else run_tiles_q<R, RZ, false>(S, n, interior, seg_rows);
 }
 
@@ -164,6 +180,12 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     S.qmode = g[21];
     S.q16 = (unsigned short *)(void *)ptr[18];
     S.smax = (unsigned *)(void *)ptr[19];
+    if (S.qmode == rtm::Q_I16) {   // the int16 block store: its exponents in the max-slot entry
+        S.smax = nullptr;
+        S.qexp = (signed char *)(void *)ptr[19];
+        S.qb0 = rtm::i16_first_block(S.TL);
+        S.nqb = rtm::i16_blocks(S.TL, S.nx);
+    }
     S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
     S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
     S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];

@@ -19,6 +19,12 @@ import oracle as O
 
 M_STENCIL, M_CPML, M_SOURCE, M_INJECT, M_SNAP, M_IMAGE, M_KAHAN = 1, 2, 4, 8, 16, 32, 64
 SNAP_WORDS = 32 * 64   # max-abs slots per snapshot (rtm::kSnapWords)
+Q_I16 = 3              # rtm::Q_I16, the int16 block store
+
+
+def i16_blocks(TL, nx):
+    """Blocks of 64 padded columns (from column 4) holding image columns."""
+    return ((TL + nx - 1 - 4) >> 6) - ((TL - 4) >> 6) + 1
 
 
 def build_harness(outdir: str) -> C.CDLL:
@@ -161,7 +167,7 @@ class DevSim:
                           self.rec_row, n, rz, seg, interior, qmode], np.int32)
             fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
         self.cur = op.nt & 1
-        if qmode and phase == 1 and (op.nt - 1) % TS == 0:
+        if qmode and qmode != Q_I16 and phase == 1 and (op.nt - 1) % TS == 0:
             c = (op.nt - 1) // TS
             qs = q16[c]
             self.hx.hx_quantize_interior(_p(self.u[op.nt & 1]), C.c_int(self.P), C.c_int(op.t_left),
@@ -170,14 +176,18 @@ class DevSim:
         return snaps, I, Is, seis_rec
 
     def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0, qmode=0):
-        """Forward pass; with qmode (1 FP16, 2 BF16) the snapshots are the
-        16-bit store and its max-abs slots: returns (q16, smax), rec."""
+        """Forward pass; with qmode (1 FP16, 2 BF16, 3 int16 blocks) the
+        snapshots are the 16-bit store and its max-abs slots (for int16
+        blocks, the block exponents): returns (q16, smax), rec."""
         op = self.op
         src_j = (shot_z + op.t_up) * self.P + shot_x + op.t_left
         flags = 4 | (2 if record else 0)
         TS = op.time_slices
         q16 = np.zeros((op.nt // TS, op.nz, op.nx), np.uint16)
-        smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
+        if qmode == Q_I16:      # the block exponents take the max-slot entry of the table
+            smax = np.zeros((op.nt // TS, op.nz, i16_blocks(op.t_left, op.nx)), np.int8)
+        else:
+            smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
         snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record,
                                             interior=interior, seg=seg, qmode=qmode, q16=q16, smax=smax)
         if qmode:

@@ -190,6 +190,62 @@ def test_sixteen_bit_store_bitexact(hx, qmode, ts, interior):
 
 
 
+@pytest.mark.parametrize("ts", [1, 4])
+@pytest.mark.parametrize("rz", [1, 2, 4, 8])
+@pytest.mark.parametrize("name", ["wide", "asym"])
+def test_int16_block_store_bitexact(hx, name, rz, ts):
+    """int16 block store (a power-of-two scale per image row and block of 64
+    padded columns, written in the step that computes the snapshot): codes
+    and exponents equal the numpy restatement of the rule applied to the
+    oracle's FP32 snapshots, and the image equals the oracle's backward pass
+    over the read-back snapshots; on grids whose image starts at different
+    offsets within a block (TL = 20, 14)."""
+    cfg, g = _case(name)
+    op = common.orc_params(cfg, time_slices=ts)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=np.float32, record=False)
+    deq, codes, exps = common.quantize_i16_blocks(S, op.t_left)
+    ds = DL.DevSim(hx, op, vpad, np.float32)
+    (q16, qexp), _ = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, qmode=DL.Q_I16)
+    assert np.array_equal(qexp, exps)
+    assert np.array_equal(q16, codes)
+    d = g["seismograms"][0]
+    I_ref, Is_ref, _, _ = O.backward(op, vpad, d, deq)
+    I2, Is2 = ds.backward_tiled(d, (q16, qexp), rz=rz, qmode=DL.Q_I16)
+    assert np.array_equal(I2, I_ref) and np.array_equal(Is2, Is_ref)
+
+
+def test_int16_block_rule_error_bound_and_exact_values():
+    """The restated rule against what the arithmetic fixes: every value
+    reads back within half a code step, 2^(E-16), of itself, and so within
+    2^-16 of its block's max; a block's max lands on a code of magnitude in
+    [16384, 32767]; powers of two and small integers times a power of two
+    come back exactly; an all-zero block stays zero."""
+    rng = np.random.default_rng(3)
+    S = (rng.standard_normal((3, 150, 7)) * np.exp2(rng.integers(-30, 30, (3, 1, 7)))).astype(np.float32)
+    S[1, 40:110, 2] = 0.0                      # a zero run, inside one block
+    S[2, :, 5] = np.exp2(np.arange(150) % 9 - 4).astype(np.float32)
+    TL = 20
+    deq, codes, exps = common.quantize_i16_blocks(S, TL)
+    blk = (TL + np.arange(150) - 4) // 64
+    for c in range(3):
+        for k in range(7):
+            for b in np.unique(blk):
+                cols = blk == b
+                seg, back = S[c, cols, k].astype(np.float64), deq[c, cols, k].astype(np.float64)
+                m = np.abs(seg).max()
+                e = int(exps[c, k, b - blk[0]])
+                assert np.all(np.abs(back - seg) <= 2.0 ** (e - 16) * (1 + 1e-12))
+                if m > 0:
+                    assert 0.5 <= m / 2.0 ** e < 1.0
+                    top = np.abs(codes[c, k][cols].view(np.int16)).max()
+                    assert 16384 <= top <= 32767
+                else:
+                    assert np.all(back == 0)
+    assert np.array_equal(deq[2, :, 5], S[2, :, 5])
+
+
 def test_boundary_saving_reconstruction_close_to_snapshots():
     """Boundary saving reconstructs the source field backward in time inside
     the image region from the final two fields and the saved ring: its

@@ -471,6 +471,86 @@ def test_batch_sixteen_bit_snapshots_bitexact(gpu_available):
             assert np.array_equal(I2, I) and np.array_equal(Is2, Is), k
 
 
+@pytest.mark.parametrize("ts,rz", [(1, 2), (4, 4), (5, 1), (4, 8)])
+def test_int16_block_snapshots_bitexact(gpu_available, ts, rz):
+    """int16 block store (a power-of-two scale per image row and 64-column
+    block, stored in the step that computes the snapshot): the image equals
+    the oracle's backward pass over the oracle's FP32 snapshots put through
+    the store's rule (numpy restatement), bit for bit."""
+    say(f"[gpu] int16 block store TS={ts} rz={rz}")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=ts)
+    p = common.rtm_params(cfg, time_slices=ts, snapshot_mode=R.SNAP_I16)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+    deq, _, _ = common.quantize_i16_blocks(S, op.t_left)
+    I, Is, _, _ = O.backward(op, vpad, seis, deq)
+    with R.Plan(p) as plan:
+        plan.set_tiling(rz, 100, 0, 0)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I2, Is2 = plan.image()
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
+def test_int16_block_store_refuses_non_tile_shapes_and_tunes_tiles_only(gpu_available):
+    """The int16 block store runs in tile shapes: a column or mixed interior
+    is refused with the reason and the current shape kept; the autotuner
+    times tile shapes only and keeps results exact."""
+    say("[gpu] int16 block store: shapes")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=R.SNAP_I16)
+    vpad = O.extend(op, model)
+    sx, sz = common.shot_positions(cfg)[0]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+    I, _, _, _ = O.backward(op, vpad, seis, common.quantize_i16_blocks(S, op.t_left)[0])
+    with R.Plan(p) as plan:
+        plan.set_tiling(2, 100, 0, 0)
+        for interior in (1, 2):
+            with pytest.raises(R.RtmError, match="tile shapes only"):
+                plan.set_tiling(4, 100, interior, 16)
+        assert plan.tiling()[2] == 0
+        tuned = plan.autotune(20)
+        assert tuned and all(t["interior"] == 0 for t in tuned)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        assert np.array_equal(plan.image()[0], I)
+
+
+def test_batch_int16_block_snapshots_bitexact(gpu_available):
+    """A batch with the int16 block store, in launch groups of 2 (a ragged
+    last group): each realization's image equals the oracle's over its own
+    quantized snapshots."""
+    say("[gpu] batch int16 block store")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=4)
+    p = common.rtm_params(cfg, time_slices=4, snapshot_mode=R.SNAP_I16)
+    sx, sz = common.shot_positions(cfg)[0]
+    vpads = [O.extend(op, model * np.float32(1 + 0.05 * k)) for k in range(3)]
+    with R.Batch(p, len(vpads)) as b:
+        b.set_group(2)
+        b.set_tiling(4, 100, 0, 0)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(wav)
+        b.set_seismogram(seis)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k, v in enumerate(vpads):
+            S, _, _, _ = O.forward(op, sx, sz, v, wav, record=False)
+            I, Is, _, _ = O.backward(op, v, seis, common.quantize_i16_blocks(S, op.t_left)[0])
+            I2, Is2 = b.image(k)
+            assert np.array_equal(I2, I) and np.array_equal(Is2, Is), k
+
+
 def test_injection_outside_stepped_region(gpu_available):
     """Source and receivers on row 2 of the padded grid (outside the stepped
     region 4 <= k < Z-4, inside the top CPML): the per-point path with the
@@ -895,6 +975,40 @@ def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape, grou
 def _sub(req):
     cfg, op, p, vpad, wav, seis, sx, sz = req
     return p, vpad, wav, seis, sx, sz
+
+
+@pytest.mark.parametrize("group_bytes", [None, 1])
+def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group_bytes):
+    """The request server with the int16 block store: requests of three grid
+    sizes arriving while others are in flight, each image equal to the
+    oracle's for that request alone under the store's rule, in one launch
+    group and with every request in a group of its own."""
+    say(f"[gpu] request server, int16 block store, group_bytes {group_bytes}")
+    reqs = _server_requests()
+    refs = []
+    for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
+        S, _, _, _ = O.forward(op, sx, sz, vpad, wav, record=False)
+        refs.append(O.backward(op, vpad, seis, common.quantize_i16_blocks(S, op.t_left)[0])[0])
+    fp = [R.request_footprint(r[2]) for r in reqs]
+    budget = int(sorted(fp)[-1] + sorted(fp)[-2] + (1 << 20))
+    got = {}
+    with R.Server(budget, max_slots=4, snapshot_mode=R.SNAP_I16) as srv:
+        srv.set_tiling(4, 50, 0, 0)
+        with pytest.raises(R.RtmError, match="tile shapes only"):
+            srv.set_tiling(4, 50, 1, 16)
+        if group_bytes is not None:
+            srv.set_group_bytes(group_bytes)
+        ids = [srv.submit(*_sub(reqs[0])), srv.submit(*_sub(reqs[1]))]
+        srv.tick()
+        ids += [srv.submit(*_sub(r)) for r in reqs[2:]]
+        while srv.tick() > 0:
+            for rid, I in srv.done():
+                got[rid] = I
+        for rid, I in srv.done():
+            got[rid] = I
+    assert sorted(got) == sorted(ids)
+    for k, rid in enumerate(ids):
+        assert np.array_equal(got[rid], refs[k]), k
 
 
 def test_server_fp32_store_matches_the_oracle(gpu_available):

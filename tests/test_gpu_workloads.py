@@ -217,6 +217,29 @@ def test_c3_sixteen_bit_store_on_c1_grid(gpu_available, mode):
     assert np.array_equal(Is, Is_ref)
 
 
+def test_c3_int16_block_store_on_c1_grid(gpu_available):
+    """The int16 block store on the c1 grid at TS = 4 (the benchmark's shape
+    and Kahan image), 400 steps: the image equals the oracle's Kahan
+    backward pass over the oracle's FP32 snapshots put through the store's
+    rule."""
+    nt = 400
+    say(f"[gpu] c3 int16 block store on the c1 grid, TS=4, {nt} steps, Kahan image")
+    p, op, vpad, w, d, S = _c3_oracle(nt)
+    deq, _, _ = common.quantize_i16_blocks(S, op.t_left)
+    I_ref, Is_ref, _, _ = O.backward(op, vpad, d, deq)
+    q = p.copy().update(snapshot_mode=R.SNAP_I16)
+    with R.Plan(q) as plan:
+        plan.set_tiling(*C1_SHAPE)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(w)
+        plan.set_seismogram(d)
+        plan.zero_image()
+        plan.migrate(p.first_location_shot, p.depth_shot)
+        I, Is = plan.image()
+    assert np.array_equal(I, I_ref), f"relL2 {common.rel_l2(I, I_ref)}"
+    assert np.array_equal(Is, Is_ref)
+
+
 def test_c3_fp32_store_kahan_on_c1_grid(gpu_available):
     """c3's FP32 store with the Kahan image on the c1 grid at TS = 4: the
     image equals the oracle's compensated image bit for bit."""
