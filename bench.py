@@ -67,10 +67,11 @@ def full_step_bytes(p, ts):
 
 def tiling_dict(shape):
     """(tile rows per thread, graph block, interior kernel 0 tiles / 1
-    columns / 2 mixed tiles, rows per interior segment) as a dict."""
+    columns / 2 mixed tiles / 3 tiles of 8 wavefronts, rows per interior
+    segment) as a dict."""
     rz, block, interior, seg = shape
     return {"rows_per_thread": rz, "graph_block": block,
-            "interior": ("tiles", "columns", "mixed")[interior], "seg_rows": seg}
+            "interior": ("tiles", "columns", "mixed", "tiles of 8 wavefronts")[interior], "seg_rows": seg}
 
 
 def log(*a):

@@ -21,7 +21,7 @@ PATH_LEN = 512
 
 SNAP_FP32, SNAP_FP16, SNAP_BF16, SNAP_BOUNDARY, SNAP_I16 = 0, 1, 2, 3, 4
 REAL_F32, REAL_F64 = 0, 1
-INTERIOR_TILES, INTERIOR_COLUMNS, INTERIOR_MIXED = 0, 1, 2
+INTERIOR_TILES, INTERIOR_COLUMNS, INTERIOR_MIXED, INTERIOR_TILES8 = 0, 1, 2, 3
 
 
 class ZfpMode(C.Structure):

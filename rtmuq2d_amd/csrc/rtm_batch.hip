@@ -165,7 +165,7 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
         std::vector<int4> tiles, inner;
         for (int s = 0; s < n; s++)
             append_work(tiles, inner, s, q->sims[s0 + s].p, rz, interior, seg_rows);
-        balance_tiles_for_xcds(tiles, 4 * rz);
+        balance_tiles_for_xcds(tiles, rtm::tile_rows(rz, interior));
         Exec &ex = q->ex[k];
         if (ex.init(q->real, n, (int)tiles.size(), rz, (int)inner.size(), interior)) return fail();
         ex.q16 = q->sims[0].qmode == rtm::Q_FP16 || q->sims[0].qmode == rtm::Q_BF16;

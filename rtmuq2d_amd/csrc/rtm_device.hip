@@ -87,7 +87,7 @@ static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, i
     if (check_shape(rz, interior) || check_store_shape(q->b.p.snapshot_mode, interior)) return -1;
     std::vector<int4> tiles, inner;
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
-    balance_tiles_for_xcds(tiles, 4 * rz);
+    balance_tiles_for_xcds(tiles, rtm::tile_rows(rz, interior));
     q->ex.release();
     if (q->ex.init(q->b.real, 1, (int)tiles.size(), rz, (int)inner.size(), interior)) return -1;
     q->ex.q16 = q->b.qmode == rtm::Q_FP16 || q->b.qmode == rtm::Q_BF16;

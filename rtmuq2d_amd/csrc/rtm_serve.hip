@@ -118,7 +118,7 @@ struct HipBackend {
         while ((int)ex.size() <= g) ex.emplace_back(new Exec());
         std::vector<int4> tiles, inner;
         for (const SReq *r : members) append_work(tiles, inner, r->slot, r->p, rz, interior, seg_rows);
-        balance_tiles_for_xcds(tiles, 4 * rz);
+        balance_tiles_for_xcds(tiles, rtm::tile_rows(rz, interior));
         Exec &e = *ex[g];
         if (!e.d_desc || (int)tiles.size() > e.cap_tiles || (int)inner.size() > e.cap_inner || e.rz != rz ||
             e.interior != interior) {

@@ -66,51 +66,54 @@ __device__ __forceinline__ void commit_snap_max(unsigned *slots, float m, int wh
     if ((threadIdx.x & 63) == 0) atomicMax(slots + (who % rtm::kSnapSlots) * rtm::kSlotStride, __float_as_uint(m));
 }
 
-// the same for a 256-thread workgroup: one atomic per workgroup
+// the same for a workgroup of NW wavefronts: one atomic per workgroup
+template <int NW = 4>
 __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, float *red)
 {
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float t = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float t = red[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) t = fmaxf(t, red[w]);
         atomicMax(slots + (blockIdx.x % rtm::kSnapSlots) * rtm::kSlotStride, __float_as_uint(t));
     }
 }
 
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false>
-__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n, int x0,
-                                         int z0, int tx, int ty)
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4>
+__device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
+                                         int x0, int z0, int tx, int ty)
 {
     const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16>(S, n);
     if (!V.active) return;
-    rtm::TileRegs<R, RZ> G;
-    rtm::tile_fetch<R, RZ, CLS, Q16, I16>(G, S, V, x0, z0, tx, ty);
-    rtm::tile_stage<R, RZ, CLS>(L, G, tx, ty);
+    rtm::TileRegs<R, RZ, NW> G;
+    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_stage<R, RZ, CLS, NW>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
-        rtm::tile_psi<R, RZ, CLS>(L, G, S, x0, z0, tx, ty);
+        rtm::tile_psi<R, RZ, CLS, NW>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16>(L, G, S, V, x0, z0, tx, ty);
-    if (Q16 && V.smax) commit_snap_max_wg(V.smax, m, red);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW>(L, G, S, V, x0, z0, tx, ty);
+    if (Q16 && V.smax) commit_snap_max_wg<NW>(V.smax, m, red);
     if (I16 && V.forward && V.qsnap) {   /* uniform: a store step */
         float rmax[RZ];
 #pragma unroll
         for (int rr = 0; rr < RZ; rr++) rmax[rr] = wave_max(fabsf(float(G.uq[rr])));   /* the row's block */
-        rtm::tile_store_i16<R, RZ>(G, S, V, rmax, x0, z0, tx, ty);
+        rtm::tile_store_i16<R, RZ, NW>(G, S, V, rmax, x0, z0, tx, ty);
     }
 }
 
-template <typename R, int RZ, bool Q16, bool I16 = false>
-__device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red, const SimDesc<R> &S, int n,
+template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4>
+__device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                              int4 tile, int tx, int ty)
 {
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3, Q16, I16>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 0: run_tile<R, RZ, 0, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
     }
 }
 
@@ -121,14 +124,15 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ> &L, float *red,
 // address is known before the tile entry arrives and the descriptor loads
 // need not wait for it.
 // I16: every sim of the launch keeps an int16 block store (FP32 stepping).
-template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false>
-__global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ sims,
-                                               const int *__restrict__ n0,
-                                               const int4 *__restrict__ tiles,
-                                               const int *__restrict__ ntiles_p, int s)
+// NW: wavefronts per workgroup, one row group each (tiles of 64 x NW*RZ).
+template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict__ sims,
+                                                   const int *__restrict__ n0,
+                                                   const int4 *__restrict__ tiles,
+                                                   const int *__restrict__ ntiles_p, int s)
 {
-    __shared__ rtm::TileLds<R, RZ> L;
-    __shared__ float red[4];
+    __shared__ rtm::TileLds<R, RZ, NW> L;
+    __shared__ float red[NW];
     const int ntiles = *ntiles_p;
     const int per = (ntiles + 7) >> 3;
     const int b = blockIdx.x;
@@ -143,7 +147,7 @@ __global__ void __launch_bounds__(256) k_tiles(const SimDesc<R> *__restrict__ si
     const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    run_tile_cls<R, RZ, Q16, I16>(L, red, S, n, tile, tx, ty);
+    run_tile_cls<R, RZ, Q16, I16, NW>(L, red, S, n, tile, tx, ty);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run

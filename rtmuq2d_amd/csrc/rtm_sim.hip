@@ -500,12 +500,15 @@ int check_shape(int rz, int interior)
         rtm_set_error("tile rows per thread must be 1, 2, 4 or 8 (got %d)", rz);
         return -1;
     }
-    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED) {
-        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns) or 2 (mixed tiles) (got %d)", interior);
+    if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED &&
+        interior != rtm::IN_TILES8) {
+        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns), 2 (mixed tiles) or 3 (tiles of 8 "
+                      "wavefronts) (got %d)", interior);
         return -1;
     }
-    if (interior == rtm::IN_MIXED && rz != 2 && rz != 4) {
-        rtm_set_error("mixed tiles take 2 or 4 rows per thread for the interior tiles (got %d)", rz);
+    if ((interior == rtm::IN_MIXED || interior == rtm::IN_TILES8) && rz != 2 && rz != 4) {
+        rtm_set_error("%s take 2 or 4 rows per thread (got %d)",
+                      interior == rtm::IN_MIXED ? "mixed tiles" : "tiles of 8 wavefronts", rz);
         return -1;
     }
     return 0;
@@ -646,6 +649,16 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
             hipLaunchKernelGGL((k_tiles_mix<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
         else
             hipLaunchKernelGGL((k_tiles_mix<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+    } else if (e.ntiles && e.interior == rtm::IN_TILES8) {
+        const int grid = 8 * ((e.cap_tiles + 7) / 8);
+        const bool one = e.cap_sims == 1;
+        if (e.rz == 2) {
+            if (one) hipLaunchKernelGGL((k_tiles<R, 2, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            else hipLaunchKernelGGL((k_tiles<R, 2, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+        } else {
+            if (one) hipLaunchKernelGGL((k_tiles<R, 4, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            else hipLaunchKernelGGL((k_tiles<R, 4, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+        }
     } else if (e.ntiles && e.cap_sims == 1) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
@@ -686,6 +699,17 @@ static void launch_tiles_i16(const Exec &e, int s_in_block, hipStream_t s)
     if (!e.ntiles) return;
     const SimDesc<float> *d = (const SimDesc<float> *)e.d_desc;
     const int grid = 8 * ((e.cap_tiles + 7) / 8);
+    if (e.interior == rtm::IN_TILES8) {
+        const bool one = e.cap_sims == 1;
+        if (e.rz == 2) {
+            if (one) hipLaunchKernelGGL((k_tiles<float, 2, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            else hipLaunchKernelGGL((k_tiles<float, 2, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+        } else {
+            if (one) hipLaunchKernelGGL((k_tiles<float, 4, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            else hipLaunchKernelGGL((k_tiles<float, 4, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+        }
+        return;
+    }
     if (e.cap_sims == 1) {
         switch (e.rz) {
         case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
@@ -705,8 +729,9 @@ static void launch_tiles_i16(const Exec &e, int s_in_block, hipStream_t s)
 
 int check_store_shape(int snapshot_mode, int interior)
 {
-    if (snapshot_mode == RTM_SNAP_I16 && interior != rtm::IN_TILES) {
-        rtm_set_error("the int16 block snapshot store runs in tile shapes only (interior kernel 0, got %d)", interior);
+    if (snapshot_mode == RTM_SNAP_I16 && interior != rtm::IN_TILES && interior != rtm::IN_TILES8) {
+        rtm_set_error("the int16 block snapshot store runs in tile shapes only (interior kernel 0 or 3, got %d)",
+                      interior);
         return -1;
     }
     return 0;
@@ -785,6 +810,7 @@ const std::vector<Shape> &tuner_candidates()
         {1, rtm::IN_TILES, 0}, {2, rtm::IN_TILES, 0}, {4, rtm::IN_TILES, 0}, {8, rtm::IN_TILES, 0},
         {4, rtm::IN_COLUMNS, 0}, {4, rtm::IN_COLUMNS, 64}, {2, rtm::IN_COLUMNS, 32}, {1, rtm::IN_COLUMNS, 8},
         {4, rtm::IN_MIXED, 0}, {2, rtm::IN_MIXED, 0},
+        {2, rtm::IN_TILES8, 0}, {4, rtm::IN_TILES8, 0},
     };
     return c;
 }

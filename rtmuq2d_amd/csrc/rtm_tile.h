@@ -6,9 +6,10 @@
  * A launch advances any number of simulations ("sims") by one time step.
  * Each sim is described by a SimDesc in device memory (its grid may differ
  * from the others'), and the launch walks a flat list of tiles
- * {sim, x0, z0, class}.  A tile covers 64 x TZ points (TZ = 4*RZ) of the
- * stepped region 4 <= i < X-4, 4 <= k < Z-4; 256 threads = 64 columns x 4
- * row groups, each thread owning RZ consecutive rows of one column.
+ * {sim, x0, z0, class}.  A tile covers 64 x TZ points (TZ = NW*RZ) of the
+ * stepped region 4 <= i < X-4, 4 <= k < Z-4; 64*NW threads = 64 columns x
+ * NW row groups (NW = 4 or 8 wavefronts), each thread owning RZ consecutive
+ * rows of one column.
  *
  * Phase 1  load u^n over the tile plus a halo into LDS: radius 4, or radius
  *          8 across a CPML strip (the strip's psi is needed 4 points out,
@@ -318,9 +319,9 @@ RTM_HD unsigned snap_max_bits(const unsigned *slots)
 #endif
 }
 
-template <typename R, int RZ>
+template <typename R, int RZ, int NW = 4>
 struct TileLds {
-    static constexpr int TZ = 4 * RZ;
+    static constexpr int TZ = NW * RZ;
     R u[TZ + 16][80];       /* rows z0-8 .. z0+TZ+7, columns x0-8 .. x0+71 */
     R px[TZ][72];           /* psiX^{n+1}: rows z0 .., columns x0-4 .. x0+67 */
     R pz[TZ + 8][64];       /* psiZ^{n+1}: rows z0-4 .., columns x0 .. x0+63 */
@@ -450,13 +451,14 @@ RTM_HD int tile_class(const SimDesc<R> &S, int x0, int z0, int TZ)
  * need.  Every load uses a clamped (always valid) address and its value is
  * selected afterwards: no load is predicated, which keeps the scalar unit
  * free of exec-mask bookkeeping. */
-template <typename R, int RZ>
+template <typename R, int RZ, int NW = 4>
 struct TileRegs {
-    R u0[RZ + 4], u1[RZ + 4];   /* u rows of the tile+halo, 2 column blocks */
+    static_assert(NW == 4 || NW == 8, "a tile has 4 or 8 row groups");
+    R u0[RZ + 16 / NW], u1[RZ + 16 / NW];   /* u rows of the tile+halo, 2 column blocks */
     R C[RZ], Uold[RZ];          /* own points: rows ty*RZ+rr, column tx */
     R csx[RZ], csz[RZ];
     R px0[RZ], px1[RZ];         /* psiX^n at the psi-phase points       */
-    R pz[RZ + 2];               /* psiZ^n at the psi-phase points       */
+    R pz[RZ + 8 / NW];          /* psiZ^n at the psi-phase points       */
     R ax0, bx0, ax1, bx1;       /* x profiles at psi columns tx-4, 64+(tx&7)-4 */
     R axo, bxo;                 /* x profiles at the own column         */
     R sv[RZ], Iv[RZ], Isv[RZ];  /* imaging: snapshot and accumulators   */
@@ -468,20 +470,19 @@ struct TileRegs {
 
 /* Phase 1a: every global load of the step.  tx in [0,64), ty in [0,4) (the
  * wavefront index: uniform within a wavefront).  CLS is the tile class. */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false>
-RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4>
+RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
                        int tx, int ty)
 {
-    constexpr int TZ = 4 * RZ;
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4, HZ = (CLS & T_ZSTRIP) ? 8 : 4;
-    constexpr int NR = RZ + HZ / 2;                  /* (TZ + 2 HZ) / 4 rows per thread */
+    constexpr int NR = RZ + 2 * HZ / NW;             /* (TZ + 2 HZ) / NW rows per thread */
     /* column blocks: 64 columns, then the 2*HX remaining ones (lanes repeat) */
     const int ia = x0 - HX + tx, ib = x0 - HX + 64 + (tx % (2 * HX));
     const bool oka = ia >= 0 && ia < S.X, okb = ib < S.X;
     const int ca = clampi(ia, 0, S.X - 1), cb = clampi(ib, 0, S.X - 1);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
-        const int k = z0 - HZ + ty + 4 * t;
+        const int k = z0 - HZ + ty + NW * t;
         const bool okk = k >= 0 && k < S.Z;
         const int base = clampi(k, 0, S.Z - 1) * S.P;
         const R a = ldg(V.u, base + ca), b = ldg(V.u, base + cb);
@@ -513,7 +514,7 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
         G.bx1 = sb ? b1 : R(0);
 #pragma unroll
         for (int t = 0; t < RZ; t++) {
-            const int k = z0 + ty + 4 * t;
+            const int k = z0 + ty + NW * t;
             const bool kok = k < S.Z - kHalf;
             const int base = clampi(k, 0, S.Z - 1) * S.P;
             const R va = ldg(V.psiXi, base + qa), vb = ldg(V.psiXi, base + qb);
@@ -524,8 +525,8 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
     if (CLS & T_ZSTRIP) {
         const bool iok = i < S.X - kHalf;
 #pragma unroll
-        for (int t = 0; t < RZ + 2; t++) {
-            const int k = z0 - 4 + ty + 4 * t;
+        for (int t = 0; t < RZ + 8 / NW; t++) {
+            const int k = z0 - 4 + ty + NW * t;
             const R v = ldg(V.psiZi, clampi(k, 0, S.Z - 1) * S.P + ic);
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
@@ -562,23 +563,23 @@ RTM_HD void tile_fetch(TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R
 }
 
 /* Phase 1b: the fetched u into LDS (row z0-8+r, column x0-8+c at [r][c]). */
-template <typename R, int RZ, int CLS>
-RTM_HD void tile_stage(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, int tx, int ty)
+template <typename R, int RZ, int CLS, int NW = 4>
+RTM_HD void tile_stage(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, int tx, int ty)
 {
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4, HZ = (CLS & T_ZSTRIP) ? 8 : 4;
-    constexpr int NR = RZ + HZ / 2;
+    constexpr int NR = RZ + 2 * HZ / NW;
     const int cb = 64 + (tx % (2 * HX)) + 8 - HX;   /* lanes beyond 2*HX repeat a column: same value */
 #pragma unroll
     for (int t = 0; t < NR; t++) {
-        const int r = ty + 4 * t + 8 - HZ;
+        const int r = ty + NW * t + 8 - HZ;
         L.u[r][tx + 8 - HX] = G.u0[t];
         L.u[r][cb] = G.u1[t];
     }
 }
 
 /* Phase 2 (strip classes): psi^{n+1} over the tile plus a radius-4 halo. */
-template <typename R, int RZ, int CLS>
-RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<R> &S, int x0, int z0,
+template <typename R, int RZ, int CLS, int NW = 4>
+RTM_HD void tile_psi(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, int x0, int z0,
                      int tx, int ty)
 {
     if (CLS & T_XSTRIP) {
@@ -586,7 +587,7 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
         const bool sa = x_strip(S, x0 - 4 + tx), sb = x_strip(S, x0 - 4 + cb);
 #pragma unroll
         for (int t = 0; t < RZ; t++) {
-            const int r = ty + 4 * t;
+            const int r = ty + NW * t;
             const bool kok = z0 + r < S.Z - kHalf;
             const R *la = &L.u[r + 8][tx + 4];
             const R *lb = &L.u[r + 8][cb + 4];
@@ -602,8 +603,8 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
         const bool iok = x0 + tx < S.X - kHalf;
         const int lc = tx + 8;
 #pragma unroll
-        for (int t = 0; t < RZ + 2; t++) {
-            const int r = ty + 4 * t;
+        for (int t = 0; t < RZ + 8 / NW; t++) {
+            const int r = ty + NW * t;
             const int k = z0 - 4 + r;
             R v = R(0);
             if (z_strip(S, k)) {   /* uniform: k is the same for the whole wavefront */
@@ -619,8 +620,8 @@ RTM_HD void tile_psi(TileLds<R, RZ> &L, const TileRegs<R, RZ> &G, const SimDesc<
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false>
-RTM_HD float tile_points(const TileLds<R, RZ> &L, TileRegs<R, RZ> &G, const SimDesc<R> &S,
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4>
+RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4;
@@ -743,8 +744,8 @@ RTM_HD float tile_points(const TileLds<R, RZ> &L, TileRegs<R, RZ> &G, const SimD
 /* Forward, int16 block store: the thread's image points of this step's
  * snapshot.  rmax[rr] is the max |uq[rr]| over the wavefront (one block:
  * the tile row's 64 columns); lane 0 writes the block's exponent. */
-template <typename R, int RZ>
-RTM_HD void tile_store_i16(const TileRegs<R, RZ> &G, const SimDesc<R> &S, const StepView<R> &V, const float *rmax,
+template <typename R, int RZ, int NW = 4>
+RTM_HD void tile_store_i16(const TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepView<R> &V, const float *rmax,
                            int x0, int z0, int tx, int ty)
 {
     const int i = x0 + tx;

@@ -3,10 +3,13 @@
  * 4 <= k < Z-4 is split into work items for the step kernels.  Shared by
  * the library and the CPU test harness so both walk the same items.
  *
- * Tiles are 64 x TZ (TZ = 4*rz) from (4, 4).  Each tile has a class: which
- * CPML strips its points touch.  The interior (class-0 tiles, whose points
- * need no CPML term) is handled by one of two kernels:
+ * Tiles are 64 x TZ (TZ = 4*rz, or 8*rz for IN_TILES8) from (4, 4).  Each
+ * tile has a class: which CPML strips its points touch.  The interior
+ * (class-0 tiles, whose points need no CPML term) is handled by one of two
+ * kernels:
  *   IN_TILES    every tile goes to the tile kernel;
+ *   IN_TILES8   the same with workgroups of 8 wavefronts, each owning rz
+ *               rows (tiles of 64 x 8rz);
  *   IN_COLUMNS  class-0 runs of two adjacent tile columns with the same
  *               rows become one 128-column item, a lone column a 64-column
  *               item, in segments of at most seg_rows rows (rtm_column.h);
@@ -25,7 +28,10 @@
 
 namespace rtm {
 
-enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1, IN_MIXED = 2 };
+enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1, IN_MIXED = 2, IN_TILES8 = 3 };
+
+/* rows of a tile of interior kind `interior` at rz rows per thread */
+inline int tile_rows(int rz, int interior) { return (interior == IN_TILES8 ? 8 : 4) * rz; }
 
 struct WorkItem {
     int a, b, c, d;   /* tile {sim, x0, z0, class}; interior {sim(+half), x0, zA, zB} */
@@ -34,7 +40,7 @@ struct WorkItem {
 inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inner, int sim, int X, int Z, int TL,
                        int TR, int TU, int TD, int rz, int interior, int seg_rows)
 {
-    const int TZ = 4 * rz;
+    const int TZ = tile_rows(rz, interior);
     SimDesc<float> d{};
     d.X = X; d.Z = Z; d.TL = TL; d.TR = TR; d.TU = TU; d.TD = TD;
     int seg = 0;

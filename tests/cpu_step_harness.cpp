@@ -42,29 +42,30 @@ static void commit_max(const rtm::StepView<R> &V, float m)
 }
 
 // Emulation of k_tiles for one simulation at step n: every tile, phase by
-// phase, all 256 threads of a phase before the next (the barriers).
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false>
-static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm::SimDesc<R> &S,
+// phase, all 64*NW threads of a phase before the next (the barriers).
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4>
+static void run_tile(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, const rtm::SimDesc<R> &S,
                      const rtm::StepView<R> &V, int x0, int z0)
 {
-    for (int ty = 0; ty < 4; ty++)
+    for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            rtm::tile_fetch<R, RZ, CLS, Q16, I16>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
-            rtm::tile_stage<R, RZ, CLS>(L, G[ty * 64 + tx], tx, ty);
+            rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            rtm::tile_stage<R, RZ, CLS, NW>(L, G[ty * 64 + tx], tx, ty);
         }
     if (CLS)
-        for (int ty = 0; ty < 4; ty++)
-            for (int tx = 0; tx < 64; tx++) rtm::tile_psi<R, RZ, CLS>(L, G[ty * 64 + tx], S, x0, z0, tx, ty);
+        for (int ty = 0; ty < NW; ty++)
+            for (int tx = 0; tx < 64; tx++)
+                rtm::tile_psi<R, RZ, CLS, NW>(L, G[ty * 64 + tx], S, x0, z0, tx, ty);
     float m = 0.0f;
-    for (int ty = 0; ty < 4; ty++)
+    for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            const float t = rtm::tile_points<R, RZ, CLS, Q16, I16>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            const float t = rtm::tile_points<R, RZ, CLS, Q16, I16, NW>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             m = t > m ? t : m;
         }
     commit_max(V, m);
     if (I16 && V.forward && V.qsnap) {
         // the kernel's wavefront max of each row's |U| (the block), then the store
-        for (int ty = 0; ty < 4; ty++) {
+        for (int ty = 0; ty < NW; ty++) {
             float rmax[RZ];
             for (int rr = 0; rr < RZ; rr++) {
                 rmax[rr] = 0.0f;
@@ -73,7 +74,8 @@ static void run_tile(rtm::TileLds<R, RZ> &L, rtm::TileRegs<R, RZ> *G, const rtm:
                     rmax[rr] = a > rmax[rr] ? a : rmax[rr];
                 }
             }
-            for (int tx = 0; tx < 64; tx++) rtm::tile_store_i16<R, RZ>(G[ty * 64 + tx], S, V, rmax, x0, z0, tx, ty);
+            for (int tx = 0; tx < 64; tx++)
+                rtm::tile_store_i16<R, RZ, NW>(G[ty * 64 + tx], S, V, rmax, x0, z0, tx, ty);
         }
     }
 }
@@ -130,6 +132,8 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
     if (!V.active) return;
     auto *L = new rtm::TileLds<R, RZ>();
     auto *G = new rtm::TileRegs<R, RZ>[256];
+    auto *L8 = new rtm::TileLds<R, RZ, 8>();
+    auto *G8 = new rtm::TileRegs<R, RZ, 8>[512];
     auto *Ls = new rtm::TileLds<R, 1>();
     auto *Gs = new rtm::TileRegs<R, 1>[256];
     std::vector<rtm::WorkItem> tiles, inner;
@@ -145,6 +149,15 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
             }
             continue;
         }
+        if (interior == rtm::IN_TILES8) {   // workgroups of 8 wavefronts (k_tiles NW = 8)
+            switch (t.d) {
+            case 0: run_tile<R, RZ, 0, Q16, I16, 8>(*L8, G8, S, V, t.b, t.c); break;
+            case 1: run_tile<R, RZ, 1, Q16, I16, 8>(*L8, G8, S, V, t.b, t.c); break;
+            case 2: run_tile<R, RZ, 2, Q16, I16, 8>(*L8, G8, S, V, t.b, t.c); break;
+            default: run_tile<R, RZ, 3, Q16, I16, 8>(*L8, G8, S, V, t.b, t.c); break;
+            }
+            continue;
+        }
         switch (t.d) {
         case 0: run_tile<R, RZ, 0, Q16, I16>(*L, G, S, V, t.b, t.c); break;
         case 1: run_tile<R, RZ, 1, Q16, I16>(*L, G, S, V, t.b, t.c); break;
@@ -155,6 +168,8 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
     for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
     delete[] Gs;
     delete Ls;
+    delete[] G8;
+    delete L8;
     delete[] G;
     delete L;
 }
@@ -223,7 +238,7 @@ int hx_balance(int X, int Z, int TL, int TR, int TU, int TD, int rz, int nsims, 
 {
     std::vector<rtm::WorkItem> tiles, inner;
     for (int s = 0; s < nsims; s++) rtm::build_work(tiles, inner, s, X, Z, TL, TR, TU, TD, rz, interior, 0);
-    rtm::balance_for_xcds(tiles, 4 * rz);
+    rtm::balance_for_xcds(tiles, rtm::tile_rows(rz, interior));
     if ((int)tiles.size() > cap) return -1;
     for (size_t i = 0; i < tiles.size(); i++) {
         out[4 * i] = tiles[i].a; out[4 * i + 1] = tiles[i].b; out[4 * i + 2] = tiles[i].c; out[4 * i + 3] = tiles[i].d;

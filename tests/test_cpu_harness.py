@@ -44,7 +44,7 @@ def test_forward_backward_bitexact(hx, name, dtype):
 @pytest.mark.parametrize("name", ["tiny", "asym", "wide"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("rz,interior,seg", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (4, 2, 0), (2, 2, 0), (4, 1, 0), (2, 1, 12),
-                                             (1, 1, 4), (8, 1, 32)])
+                                             (1, 1, 4), (8, 1, 32), (2, 3, 0), (4, 3, 0)])
 def test_tiled_kernel_bitexact(hx, name, dtype, rz, interior, seg):
     """The step kernels' bodies run on the CPU against the oracle: the
     LDS-tiled kernel (rtm_tile.h, its phases in sequence for all threads of
@@ -166,7 +166,7 @@ def test_bf16_rule_matches_bit_rounding():
 
 
 @pytest.mark.parametrize("qmode", [1, 2])
-@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0)])
+@pytest.mark.parametrize("ts,interior", [(1, 0), (4, 1), (5, 0), (4, 3)])
 def test_sixteen_bit_store_bitexact(hx, qmode, ts, interior):
     """FP16 / BF16 snapshot store (per-snapshot power-of-two max-abs scale,
     filled one step late from u^n, the last snapshot flushed when TS = 1):
@@ -191,9 +191,9 @@ def test_sixteen_bit_store_bitexact(hx, qmode, ts, interior):
 
 
 @pytest.mark.parametrize("ts", [1, 4])
-@pytest.mark.parametrize("rz", [1, 2, 4, 8])
+@pytest.mark.parametrize("rz,interior", [(1, 0), (2, 0), (4, 0), (8, 0), (2, 3), (4, 3)])
 @pytest.mark.parametrize("name", ["wide", "asym"])
-def test_int16_block_store_bitexact(hx, name, rz, ts):
+def test_int16_block_store_bitexact(hx, name, rz, interior, ts):
     """int16 block store (a power-of-two scale per image row and block of 64
     padded columns, written in the step that computes the snapshot): codes
     and exponents equal the numpy restatement of the rule applied to the
@@ -207,12 +207,12 @@ def test_int16_block_store_bitexact(hx, name, rz, ts):
     S, _, _, _ = O.forward(op, sx, sz, vpad, g["wavelet"], dtype=np.float32, record=False)
     deq, codes, exps = common.quantize_i16_blocks(S, op.t_left)
     ds = DL.DevSim(hx, op, vpad, np.float32)
-    (q16, qexp), _ = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, qmode=DL.Q_I16)
+    (q16, qexp), _ = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, interior=interior, qmode=DL.Q_I16)
     assert np.array_equal(qexp, exps)
     assert np.array_equal(q16, codes)
     d = g["seismograms"][0]
     I_ref, Is_ref, _, _ = O.backward(op, vpad, d, deq)
-    I2, Is2 = ds.backward_tiled(d, (q16, qexp), rz=rz, qmode=DL.Q_I16)
+    I2, Is2 = ds.backward_tiled(d, (q16, qexp), rz=rz, interior=interior, qmode=DL.Q_I16)
     assert np.array_equal(I2, I_ref) and np.array_equal(Is2, Is_ref)
 
 
@@ -315,7 +315,7 @@ def test_mixed_tile_list_covers_the_stepped_region_once(hx, rz):
         assert (full[:, 3] == 0).all() and len(full) < len(t)   # a narrow grid may have no full tile
 
 
-@pytest.mark.parametrize("nsims,interior", [(1, 0), (3, 0), (1, 2), (3, 2)])
+@pytest.mark.parametrize("nsims,interior", [(1, 0), (3, 0), (1, 2), (3, 2), (3, 3)])
 def test_xcd_balance_keeps_every_tile_once(hx, nsims, interior):
     """The tile list reordered for the eight XCDs (rtm::balance_for_xcds):
     it holds build_work's tiles each exactly once; the eight shares have one
@@ -335,7 +335,7 @@ def test_xcd_balance_keeps_every_tile_once(hx, nsims, interior):
             ref = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, interior, 0)
             want = sorted((s, *map(int, r[1:])) for s in range(nsims) for r in ref)
             assert sorted(tuple(map(int, r)) for r in real) == want
-            TZ = 4 * rz
+            TZ = (8 if interior == 3 else 4) * rz
             cost_of = lambda r: (3 if r[3] & 3 else 1) * (4 / TZ if r[3] & 4 else 1)  # noqa: E731
             col_share = {}
             for k in range(8):

@@ -105,7 +105,7 @@ def _wide_case():
 
 
 @pytest.mark.parametrize("rz,interior,seg", [(4, 1, 0), (2, 1, 12), (1, 1, 4), (8, 1, 32), (4, 1, 16), (4, 2, 0),
-                                             (2, 2, 0)])
+                                             (2, 2, 0), (2, 3, 0), (4, 3, 0)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
     """Strip tiles plus the lane-shift column kernel on the interior
@@ -163,13 +163,13 @@ def test_batch_marched_bitexact(gpu_available, interior, seg):
             assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
 
 
-@pytest.mark.parametrize("rz,group", [(2, 0), (8, 0), (4, 2), (2, 4)])
-def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group):
+@pytest.mark.parametrize("rz,group,interior", [(2, 0, 0), (8, 0, 0), (4, 2, 0), (2, 4, 0), (2, 4, 3), (4, 0, 3)])
+def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior):
     """A batch of realizations advanced in lockstep: each image bit-identical
     to the oracle's for that realization (alone or in the batch, in one
     launch group or in groups of 2 or 4 with a ragged last group), and the
     FP64 Welford mean/variance equal to numpy's over the same images."""
-    say(f"[gpu] batch ensemble rz={rz} group={group}")
+    say(f"[gpu] batch ensemble rz={rz} group={group} interior={interior}")
     cfg = common.REF_CONFIGS["asym"]
     g = golden("asym")
     op = common.orc_params(cfg)
@@ -182,7 +182,7 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group):
     vpads = [O.extend(op, m) for m in models]
     refs = [O.rtm_shot(op, sx, sz, v, g["wavelet"], d) for v in vpads]
     with R.Batch(p, len(models)) as b:
-        b.set_tiling(rz, 100)
+        b.set_tiling(rz, 100, interior)
         b.set_group(group)
         assert b.group() == ((group, (len(models) + group - 1) // group) if group else (len(models), 1))
         for k, v in enumerate(vpads):
@@ -363,7 +363,7 @@ def test_boundary_saving_records_traces(gpu_available, spacing):
             assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
-@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 16), (2, 1, 0), (4, 2, 0)])
+@pytest.mark.parametrize("rz,interior,seg", [(4, 1, 16), (2, 1, 0), (4, 2, 0), (2, 3, 0)])
 def test_boundary_saving_with_column_interior_bitexact(gpu_available, rz, interior, seg):
     """Boundary saving on a grid wide enough for column items (a 128-column
     and a 64-column item): the column kernel images against the rebuilt
@@ -471,13 +471,13 @@ def test_batch_sixteen_bit_snapshots_bitexact(gpu_available):
             assert np.array_equal(I2, I) and np.array_equal(Is2, Is), k
 
 
-@pytest.mark.parametrize("ts,rz", [(1, 2), (4, 4), (5, 1), (4, 8)])
-def test_int16_block_snapshots_bitexact(gpu_available, ts, rz):
+@pytest.mark.parametrize("ts,rz,interior", [(1, 2, 0), (4, 4, 0), (5, 1, 0), (4, 8, 0), (1, 2, 3), (4, 4, 3)])
+def test_int16_block_snapshots_bitexact(gpu_available, ts, rz, interior):
     """int16 block store (a power-of-two scale per image row and 64-column
     block, stored in the step that computes the snapshot): the image equals
     the oracle's backward pass over the oracle's FP32 snapshots put through
     the store's rule (numpy restatement), bit for bit."""
-    say(f"[gpu] int16 block store TS={ts} rz={rz}")
+    say(f"[gpu] int16 block store TS={ts} rz={rz} interior={interior}")
     cfg, model, wav, seis = _wide_case()
     op = common.orc_params(cfg, time_slices=ts)
     p = common.rtm_params(cfg, time_slices=ts, snapshot_mode=R.SNAP_I16)
@@ -487,7 +487,7 @@ def test_int16_block_snapshots_bitexact(gpu_available, ts, rz):
     deq, _, _ = common.quantize_i16_blocks(S, op.t_left)
     I, Is, _, _ = O.backward(op, vpad, seis, deq)
     with R.Plan(p) as plan:
-        plan.set_tiling(rz, 100, 0, 0)
+        plan.set_tiling(rz, 100, interior, 0)
         plan.set_velocity(vpad)
         plan.set_wavelet(wav)
         plan.set_seismogram(seis)
@@ -498,9 +498,10 @@ def test_int16_block_snapshots_bitexact(gpu_available, ts, rz):
 
 
 def test_int16_block_store_refuses_non_tile_shapes_and_tunes_tiles_only(gpu_available):
-    """The int16 block store runs in tile shapes: a column or mixed interior
-    is refused with the reason and the current shape kept; the autotuner
-    times tile shapes only and keeps results exact."""
+    """The int16 block store runs in tile shapes (of 4 or 8 wavefronts): a
+    column or mixed interior is refused with the reason and the current
+    shape kept; the autotuner times tile shapes only and keeps results
+    exact."""
     say("[gpu] int16 block store: shapes")
     cfg, model, wav, seis = _wide_case()
     op = common.orc_params(cfg, time_slices=4)
@@ -516,7 +517,7 @@ def test_int16_block_store_refuses_non_tile_shapes_and_tunes_tiles_only(gpu_avai
                 plan.set_tiling(4, 100, interior, 16)
         assert plan.tiling()[2] == 0
         tuned = plan.autotune(20)
-        assert tuned and all(t["interior"] == 0 for t in tuned)
+        assert tuned and all(t["interior"] in (0, 3) for t in tuned)   # tiles of 4 or 8 wavefronts
         plan.set_velocity(vpad)
         plan.set_wavelet(wav)
         plan.set_seismogram(seis)
@@ -928,7 +929,8 @@ def _server_requests():
 
 
 @pytest.mark.parametrize("shape,group_bytes", [((4, 50, 1, 16), None), ((4, 50, 2, 0), None),
-                                               ((2, 30, 0, 0), None), ((4, 50, 0, 0), 1)])
+                                               ((2, 30, 0, 0), None), ((4, 50, 0, 0), 1), ((2, 50, 3, 0), None),
+                                               ((2, 50, 3, 0), 1)])
 def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape, group_bytes):
     """Continuous batching: requests of three grid sizes arrive while others
     are in flight, are admitted by HBM footprint (the arena holds about two at
