@@ -67,11 +67,12 @@ def full_step_bytes(p, ts):
 
 def tiling_dict(shape):
     """(tile rows per thread, graph block, interior kernel 0 tiles / 1
-    columns / 2 mixed tiles / 3 tiles of 8 wavefronts, rows per interior
-    segment) as a dict."""
+    columns / 2 mixed tiles / 3 tiles of 8 wavefronts / 4 two-step pair
+    items, rows per interior segment) as a dict."""
     rz, block, interior, seg = shape
     return {"rows_per_thread": rz, "graph_block": block,
-            "interior": ("tiles", "columns", "mixed", "tiles of 8 wavefronts")[interior], "seg_rows": seg}
+            "interior": ("tiles", "columns", "mixed", "tiles of 8 wavefronts", "two-step pairs")[interior],
+            "seg_rows": seg}
 
 
 def log(*a):
@@ -376,6 +377,8 @@ def main() -> None:
                     help="profile workload: steps per captured graph of the batch (1: eager launches)")
     ap.add_argument("--profile-batch-only", action="store_true",
                     help="profile workload: only the c2 launch group in 64x16 tiles (counter passes)")
+    ap.add_argument("--profile-batch-shapes", default=None,
+                    help="profile workload: the launch group's kernel shapes, 'rz,block,interior,seg;...'")
     ap.add_argument("--no-c0", dest="c0", action="store_false",
                     help="skip the c0 leg (two 600x200 simulations through rtm_routine, GPU vs CPU)")
     args = ap.parse_args()
@@ -389,8 +392,11 @@ def main() -> None:
         log("no HIP device")
         sys.exit(1)
     if args.profile:
+        shapes = None
+        if args.profile_batch_shapes:
+            shapes = [tuple(int(v) for v in sh.split(",")) for sh in args.profile_batch_shapes.split(";")]
         run_profile(R, args.profile_nt, args.profile_sims, batch_only=args.profile_batch_only,
-                    block=args.profile_block)
+                    block=args.profile_block, shapes=shapes)
         return
     if args.only_serve:
         print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
@@ -467,25 +473,14 @@ def main() -> None:
     bytes_per_launch = BYTES_PER_UPDATE * X * Z
     full_step = full_step_bytes(p, ts=1)
     achieved = bytes_per_launch / (step_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": round(3600.0 / secs, 3),
-        "unit": "images/hour",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(secs * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
-                        "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
-            "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
-            "tiling": tiling_dict(best[1]),
-        },
+    plain = {
+        "workload": "c1 plain single-simulation RTM: 1701x351, h=10 m, CPML 50, dt=0.8 ms, "
+                    "nt=4000, 1701 receivers, FP32 snapshots every step on HBM",
+        "images_per_hour": round(3600.0 / secs, 3),
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_image": round(secs * 1e3, 3),
+        "grid_padded": [X, Z], "nt": nt, "time_slices": 1, "simulations": 1,
+        "tiling": tiling_dict(best[1]),
         "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
                           us_per_step=round(t["ms_per_step"] * 1e3, 3)) for t in tuning],
         "gupdates_per_s": round(updates / secs / 1e9, 3),
@@ -505,16 +500,53 @@ def main() -> None:
         "image_norm": float(np.linalg.norm(I.astype(np.float64))),
         "image_accuracy": accuracy,
         "meets_gate": accuracy["meets_gate"],
+    }
+    out = {
+        "metric": METRIC,
+        "value": plain["images_per_hour"],
+        "unit": "images/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": plain["ms_per_image"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": plain["workload"], "grid_padded": [X, Z], "nt": nt, "time_slices": 1,
+                   "simulations": 1, "tiling": plain["tiling"]},
+        "roofline": plain["roofline"],
+        "gupdates_per_s": plain["gupdates_per_s"],
+        "image_accuracy": accuracy,
+        "meets_gate": accuracy["meets_gate"],
+        "plain": plain,
         "zfp_host": zfp_host,
     }
     if args.batch > 0:
-        out["batched"] = run_batch(R, p, v_true, w, seis, args.batch, args.batch_ts)
-        b = out["batched"]
+        # the headline: the batched ensemble (c2), one step = one migration of
+        # the whole batch; the plain path beside it at the same snapshot interval
+        b = run_batch(R, p, v_true, w, seis, args.batch, args.batch_ts, steps=args.steps, warmup=args.warmup)
+        out["batched"] = b
+        out.update({
+            "value": b["images_per_hour"],
+            "ms_per_step": round(b["seconds"] * 1e3, 3),
+            "config": {"workload": b["workload"], "grid_padded": [X, Z], "nt": nt,
+                       "time_slices": b["time_slices"], "simulations": b["simulations"],
+                       "launch_groups": b["launch_groups"], "tiling": b["tiling"]},
+            "roofline": b["roofline"],
+            "image_accuracy": {"reference": "FP64 GPU plan of the same scheme, realization 0",
+                               "relL2_vs_fp64": b["image0_relL2_vs_fp64"], "gate": RELL2_GATE,
+                               "meets_gate": b["meets_gate"]},
+            "meets_gate": bool(b["meets_gate"] and accuracy["meets_gate"]),
+            "gupdates_per_s": b["gupdates_per_s"],
+        })
         out["batched_over_plain"] = {
             "ratio": b["batched_over_plain"],
             "batched": b["workload"] + f"; {b['images_per_hour']} images/h",
             "plain": f"the same realization alone on the plain path at TS={b['time_slices']}; "
                      f"{b['plain_images_per_hour_same_ts']} images/h",
+            "plain_ts1_images_per_hour": plain["images_per_hour"],
             "target": 5.0,
         }
     if args.c0:
@@ -532,7 +564,7 @@ def main() -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_profile(R, nt=400, nsims=13, batch_only=False, block=100):
+def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None):
     """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
     pass per counter group over the same dispatches):
       1. a 1 GiB device-to-device copy (torch), the calibration of the
@@ -543,7 +575,9 @@ def run_profile(R, nt=400, nsims=13, batch_only=False, block=100):
          kernel.
     batch_only: the copy, then step 3 in 64x16 tiles alone, so that every
     step-kernel dispatch of the run is the launch group's.  block: steps per
-    captured graph of step 3 (1: every step launched on its own)."""
+    captured graph of step 3 (1: every step launched on its own).  shapes:
+    the batch's kernel shapes for step 3 instead (rows_per_thread,
+    graph_block, interior, seg_rows each), one run per shape."""
     import torch
     x = torch.rand(1 << 28, device="cuda")
     y = torch.empty_like(x)
@@ -570,7 +604,9 @@ def run_profile(R, nt=400, nsims=13, batch_only=False, block=100):
             b.set_velocity(k, vpad)
         b.set_wavelet(w)
         b.set_seismogram(seis)
-        for shape in ((4, block, 0, 0),) if batch_only else ((4, block, 0, 0), (4, block, 1, 32)):
+        if shapes is None:
+            shapes = ((4, block, 0, 0),) if batch_only else ((4, block, 0, 0), (4, block, 1, 32))
+        for shape in shapes:
             b.set_tiling(*shape)
             b.zero_images()
             b.migrate(pb.first_location_shot, pb.depth_shot)
@@ -849,7 +885,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_batch(R, p1, v_true, w, seis, nsims, ts):
+def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
     """Config c2: `nsims` realizations of the c1 model, v_k = gauss(v_true, 10)
     * exp(0.03 g_k) with g_k a Gaussian random field of 20-point correlation
     length, sharing one seismogram; snapshots every `ts` steps (FP32);
@@ -909,24 +945,34 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
             f"{t['ms_per_step'] * 1e3:.1f} us/step")
     bbest = (None, b.tiling())
     gsize, ngroups = b.group()
+    b_group = (gsize, ngroups)
     log(f"[bench] batch using {tiling_dict(bbest[1])}, {ngroups} launch groups of {gsize}")
+    for i in range(warmup):
+        b.zero_images()
+        b.migrate(sx, sz)
+        log(f"[bench] batch warmup {i + 1}/{warmup} done")
+    # the timed region: `steps` whole-batch migrations (each returns when its
+    # last pass has ended, so the host clock brackets finished device work)
     runs, steps_fb = [], []
-    for _ in range(3):                      # migrate returns when its last pass has ended
+    t_all = time.perf_counter()
+    for i in range(steps):
         b.zero_images()
         t0 = time.perf_counter()
         b.migrate(sx, sz)
         runs.append(time.perf_counter() - t0)
         steps_fb.append(b.last_timing()[:2])
-    k = int(np.argsort(runs)[1])            # the median run
-    batch_secs = runs[k]
-    f_ms, b_ms = steps_fb[k]
+        log(f"[bench] batch step {i + 1}/{steps}: {runs[-1]:.3f} s")
+    batch_secs = (time.perf_counter() - t_all) / steps
+    f_ms = float(np.mean([t[0] for t in steps_fb]))
+    b_ms = float(np.mean([t[1] for t in steps_fb]))
     t0 = time.perf_counter()
     mean, var = b.welford()
     welford_s = time.perf_counter() - t0
     I0, _ = b.image(0)
     b.close()
-    step_ms = (f_ms + b_ms) / 2.0
+    step_ms = (f_ms + b_ms) / 2.0          # one time step of every launch group
     achieved = BYTES_PER_UPDATE * X * Z * nsims / (step_ms * 1e-3) / 1e9
+    gsize_, ngroups_ = b_group
     plain_iph = 3600.0 / plain_secs
     batch_iph = nsims * 3600.0 / batch_secs
     out = {
@@ -936,6 +982,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "images_per_hour": round(batch_iph, 1),
         "gupdates_per_s": round(2.0 * nt * X * Z * nsims / batch_secs / 1e9, 2),
         "seconds": round(batch_secs, 4),
+        "steps": steps, "warmup": warmup,
         "run_seconds": [round(t, 4) for t in runs],
         "plain_images_per_hour_same_ts": round(plain_iph, 1),
         "batched_over_plain": round(batch_iph / plain_iph, 3),
@@ -949,7 +996,11 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "traffic_note": "not measured: under a rocprofv3 counter pass the batched kernels skip their "
-                                     "work (zero images, 3.6 us dispatches), so no counter reading describes them"},
+                                     "work (zero images, 3.6 us dispatches), so no counter reading describes them",
+                     "algorithmic_bytes_per_launch": BYTES_PER_UPDATE * X * Z * gsize_,
+                     "launch_us": round(step_ms * 1e3 / ngroups_, 3),
+                     "launch_note": f"one launch steps a group of {gsize_} realizations; {ngroups_} launches per "
+                                    "time step, the last group ragged"},
         "welford_seconds": round(welford_s, 4),
         "image0_relL2_vs_fp64": rel_l2(I0, I64),
         "meets_gate": gate(rel_l2(I0, I64)),

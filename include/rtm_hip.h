@@ -177,14 +177,24 @@ size_t rtm_plan_footprint(const rtm_params *p, int real);   /* HBM bytes a plan 
  * COLUMNS wavefronts march down 128-column strips with a register window
  *         and lane shifts (no LDS, no barrier), beside the strip tiles;
  * MIXED   one tile launch with 64 x 4*rows_per_thread interior tiles and
- *         64 x 4 tiles wherever a CPML strip is touched. */
-enum rtm_interior { RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1, RTM_INTERIOR_MIXED = 2 };
+ *         64 x 4 tiles wherever a CPML strip is touched;
+ * TILES8  tiles of 64 x 8*rows_per_thread in workgroups of 8 wavefronts;
+ * PAIRS   the whole tiles of the image region (but those within 4 points of
+ *         its edge) are stepped two time steps per launch by wavefronts
+ *         marching 112-column strips (segments of seg_rows rows), reading
+ *         u^n, u^{n-1} and C once for both steps; every other tile one step
+ *         per launch.  Four wavefield buffers per simulation; FP32 fields and
+ *         the FP32 snapshot store only (the request server does not take it). */
+enum rtm_interior {
+    RTM_INTERIOR_TILES = 0, RTM_INTERIOR_COLUMNS = 1, RTM_INTERIOR_MIXED = 2, RTM_INTERIOR_TILES8 = 3,
+    RTM_INTERIOR_PAIRS = 4
+};
 
 /* Kernel shape: rows_per_thread in {1,2,4,8} gives tiles of 64 x
  * 4*rows_per_thread points (MIXED: 2 or 4, for the interior tiles);
  * graph_block = time steps per captured hipGraph block (<= 1: one eager
  * launch per step); interior = enum rtm_interior; seg_rows = rows per column
- * segment (COLUMNS; <= 0: whole strips).  Tiles stay wherever a CPML strip is
+ * segment (COLUMNS, PAIRS; <= 0: whole strips).  Tiles stay wherever a CPML strip is
  * touched.  Defaults: 4, 100, TILES, 0.  Results are bit-identical for every
  * shape. */
 int rtm_plan_set_tiling(rtm_plan *plan, int rows_per_thread, int graph_block, int interior, int seg_rows);

@@ -21,7 +21,7 @@ PATH_LEN = 512
 
 SNAP_FP32, SNAP_FP16, SNAP_BF16, SNAP_BOUNDARY, SNAP_I16 = 0, 1, 2, 3, 4
 REAL_F32, REAL_F64 = 0, 1
-INTERIOR_TILES, INTERIOR_COLUMNS, INTERIOR_MIXED, INTERIOR_TILES8 = 0, 1, 2, 3
+INTERIOR_TILES, INTERIOR_COLUMNS, INTERIOR_MIXED, INTERIOR_TILES8, INTERIOR_PAIRS = 0, 1, 2, 3, 4
 
 
 class ZfpMode(C.Structure):
@@ -276,7 +276,7 @@ class Plan:
     def set_tiling(self, rows_per_thread: int = 4, graph_block: int = 100, interior: int = 0,
                    seg_rows: int = 0) -> None:
         """Kernel shape (rtm_plan_set_tiling): tile rows per thread, steps per
-        graph block, interior kernel (INTERIOR_TILES / _COLUMNS / _MIXED) and
+        graph block, interior kernel (INTERIOR_TILES / _COLUMNS / _MIXED / _TILES8 / _PAIRS) and
         rows per column segment."""
         _check(lib().rtm_plan_set_tiling(self.h, int(rows_per_thread), int(graph_block), int(interior),
                                          int(seg_rows)), "set_tiling")

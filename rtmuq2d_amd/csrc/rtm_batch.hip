@@ -155,7 +155,11 @@ static void batch_drop_execs(rtm_batch *q)
 static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int seg_rows)
 {
     // the current shape stays when the new one is refused
-    if (check_shape(rz, interior) || check_store_shape(q->sims[0].p.snapshot_mode, interior)) return -1;
+    if (check_shape(rz, interior) || check_store_shape(q->sims[0].p.snapshot_mode, interior) ||
+        check_pair_real(q->real, interior))
+        return -1;
+    for (auto &b : q->sims)
+        if (b.set_quad(interior == rtm::IN_PAIRS)) return -1;
     const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
     batch_drop_execs(q);
     q->ex.resize(ng);       // never resized again before the next release
@@ -508,14 +512,17 @@ static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, doubl
     double best_t = 1e30;
     Shape best = cand[0];
     int k = 0;
-    std::vector<SimDesc<R>> d;
-    for (int s = 0; s < group_size(q); s++) d.push_back(q->sims[s].desc<R>());
     // the snapshot stores are rewritten by the next forward pass; boundary
     // saving keeps none
     const int flags = snaps && !q->sims[0].ringmode ? rtm::F_SNAP : 0;
     for (const Shape &c : cand) {
-        if (check_store_shape(q->sims[0].p.snapshot_mode, c.interior)) continue;   // not a shape of this store
+        // not a shape of this store or element type
+        if (check_store_shape(q->sims[0].p.snapshot_mode, c.interior) || check_pair_real(q->real, c.interior))
+            continue;
         if (batch_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
+        // descriptors after the tiling call: a pair shape steps four u buffers
+        std::vector<SimDesc<R>> d;
+        for (int s = 0; s < group_size(q); s++) d.push_back(q->sims[s].desc<R>());
         double t = 0.0;
         if (time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, &t, flags)) return -1;
         if (k < cap) {

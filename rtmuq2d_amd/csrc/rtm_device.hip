@@ -56,6 +56,7 @@ struct rtm_plan {
     int seg_rows = 0;               // rows per interior column segment
     int last_nt = 0;                // steps of the last pass
     bool last_fwd_ring = false;     // the last pass was a boundary-saving forward pass
+    int last_umask = 1;             // u buffer mask of the last pass (3: pair launches)
     int fwd_src_j = -1;             // source index of the last forward pass (the reconstruction re-adds it)
     double fwd_ms = 0.0, bwd_ms = 0.0;
     int last_steps = 0;
@@ -84,7 +85,10 @@ static void plan_free(rtm_plan *q)
 static int plan_set_tiling(rtm_plan *q, int rz, int graph_block, int interior, int seg_rows)
 {
     // the current shape stays when the new one is refused
-    if (check_shape(rz, interior) || check_store_shape(q->b.p.snapshot_mode, interior)) return -1;
+    if (check_shape(rz, interior) || check_store_shape(q->b.p.snapshot_mode, interior) ||
+        check_pair_real(q->b.real, interior))
+        return -1;
+    if (q->b.set_quad(interior == rtm::IN_PAIRS)) return -1;
     std::vector<int4> tiles, inner;
     append_work(tiles, inner, 0, q->b.p, rz, interior, seg_rows);
     balance_tiles_for_xcds(tiles, rtm::tile_rows(rz, interior));
@@ -315,11 +319,14 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         if (record && !(d.flags & rtm::F_RECORD)) {
             // receivers sharing a point or outside: record each one per step
+            // (with pair items an even step's launch also steps them at n+1,
+            // and the next launch completes step n+1 on the other points)
             void *const *uf = b.ringmode ? b.sbuf : b.u;   // the buffers this pass steps
+            const int m = b.ringmode ? 1 : b.umask;
             for (int n = 0; n < nt; n++) {
                 q->ex.launch_one(n, q->stream);
                 hipLaunchKernelGGL(k_record<R>, dim3((p.number_of_receivers + 255) / 256), dim3(256), 0,
-                                   q->stream, (const R *)uf[(n + 1) & 1], b.rec_j, p.number_of_receivers,
+                                   q->stream, (const R *)uf[(n + 1) & m], b.rec_j, p.number_of_receivers,
                                    b.seis_rec + (size_t)n * p.number_of_receivers);
             }
         } else if (q->ex.run(nt, q->graph_block, q->stream)) {
@@ -338,6 +345,7 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
     q->last_steps = nt;
     q->last_nt = nt;
     q->last_fwd_ring = b.ringmode;
+    q->last_umask = tiled && !b.ringmode ? b.umask : 1;
     q->fwd_src_j = src_j;
     b.have_snaps = true;
     return 0;
@@ -376,6 +384,7 @@ static int backward_t(rtm_plan *q)
     q->last_steps = nt;
     q->last_nt = nt;
     q->last_fwd_ring = false;
+    q->last_umask = b.n_out == 0 ? b.umask : 1;
     if (b.ringmode) b.have_snaps = false;   // the reconstruction consumed the saved source field
     return 0;
 }
@@ -603,10 +612,10 @@ static int get_fields_t(rtm_plan *q, O *u_prev, O *u_last)
     const SimBuffers &b = q->b;
     std::vector<R> h(b.nfield);
     HIP_TRY(hipStreamSynchronize(q->stream));
-    // after nt steps u[nt & 1] holds u^{nt} and the other buffer u^{nt-1}
-    const int nt = q->last_nt;
+    // after nt steps u[nt & m] holds u^{nt} and u[(nt-1) & m] u^{nt-1}
+    const int nt = q->last_nt, m = q->last_umask;
     void *const *w = q->last_fwd_ring ? b.sbuf : b.u;
-    void *src[2] = {w[(nt + 1) & 1], w[nt & 1]};
+    void *src[2] = {w[(nt - 1) & m], w[nt & m]};
     O *dst[2] = {u_prev, u_last};
     for (int t = 0; t < 2; t++) {
         if (!dst[t]) continue;
@@ -650,7 +659,8 @@ static int plan_autotune_t(rtm_plan *q, int steps, int cap, int *shapes, double 
     Shape best = cand[0];
     int k = 0;
     for (const Shape &c : cand) {
-        if (check_store_shape(q->b.p.snapshot_mode, c.interior)) continue;   // not a shape of this store
+        // not a shape of this store or element type
+        if (check_store_shape(q->b.p.snapshot_mode, c.interior) || check_pair_real(q->b.real, c.interior)) continue;
         if (plan_set_tiling(q, c.rz, q->graph_block, c.interior, c.seg)) return -1;
         double t = 0.0;
         if (time_steps<R>(q->ex, {q->b.desc<R>()}, steps, q->graph_block, q->stream, &t)) return -1;

@@ -260,6 +260,10 @@ extern "C" int rtm_server_set_tiling(rtm_server *s, int rows_per_thread, int gra
 {
     if (!s || graph_block < 1) { rtm_set_error("rtm_server_set_tiling: bad argument"); return -1; }
     if (check_shape(rows_per_thread, interior) || check_store_shape(s->be.snapshot_mode, interior)) return -1;
+    if (interior == rtm::IN_PAIRS) {
+        rtm_set_error("the request server steps tile and column shapes (interior kernel 0-3), not pair items");
+        return -1;
+    }
     std::lock_guard<std::mutex> l(s->core.tick_mu);
     s->be.rz = rows_per_thread;
     s->core.block = graph_block;

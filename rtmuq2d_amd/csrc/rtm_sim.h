@@ -19,6 +19,7 @@
 #include "rtm_tile.h"
 #include "rtm_column.h"
 #include "rtm_work.h"
+#include "rtm_pair.h"
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
@@ -81,11 +82,11 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
     }
 }
 
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4>
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                          int x0, int z0, int tx, int ty)
 {
-    const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16>(S, n);
+    const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16, UM>(S, n);
     if (!V.active) return;
     rtm::TileRegs<R, RZ, NW> G;
     rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW>(G, S, V, x0, z0, tx, ty);
@@ -105,15 +106,15 @@ __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red,
     }
 }
 
-template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4>
+template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4, int UM = 1>
 __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                              int4 tile, int tx, int ty)
 {
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3, Q16, I16, NW>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 0: run_tile<R, RZ, 0, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    default: run_tile<R, RZ, 3, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
     }
 }
 
@@ -125,7 +126,9 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *
 // need not wait for it.
 // I16: every sim of the launch keeps an int16 block store (FP32 stepping).
 // NW: wavefronts per workgroup, one row group each (tiles of 64 x NW*RZ).
-template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4>
+// UM: u buffer mask, 1 (two buffers, in-place update) or 3 (four buffers:
+// the tiles beside two-step pair items).
+template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4, int UM = 1>
 __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                    const int *__restrict__ n0,
                                                    const int4 *__restrict__ tiles,
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict_
     const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    run_tile_cls<R, RZ, Q16, I16, NW>(L, red, S, n, tile, tx, ty);
+    run_tile_cls<R, RZ, Q16, I16, NW, UM>(L, red, S, n, tile, tx, ty);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run
@@ -369,6 +372,208 @@ __global__ void __launch_bounds__(256) k_columns(const SimDesc<R> *__restrict__ 
     column_item<R, Q16>(S, n0[sim] + s, it, item);
 }
 
+// ---- two steps per launch over the inner image region (rtm_pair.h) ----
+
+// Per-wavefront constants of a pair item.
+template <typename R>
+struct PairCtx {
+    const rtm::Pair<R> *u2, *O2, *C2;   // u^n, u^{n-1}, C
+    rtm::Pair<R> *U1, *U2;              // u^{n+1}, u^{n+2}
+    int Ph;                             // row pitch in pairs
+    int xh;                             // the lane's pair index (clamped into the row)
+    int xl;                             // the lane's first column
+    int zA, zB;                         // the item's rows
+    int ulast, clast;                   // last rows the u^n and C/u^{n-1} loads may touch
+    bool own;                           // the lane's columns are the item's
+};
+
+// Register windows of the march: row r of every window is slot
+// (r - zA + 8) mod NS, NS = 8 + 8 PD, where PD is the prefetch distance in
+// chunks: with the loop unrolled over NS/4 chunks every slot index is a
+// compile-time constant and nothing is copied between registers.
+template <int PD>
+constexpr int pair_slots() { return 8 + 8 * PD; }
+
+// Loads for chunk c (step-1 rows m0 = zA-4+4c .. m0+3): u^n rows m0+4 ..
+// m0+7 and C, u^{n-1} rows m0 .. m0+3, into the slots of chunk c = CI mod
+// NS/4.  Rows past the item are clamped, so every chunk issues the same
+// loads and waits for exactly those issued PD chunks earlier.
+template <typename R, int PD, int CI>
+__device__ __forceinline__ void pair_fetch(rtm::Pair<R> (&a)[pair_slots<PD>()], rtm::Pair<R> (&c)[pair_slots<PD>()],
+                                           rtm::Pair<R> (&o)[pair_slots<PD>()], const PairCtx<R> &X, int m0)
+{
+    constexpr int NS = pair_slots<PD>();
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int ku = m0 + 4 + rr < X.ulast ? m0 + 4 + rr : X.ulast;
+        const int kc = m0 + rr < X.clast ? m0 + rr : X.clast;
+        a[(8 + 4 * CI + rr) % NS] = rtm::ldg(X.u2 + ku * X.Ph, X.xh);
+        c[(4 + 4 * CI + rr) % NS] = rtm::ldg(X.C2 + kc * X.Ph, X.xh);
+        o[(4 + 4 * CI + rr) % NS] = rtm::ldg(X.O2 + kc * X.Ph, X.xh);
+    }
+}
+
+// x-neighbours of a lane's pair: lanes l-1, l-2, l+1, l+2 (a lane with no
+// neighbour keeps its own value; those lanes' results are never used)
+template <typename R>
+__device__ __forceinline__ void pair_shifts(rtm::Pair<R> c, rtm::Pair<R> &L1, rtm::Pair<R> &L2, rtm::Pair<R> &R1,
+                                            rtm::Pair<R> &R2)
+{
+    L1.a = lane_shr1(c.a, c.a);
+    L1.b = lane_shr1(c.b, c.b);
+    L2.a = lane_shr1(L1.a, L1.a);
+    L2.b = lane_shr1(L1.b, L1.b);
+    R1.a = lane_shl1(c.a, c.a);
+    R1.b = lane_shl1(c.b, c.b);
+    R2.a = lane_shl1(R1.a, R1.a);
+    R2.b = lane_shl1(R1.b, R1.b);
+}
+
+struct PairRun {
+    bool two;           // the second step runs
+    int srow1, srow2;   // source rows of the two steps (-1: none)
+    int clast;          // last chunk index
+};
+
+// Chunk c = CI mod NS/4: prefetch chunk c+PD, u^{n+1} on rows m0 .. m0+3
+// (kept in the window b, stored on the item's rows), then, from the third
+// chunk on, u^{n+2} on rows m0-4 .. m0-1.  The extras (source, receiver,
+// snapshot, imaging) are branched around on rows that carry none.
+template <typename R, int PD, bool IMG, int CI>
+__device__ __forceinline__ void pair_chunk(rtm::Pair<R> (&a)[pair_slots<PD>()], rtm::Pair<R> (&b)[pair_slots<PD>()],
+                                           rtm::Pair<R> (&c)[pair_slots<PD>()], rtm::Pair<R> (&o)[pair_slots<PD>()],
+                                           const PairCtx<R> &X, const SimDesc<R> &S, const rtm::StepView<R> &V1,
+                                           const rtm::StepView<R> &V2, const rtm::PairRec<R> &rec, const PairRun &P,
+                                           int cidx)
+{
+    constexpr int NS = pair_slots<PD>();
+    const int m0 = X.zA - 4 + 4 * cidx;
+    pair_fetch<R, PD, (CI + PD) % (NS / 4)>(a, c, o, X, m0 + 4 * PD);
+    const bool ex1 = (P.srow1 >= m0 && P.srow1 < m0 + 4) || (S.rec_row >= m0 && S.rec_row < m0 + 4)
+                     || (V1.snap && m0 + 4 > X.zA && m0 < X.zB);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int m = m0 + rr;
+        rtm::Pair<R> win[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) win[j] = a[(4 * CI + rr + j) % NS];
+        rtm::Pair<R> L1, L2, R1, R2;
+        pair_shifts(win[4], L1, L2, R1, R2);
+        const int sm = (4 + 4 * CI + rr) % NS;
+        rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, c[sm], o[sm]);
+        const bool mine = m >= X.zA && m < X.zB;   // uniform
+        if (ex1) rtm::pair_extras<R, IMG>(S, V1, rec, false, m, X.xl, X.own && mine, P.srow1, U, c[sm]);
+        b[sm] = U;
+        if (mine && X.own) rtm::stg(X.U1 + m * X.Ph, X.xh, U);
+    }
+    if (!P.two || cidx < 2) return;
+    const int q0 = m0 - 4;
+    const bool ex2 = (P.srow2 >= q0 && P.srow2 < q0 + 4) || (S.rec_row >= q0 && S.rec_row < q0 + 4) || V2.snap;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        const int q = q0 + rr;
+        rtm::Pair<R> win[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) win[j] = b[(4 * CI + rr + NS - 4 + j) % NS];
+        rtm::Pair<R> L1, L2, R1, R2;
+        pair_shifts(win[4], L1, L2, R1, R2);
+        const int sq = (4 * CI + rr) % NS;
+        rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, c[sq], a[sq]);
+        if (ex2) rtm::pair_extras<R, IMG>(S, V2, rec, true, q, X.xl, X.own, P.srow2, U, c[sq]);
+        if (X.own) rtm::stg(X.U2 + q * X.Ph, X.xh, U);
+    }
+}
+
+// One turn of the window cycle: chunks c0 .. c0 + NS/4 - 1; true when the
+// item's last chunk has run.
+template <typename R, int PD, bool IMG, int CI>
+__device__ __forceinline__ bool pair_cycle(rtm::Pair<R> (&a)[pair_slots<PD>()], rtm::Pair<R> (&b)[pair_slots<PD>()],
+                                           rtm::Pair<R> (&c)[pair_slots<PD>()], rtm::Pair<R> (&o)[pair_slots<PD>()],
+                                           const PairCtx<R> &X, const SimDesc<R> &S, const rtm::StepView<R> &V1,
+                                           const rtm::StepView<R> &V2, const rtm::PairRec<R> &rec, const PairRun &P,
+                                           int c0)
+{
+    pair_chunk<R, PD, IMG, CI>(a, b, c, o, X, S, V1, V2, rec, P, c0 + CI);
+    if (c0 + CI == P.clast) return true;
+    if constexpr (CI + 1 < pair_slots<PD>() / 4)
+        return pair_cycle<R, PD, IMG, CI + 1>(a, b, c, o, X, S, V1, V2, rec, P, c0);
+    else return false;
+}
+
+// One pair item (the body of k_pairs) for a sim with four u buffers at
+// steps n and, when `two`, n+1.  IMG: an instance for items whose steps
+// image (backward, snapshot steps), the only one that loads inside the march.
+template <typename R, int PD, bool IMG>
+__device__ __forceinline__ void pair_march(const SimDesc<R> &S, const rtm::StepView<R> &V1, const rtm::StepView<R> &V2,
+                                           bool two, int4 it)
+{
+    constexpr int NS = pair_slots<PD>();
+    PairRun P;
+    P.two = two;
+    const int lane = threadIdx.x & 63;
+    const int wp = rtm::pair_wp(it.x);
+    PairCtx<R> X;
+    X.u2 = (const rtm::Pair<R> *)V1.u;
+    X.O2 = (const rtm::Pair<R> *)V1.Uo;
+    X.C2 = (const rtm::Pair<R> *)S.C;
+    X.U1 = (rtm::Pair<R> *)V1.U;
+    X.U2 = (rtm::Pair<R> *)V2.U;
+    X.Ph = S.P >> 1;
+    X.xl = it.y - 8 + 2 * lane;
+    X.xh = rtm::clampi(X.xl, 0, S.P - 2) >> 1;
+    X.zA = it.z;
+    X.zB = it.w;
+    X.ulast = X.zB + 7;
+    X.clast = X.zB + 3;
+    X.own = lane >= 4 && lane < 4 + wp;
+    P.srow1 = rtm::col_src_row(S, V1);
+    P.srow2 = P.two ? rtm::col_src_row(S, V2) : -1;
+    P.clast = (X.zB - X.zA) / 4 + 1;   // chunks 0 .. clast
+    const rtm::PairRec<R> rec =
+        rtm::pair_rec(S, V1, V2, P.two, X.xl, S.rec_row >= X.zA - 4 && S.rec_row < X.zB + 4);
+    rtm::Pair<R> a[NS], b[NS], c[NS], o[NS];
+#pragma unroll
+    for (int r = 0; r < 8; r++) a[r] = rtm::ldg(X.u2 + (X.zA - 8 + r) * X.Ph, X.xh);
+    pair_fetch<R, PD, 0>(a, c, o, X, X.zA - 4);
+    if constexpr (PD >= 2) pair_fetch<R, PD, 1>(a, c, o, X, X.zA);
+    if constexpr (PD >= 3) pair_fetch<R, PD, 2>(a, c, o, X, X.zA + 4);
+    for (int c0 = 0;; c0 += NS / 4)
+        if (pair_cycle<R, PD, IMG, 0>(a, b, c, o, X, S, V1, V2, rec, P, c0)) break;
+}
+
+template <typename R, int PD>
+__device__ __forceinline__ void pair_item(const SimDesc<R> &S, int n, bool two, int4 it)
+{
+    const rtm::StepView<R> V1 = rtm::step_view_q<R, false, false, 3>(S, n);
+    if (!V1.active) return;
+    const rtm::StepView<R> V2 = rtm::step_view_q<R, false, false, 3>(S, n + 1);
+    two = two && V2.active;
+    if (!V1.forward && (V1.snap || (two && V2.snap))) pair_march<R, PD, true>(S, V1, V2, two, it);
+    else pair_march<R, PD, false>(S, V1, V2, two, it);
+}
+
+// Two steps (s and s+1 of the block) of every pair item, four independent
+// wavefronts per workgroup; the second step only when s+1 < lim (the steps
+// of the block or of the eager remainder).  XCD-contiguous runs of items as
+// in k_columns.
+template <typename R, bool ONE, int PD>
+__global__ void __launch_bounds__(256) k_pairs(const SimDesc<R> *__restrict__ sims, const int *__restrict__ n0,
+                                               const int4 *__restrict__ items, const int *__restrict__ nitems_p,
+                                               int s, int lim)
+{
+    const int nitems = *nitems_p;
+    const int nblk = (nitems + 3) >> 2;
+    const int per = (nblk + 7) >> 3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if ((b >> 3) >= per || t >= nblk) return;
+    const int item = t * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int4 it = items[item];
+    const int sim = ONE ? 0 : rtm::pair_sim(it.x);
+    pair_item<R, PD>(sims[sim], n0[sim] + s, s + 1 < lim, it);
+}
+
 // Boundary saving, forward: after step n, U = u^{n+1} on the ring (rtm::ring_point)
 // into ring[n][*].  grid = (ring blocks, sims).
 template <typename R>
@@ -579,7 +784,10 @@ struct SimBuffers {
     size_t esz = 4;
     int X = 0, Z = 0, P = 0;
     size_t nfield = 0, nint = 0, nsnap = 0;
-    void *u[2] = {nullptr, nullptr};
+    // u^n is u[n & umask]: two buffers, or four for two-step pair launches
+    // (u[2], u[3] are allocated when a pair shape is first set)
+    void *u[4] = {nullptr, nullptr, nullptr, nullptr};
+    int umask = 1;
     void *C = nullptr;
     void *psiX[2] = {nullptr, nullptr}, *psiZ[2] = {nullptr, nullptr};
     void *csiX = nullptr, *csiZ = nullptr;
@@ -613,6 +821,9 @@ struct SimBuffers {
     // shot gather)
     int alloc(const rtm_params &pp, int real_type, const SimBuffers *share = nullptr);
     void release();
+    // step with four u buffers (pair shapes) or two; the extra buffers stay
+    // allocated once made, and are zeroed by reset_state while in use
+    int set_quad(bool on);
     int upload_profiles();
     int upload_receivers();
     // uploads; with wait == false the host data must stay alive until the
@@ -666,6 +877,9 @@ int check_shape(int rz, int interior);
 // 0 if the snapshot store (enum rtm_snapshot_mode) runs with that interior
 // kernel: the int16 block store runs in tile shapes only
 int check_store_shape(int snapshot_mode, int interior);
+// 0 if a plan or batch of this element type runs with that interior kernel:
+// two-step pair items step FP32 fields only
+int check_pair_real(int real, int interior);
 
 struct Exec {
     int real = RTM_REAL_F32;
@@ -716,7 +930,10 @@ struct Exec {
     // `steps` launches from the current n0 (eager, or in graph blocks of
     // `block` steps when block > 1); n0 advances by `steps` for all sims
     int run(int steps, int block, hipStream_t s);
-    int launch_one(int s_in_block, hipStream_t s);
+    // the launch of step s_in_block of a block; with pair items (IN_PAIRS)
+    // an even s_in_block also steps them at s_in_block + 1 when that is
+    // below `lim` (the block's length)
+    int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30);
     // every captured graph, once the stream they ran on has drained
     int drop_graph();
     // the boundary-saving kernel of the coming pass (0: none; its grid

@@ -216,7 +216,8 @@ void SimBuffers::release()
         else (void)hipFree(ptr);
     }
     owned.clear();
-    u[0] = u[1] = C = psiX[0] = psiX[1] = psiZ[0] = psiZ[1] = csiX = csiZ = nullptr;
+    u[0] = u[1] = u[2] = u[3] = C = psiX[0] = psiX[1] = psiZ[0] = psiZ[1] = csiX = csiZ = nullptr;
+    umask = 1;
     ax = bx = az = bz = nullptr;
     recmap = rec_j = out_j = out_r = nullptr;
     wav = seis_t = seis_rec = nullptr;
@@ -363,16 +364,43 @@ int SimBuffers::set_seismogram(const float *d, hipStream_t s, bool wait)
     return 0;
 }
 
+int SimBuffers::set_quad(bool on)
+{
+    if (on && !u[3]) {
+        const size_t fb = nfield * esz;
+        for (int t = 2; t < 4; t++) {
+            if (arena) {
+                u[t] = arena->alloc(fb);
+                if (!u[t]) {
+                    rtm_set_error("device arena: no free block of %zu bytes", fb);
+                    return -1;
+                }
+            } else {
+                hipError_t e = hipMalloc(&u[t], fb);
+                if (e != hipSuccess) {
+                    u[t] = nullptr;
+                    rtm_set_error("hipMalloc(%zu) failed: %s", fb, hipGetErrorString(e));
+                    return -1;
+                }
+            }
+            owned.push_back(u[t]);
+        }
+    }
+    umask = on ? 3 : 1;
+    return 0;
+}
+
 void SimBuffers::reset_state(hipStream_t s, bool source_field)
 {
     void *w0 = source_field ? sbuf[0] : u[0], *w1 = source_field ? sbuf[1] : u[1];
+    void *w2 = !source_field && umask == 3 ? u[2] : nullptr, *w3 = !source_field && umask == 3 ? u[3] : nullptr;
     if (real == RTM_REAL_F64) {
-        double *f[] = {(double *)w0, (double *)w1, (double *)psiX[0], (double *)psiX[1],
-                       (double *)psiZ[0], (double *)psiZ[1], (double *)csiX, (double *)csiZ};
+        double *f[] = {(double *)w0, (double *)w1, (double *)w2, (double *)w3, (double *)psiX[0],
+                       (double *)psiX[1], (double *)psiZ[0], (double *)psiZ[1], (double *)csiX, (double *)csiZ};
         for (double *ptr : f) fill_zero(ptr, nfield, s);
     } else {
-        float *f[] = {(float *)w0, (float *)w1, (float *)psiX[0], (float *)psiX[1],
-                      (float *)psiZ[0], (float *)psiZ[1], (float *)csiX, (float *)csiZ};
+        float *f[] = {(float *)w0, (float *)w1, (float *)w2, (float *)w3, (float *)psiX[0],
+                      (float *)psiX[1], (float *)psiZ[0], (float *)psiZ[1], (float *)csiX, (float *)csiZ};
         for (float *ptr : f) fill_zero(ptr, nfield, s);
     }
 }
@@ -403,7 +431,7 @@ int SimBuffers::flush_last_snapshot(hipStream_t s)
     if (qmode == rtm::Q_NONE || qmode == rtm::Q_I16 || (nt - 1) % TS != 0) return 0;
     const int c = (nt - 1) / TS;
     const dim3 grid((p.nx + 255) / 256, p.nz);
-    hipLaunchKernelGGL(k_quantize_interior<float>, grid, dim3(256), 0, s, (const float *)u[nt & 1], P,
+    hipLaunchKernelGGL(k_quantize_interior<float>, grid, dim3(256), 0, s, (const float *)u[nt & umask], P,
                        p.thickness_left, p.thickness_up, p.nx, p.nz, q16 + (size_t)c * nint,
                        smax + (size_t)c * rtm::kSnapWords, qmode);
     HIP_TRY(hipGetLastError());
@@ -442,7 +470,8 @@ SimDesc<R> SimBuffers::desc() const
         const double h = p.spacing_grid;
         d.h1 = R(1.0 / h); d.h2 = R(h * h); d.h3 = R(1.0 / (h * h));
     }
-    d.ubuf[0] = (R *)u[0]; d.ubuf[1] = (R *)u[1];
+    for (int t = 0; t < 4; t++) d.ubuf[t] = (R *)u[t];
+    d.umask = umask;
     d.C = (const R *)C;
     d.ax = (const R *)ax; d.bx = (const R *)bx; d.az = (const R *)az; d.bz = (const R *)bz;
     d.psiX[0] = (R *)psiX[0]; d.psiX[1] = (R *)psiX[1];
@@ -501,9 +530,9 @@ int check_shape(int rz, int interior)
         return -1;
     }
     if (interior != rtm::IN_TILES && interior != rtm::IN_COLUMNS && interior != rtm::IN_MIXED &&
-        interior != rtm::IN_TILES8) {
-        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns), 2 (mixed tiles) or 3 (tiles of 8 "
-                      "wavefronts) (got %d)", interior);
+        interior != rtm::IN_TILES8 && interior != rtm::IN_PAIRS) {
+        rtm_set_error("interior kernel must be 0 (tiles), 1 (columns), 2 (mixed tiles), 3 (tiles of 8 "
+                      "wavefronts) or 4 (two-step pair items) (got %d)", interior);
         return -1;
     }
     if ((interior == rtm::IN_MIXED || interior == rtm::IN_TILES8) && rz != 2 && rz != 4) {
@@ -734,12 +763,59 @@ int check_store_shape(int snapshot_mode, int interior)
                       interior);
         return -1;
     }
+    if (interior == rtm::IN_PAIRS && snapshot_mode != RTM_SNAP_FP32) {
+        rtm_set_error("two-step pair items (interior kernel 4) run with the FP32 snapshot store only");
+        return -1;
+    }
     return 0;
 }
 
-int Exec::launch_one(int s_in_block, hipStream_t s)
+int check_pair_real(int real, int interior)
 {
-    if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
+    if (interior == rtm::IN_PAIRS && real != RTM_REAL_F32) {
+        rtm_set_error("two-step pair items (interior kernel 4) step FP32 fields only");
+        return -1;
+    }
+    return 0;
+}
+
+// Launch of step s of a block with pair items: the pair items at s and s+1
+// (an even s), then the tiles of the rest of the grid at s (k_tiles over
+// four u buffers).  FP32, FP32 store (check_store_shape, check_pair_real).
+static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s)
+{
+    const SimDesc<float> *d = (const SimDesc<float> *)e.d_desc;
+    const bool one = e.cap_sims == 1;
+    if (e.ninner && (s_in_block & 1) == 0) {
+        const int grid = 8 * (((e.cap_inner + 3) / 4 + 7) / 8);
+        // prefetch distance 1: distances 2 and 3 measured the same on the c2
+        // launch group (the march is not waiting on its loads)
+        if (one) hipLaunchKernelGGL((k_pairs<float, true, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block, lim);
+        else hipLaunchKernelGGL((k_pairs<float, false, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block, lim);
+    }
+    if (!e.ntiles) return;
+    const int grid = 8 * ((e.cap_tiles + 7) / 8);
+    if (one) {
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
+    } else {
+        switch (e.rz) {
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        }
+    }
+}
+
+int Exec::launch_one(int s_in_block, hipStream_t s, int lim)
+{
+    if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s);
+    else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
     else if (i16) launch_tiles_i16(*this, s_in_block, s);
     else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
     else launch_tiles<float, false>(*this, s_in_block, s);
@@ -776,7 +852,7 @@ int Exec::run(int steps, int block, hipStream_t s)
             if (c.exec && g_stream) (void)hipStreamSynchronize(g_stream);
             destroy_captured(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int t = 0; t < block; t++) launch_one(t, s);
+            for (int t = 0; t < block; t++) launch_one(t, s, block);
             hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
             hipError_t e = hipStreamEndCapture(s, &c.graph);
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
@@ -793,7 +869,7 @@ int Exec::run(int steps, int block, hipStream_t s)
         }
     }
     const int rem = steps - done;
-    for (int t = 0; t < rem; t++) launch_one(t, s);
+    for (int t = 0; t < rem; t++) launch_one(t, s, rem);
     if (rem) hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, rem);
     HIP_TRY(hipGetLastError());
     for (int i = 0; i < cap_sims; i++) h_n0[i] += steps;
@@ -811,6 +887,7 @@ const std::vector<Shape> &tuner_candidates()
         {4, rtm::IN_COLUMNS, 0}, {4, rtm::IN_COLUMNS, 64}, {2, rtm::IN_COLUMNS, 32}, {1, rtm::IN_COLUMNS, 8},
         {4, rtm::IN_MIXED, 0}, {2, rtm::IN_MIXED, 0},
         {2, rtm::IN_TILES8, 0}, {4, rtm::IN_TILES8, 0},
+        {2, rtm::IN_PAIRS, 24},
     };
     return c;
 }

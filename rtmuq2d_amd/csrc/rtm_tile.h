@@ -104,7 +104,11 @@ struct SimDesc {
     int src_j;              /* device index of the source, -1 if none */
     int rec_row;
     R h1, h2, h3;
-    R *ubuf[2];             /* u at step n is ubuf[n & 1]              */
+    /* u at step n is ubuf[n & umask]: two buffers (umask 1, the update
+     * overwrites u^{n-1} in place) or four (umask 3, two-step launches:
+     * a launch reads two time levels and writes the next two) */
+    R *ubuf[4];
+    int umask;
     const R *C;
     const R *ax, *bx, *az, *bz;
     R *psiX[2], *psiZ[2];   /* psi^n is psi?[n & 1]                    */
@@ -332,7 +336,8 @@ template <typename R>
 struct StepView {
     int n, active, forward;
     const R *u;
-    R *U;
+    R *U;                   /* u^{n+1} written */
+    const R *Uo;            /* u^{n-1} read (U itself with two buffers) */
     const R *psiXi, *psiZi;
     R *psiXo, *psiZo;
     int src;                /* add the source this step */
@@ -353,7 +358,7 @@ struct StepView {
 /* Q16: the sim keeps a 16-bit snapshot store (SimDesc::qmode != 0).  The
  * kernels dispatch on it per sim, so the FP32-store code carries none of the
  * 16-bit fields. */
-template <typename R, bool Q16, bool I16 = false>
+template <typename R, bool Q16, bool I16 = false, int UM = 1>
 RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 {
     StepView<R> v{};
@@ -361,8 +366,9 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
     v.active = S.phase != PH_IDLE && n >= 0 && n < S.nt;
     if (!v.active) return v;
     v.forward = S.phase == PH_FORWARD;
-    v.u = S.ubuf[n & 1];
-    v.U = S.ubuf[(n + 1) & 1];
+    v.u = S.ubuf[n & UM];
+    v.U = S.ubuf[(n + 1) & UM];
+    v.Uo = UM == 1 ? v.U : S.ubuf[(n - 1) & UM];
     v.psiXi = S.psiX[n & 1];
     v.psiXo = S.psiX[(n + 1) & 1];
     v.psiZi = S.psiZ[n & 1];
@@ -415,6 +421,7 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 template <typename R>
 RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
 {
+    if (S.umask == 3) return step_view_q<R, false, false, 3>(S, n);
     return S.qmode == Q_I16 ? step_view_q<R, false, true>(S, n)
            : S.qmode        ? step_view_q<R, true>(S, n)
                             : step_view_q<R, false>(S, n);
@@ -497,7 +504,7 @@ RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepVi
         const int k = clampi(z0 + ty * RZ + rr, 0, S.Z - 1);
         const int j = k * S.P + ic;
         G.C[rr] = ldg(S.C, j);
-        G.Uold[rr] = ldg(V.U, j);
+        G.Uold[rr] = ldg(V.Uo, j);
         if (CLS & T_XSTRIP) G.csx[rr] = ldg(S.csiX, j);
         if (CLS & T_ZSTRIP) G.csz[rr] = ldg(S.csiZ, j);
     }

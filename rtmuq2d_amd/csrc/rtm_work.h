@@ -16,7 +16,11 @@
  *   IN_MIXED    class-0 tiles stay 64 x TZ and every other tile is cut into
  *               64 x 4 tiles (class | T_SMALL, the class of the small tile's
  *               own rows), all in one tile launch: the strip tiles' longer
- *               chain runs on less work per workgroup.
+ *               chain runs on less work per workgroup;
+ *   IN_PAIRS    the tiles of the two-step region D2 (rtm_pair.h) become
+ *               pair items of up to 112 columns in segments of at most
+ *               seg_rows rows, stepped two steps per launch; every other
+ *               tile stays a tile.
  */
 #ifndef RTM_WORK_H
 #define RTM_WORK_H
@@ -24,11 +28,11 @@
 #include <algorithm>
 #include <vector>
 
-#include "rtm_column.h"
+#include "rtm_pair.h"
 
 namespace rtm {
 
-enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1, IN_MIXED = 2, IN_TILES8 = 3 };
+enum Interior : int { IN_TILES = 0, IN_COLUMNS = 1, IN_MIXED = 2, IN_TILES8 = 3, IN_PAIRS = 4 };
 
 /* rows of a tile of interior kind `interior` at rz rows per thread */
 inline int tile_rows(int rz, int interior) { return (interior == IN_TILES8 ? 8 : 4) * rz; }
@@ -45,6 +49,13 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
     d.X = X; d.Z = Z; d.TL = TL; d.TR = TR; d.TU = TU; d.TD = TD;
     int seg = 0;
     if (interior == IN_COLUMNS) seg = seg_rows > 0 ? (seg_rows + 3) / 4 * 4 : 1 << 30;
+    PairRegion D2 = interior == IN_PAIRS ? pair_region(X, Z, TL, TR, TU, TD, TZ) : PairRegion{0, 0, 0, 0};
+    /* the pair items' bounds hold at the region's corners, hence for all */
+    if (!D2.empty() && !(pair_item_ok(1, D2.x0, D2.z0, D2.z1, X, Z) &&
+                         pair_item_ok((D2.x1 - D2.x0 - 1) % kPairCols / 2 + 1,
+                                      D2.x0 + (D2.x1 - D2.x0 - 1) / kPairCols * kPairCols, D2.z0, D2.z1, X, Z)))
+        D2 = PairRegion{0, 0, 0, 0};
+    auto in_d2 = [&](int x0, int z0) { return x0 >= D2.x0 && x0 < D2.x1 && z0 >= D2.z0 && z0 < D2.z1; };
     /* Items are listed x fastest (a row of tiles, then the next row), so the
      * workgroups an XCD runs at the same time cover whole grid rows: every
      * DRAM page opened for a row is read through, not 256 bytes of it. */
@@ -66,6 +77,7 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
         for (int x0 = kHalf; x0 < X - kHalf; x0 += 64, t++) {
             const Run &r = runs[t];
             if (r.zA >= 0 && z0 >= r.zA && z0 < r.zB) continue;
+            if (!D2.empty() && in_d2(x0, z0)) continue;
             const int cls = tile_class(d, x0, z0, TZ);
             if (interior == IN_MIXED && cls != 0) {
                 for (int zs = z0; zs < z0 + TZ && zs < Z - kHalf; zs += 4)
@@ -86,6 +98,18 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
                           && runs[t + 1].zB == r.zB;
         cols.push_back({r.x0, r.zA, r.zB, pair ? 0 : 1});
         if (pair) t++;
+    }
+    if (interior == IN_PAIRS && !D2.empty()) {
+        /* segment-major, x fastest: the four items of a workgroup and the
+         * workgroups an XCD runs together share their halo columns */
+        const int ps = seg_rows > 0 ? (seg_rows + 3) / 4 * 4 : D2.z1 - D2.z0;
+        for (int z = D2.z0; z < D2.z1; z += ps) {
+            const int e = z + ps < D2.z1 ? z + ps : D2.z1;
+            for (int x0 = D2.x0; x0 < D2.x1; x0 += kPairCols) {
+                const int w = D2.x1 - x0 < kPairCols ? D2.x1 - x0 : kPairCols;
+                inner.push_back({pair_pack(sim, w / 2), x0, z, e});
+            }
+        }
     }
     if (interior != IN_COLUMNS) return;
     int zmax = 0;

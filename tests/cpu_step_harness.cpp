@@ -123,10 +123,80 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
     commit_max(V, umax);
 }
 
-// One step of one simulation over the work items of the library's builder
-// (rtm::build_work): tiles, plus column interior items.
+// Emulation of k_pairs for one item {sim | wp << 24, x0, zA, zB}: u^{n+1}
+// on rows zA-4 .. zB+3 for all 64 lanes, then u^{n+2} on rows zA .. zB-1,
+// with the DPP lane shifts done as array shifts (a lane with no neighbour
+// keeps its own value, as the kernel's shifts do).  `two`: the second step
+// runs (it is part of the launch and the simulation is still active).
+template <typename R>
+static void run_pair_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V1, const rtm::StepView<R> &V2,
+                          bool two, int packed, int x0, int zA, int zB)
+{
+    using P2 = rtm::Pair<R>;
+    const int P = S.P, wp = rtm::pair_wp(packed);
+    const int srow1 = rtm::col_src_row(S, V1), srow2 = two ? rtm::col_src_row(S, V2) : -1;
+    const bool on_row = S.rec_row >= zA - 4 && S.rec_row < zB + 4;
+    const int m_lo = zA - 4, m_hi = zB + 4;
+    std::vector<P2> U1((size_t)(m_hi - m_lo) * 64);
+    auto row1 = [&](int m) { return &U1[(size_t)(m - m_lo) * 64]; };
+    int xl[64], xc[64];
+    for (int l = 0; l < 64; l++) {
+        xl[l] = x0 - 8 + 2 * l;
+        xc[l] = rtm::clampi(xl[l], 0, P - 2);
+    }
+    auto shifts = [](const P2 *c, P2 *L1, P2 *L2, P2 *R1, P2 *R2) {
+        for (int l = 0; l < 64; l++) L1[l] = l > 0 ? c[l - 1] : c[l];
+        for (int l = 0; l < 64; l++) L2[l] = l > 0 ? L1[l - 1] : L1[l];
+        for (int l = 0; l < 64; l++) R1[l] = l < 63 ? c[l + 1] : c[l];
+        for (int l = 0; l < 64; l++) R2[l] = l < 63 ? R1[l + 1] : R1[l];
+    };
+    auto at = [&](const R *f, int k, int x) { return P2{f[(size_t)k * P + x], f[(size_t)k * P + x + 1]}; };
+    for (int m = m_lo; m < m_hi; m++) {
+        P2 c[64], L1[64], L2[64], R1[64], R2[64];
+        for (int l = 0; l < 64; l++) c[l] = at(V1.u, m, xc[l]);
+        shifts(c, L1, L2, R1, R2);
+        for (int l = 0; l < 64; l++) {
+            P2 w[9];
+            for (int r = 0; r < 9; r++) w[r] = at(V1.u, m - 4 + r, xc[l]);
+            const P2 Cp = at(S.C, m, xc[l]);
+            P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, at(V1.Uo, m, xc[l]));
+            const bool own = l >= 4 && l < 4 + wp && m >= zA && m < zB;
+            rtm::pair_extras<R, true>(S, V1, rtm::pair_rec(S, V1, V2, two, xl[l], on_row), false, m, xl[l], own,
+                                      srow1, U, Cp);
+            row1(m)[l] = U;
+            if (own) {
+                V1.U[(size_t)m * P + xl[l]] = U.a;
+                V1.U[(size_t)m * P + xl[l] + 1] = U.b;
+            }
+        }
+    }
+    if (!two) return;
+    for (int q = zA; q < zB; q++) {
+        P2 L1[64], L2[64], R1[64], R2[64];
+        shifts(row1(q), L1, L2, R1, R2);
+        for (int l = 0; l < 64; l++) {
+            P2 w[9];
+            for (int r = 0; r < 9; r++) w[r] = row1(q - 4 + r)[l];
+            const P2 Cp = at(S.C, q, xc[l]);
+            P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, at(V1.u, q, xc[l]));
+            const bool own = l >= 4 && l < 4 + wp;
+            rtm::pair_extras<R, true>(S, V2, rtm::pair_rec(S, V1, V2, two, xl[l], on_row), true, q, xl[l], own,
+                                      srow2, U, Cp);
+            if (own) {
+                V2.U[(size_t)q * P + xl[l]] = U.a;
+                V2.U[(size_t)q * P + xl[l] + 1] = U.b;
+            }
+        }
+    }
+}
+
+// One launch of one simulation over the work items of the library's builder
+// (rtm::build_work): tiles, plus column interior items, or (interior
+// IN_PAIRS) the pair items at steps n and n+1 when `pair_launch` (the first
+// launch of a pair; `two`: its second step is part of the run).
 template <typename R, int RZ, bool Q16, bool I16 = false>
-static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
+static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows, int pair_launch = 0,
+                        bool two = false)
 {
     const rtm::StepView<R> V = rtm::step_view(S, n);
     if (!V.active) return;
@@ -165,7 +235,14 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
         default: run_tile<R, RZ, 3, Q16, I16>(*L, G, S, V, t.b, t.c); break;
         }
     }
-    for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
+    if (interior == rtm::IN_PAIRS) {
+        if (pair_launch) {
+            const rtm::StepView<R> V2 = rtm::step_view(S, n + 1);
+            for (const auto &it : inner) run_pair_item<R>(S, V, V2, two && V2.active, it.a, it.b, it.c, it.d);
+        }
+    } else {
+        for (const auto &it : inner) run_column_item<R, Q16>(S, V, it.a, it.b, it.c, it.d);
+    }
     delete[] Gs;
     delete Ls;
     delete[] G8;
@@ -175,11 +252,11 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
 }
 
 template <typename R, int RZ>
-static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows)
+static void run_tiles(const rtm::SimDesc<R> &S, int n, int interior, int seg_rows, int pair_launch, bool two)
 {
     if (S.qmode == rtm::Q_I16) run_tiles_q<R, RZ, false, true>(S, n, interior, seg_rows);
     else if (S.qmode) run_tiles_q<R, RZ, true>(S, n, interior, seg_rows);
-    else run_tiles_q<R, RZ, false>(S, n, interior, seg_rows);
+    else run_tiles_q<R, RZ, false>(S, n, interior, seg_rows, pair_launch, two);
 }
 
 template <typename R>
@@ -203,16 +280,26 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
     }
     S.h1 = hs[0]; S.h2 = hs[1]; S.h3 = hs[2];
     S.ubuf[0] = ptr[0]; S.ubuf[1] = ptr[1]; S.C = ptr[2];
+    S.umask = 1;
+    // two-step launches (IN_PAIRS): four u buffers; g[22] = 1 for the first
+    // launch of a pair, g[23] = 1 when its second step is part of the run
+    const int pair_launch = interior == rtm::IN_PAIRS ? g[22] : 0;
+    const bool two = interior == rtm::IN_PAIRS && g[23];
+    if (interior == rtm::IN_PAIRS) {
+        S.ubuf[2] = ptr[20];
+        S.ubuf[3] = ptr[21];
+        S.umask = 3;
+    }
     S.ax = ptr[3]; S.bx = ptr[4]; S.az = ptr[5]; S.bz = ptr[6];
     S.psiX[0] = ptr[7]; S.psiX[1] = ptr[8]; S.psiZ[0] = ptr[9]; S.psiZ[1] = ptr[10];
     S.csiX = ptr[11]; S.csiZ = ptr[12];
     S.snaps = ptr[13]; S.I = ptr[14]; S.Is = ptr[15]; S.Ic = ptr[16]; S.Isc = ptr[17];
     S.recmap = recmap; S.wav = wav; S.seis_t = seis_t; S.seis_rec = seis_rec;
     switch (rz) {
-    case 1: run_tiles<R, 1>(S, n, interior, seg_rows); break;
-    case 2: run_tiles<R, 2>(S, n, interior, seg_rows); break;
-    case 8: run_tiles<R, 8>(S, n, interior, seg_rows); break;
-    default: run_tiles<R, 4>(S, n, interior, seg_rows); break;
+    case 1: run_tiles<R, 1>(S, n, interior, seg_rows, pair_launch, two); break;
+    case 2: run_tiles<R, 2>(S, n, interior, seg_rows, pair_launch, two); break;
+    case 8: run_tiles<R, 8>(S, n, interior, seg_rows, pair_launch, two); break;
+    default: run_tiles<R, 4>(S, n, interior, seg_rows, pair_launch, two); break;
     }
 }
 
@@ -228,6 +315,20 @@ int hx_work_list(int X, int Z, int TL, int TR, int TU, int TD, int rz, int inter
         out[4 * i] = tiles[i].a; out[4 * i + 1] = tiles[i].b; out[4 * i + 2] = tiles[i].c; out[4 * i + 3] = tiles[i].d;
     }
     return (int)tiles.size();
+}
+
+/* rtm::build_work's interior items (column or pair items) for one grid as
+ * {packed sim, x0, zA, zB} quadruples (capacity `cap` items); returns its
+ * length or -1. */
+int hx_inner_list(int X, int Z, int TL, int TR, int TU, int TD, int rz, int interior, int seg_rows, int *out, int cap)
+{
+    std::vector<rtm::WorkItem> tiles, inner;
+    rtm::build_work(tiles, inner, 0, X, Z, TL, TR, TU, TD, rz, interior, seg_rows);
+    if ((int)inner.size() > cap) return -1;
+    for (size_t i = 0; i < inner.size(); i++) {
+        out[4 * i] = inner[i].a; out[4 * i + 1] = inner[i].b; out[4 * i + 2] = inner[i].c; out[4 * i + 3] = inner[i].d;
+    }
+    return (int)inner.size();
 }
 
 /* rtm::balance_for_xcds over the tile lists of nsims same-sized grids (sim

@@ -19,6 +19,7 @@ import oracle as O
 
 M_STENCIL, M_CPML, M_SOURCE, M_INJECT, M_SNAP, M_IMAGE, M_KAHAN = 1, 2, 4, 8, 16, 32, 64
 SNAP_WORDS = 32 * 64   # max-abs slots per snapshot (rtm::kSnapWords)
+IN_PAIRS = 4           # rtm::IN_PAIRS, two-step launches over the inner image region
 Q_I16 = 3              # rtm::Q_I16, the int16 block store
 
 
@@ -81,6 +82,7 @@ class DevSim:
         self.psiZ = [z(), z()]
         self.csiX, self.csiZ = z(), z()
         self.cur = self.pc = 0
+        self.last = None
 
     def step(self, mode, src_j=-1, src_w=0.0, seis_row=None, snap=None, I=None, Is=None, Ic=None,
              Isc=None, edges=False):
@@ -134,12 +136,14 @@ class DevSim:
         return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)
 
     def last_fields(self):
+        if self.last is not None:
+            return self.from_dev(self.last[0]), self.from_dev(self.last[1])
         return self.from_dev(self.u[self.cur ^ 1]), self.from_dev(self.u[self.cur])
 
     # ---- the tiled kernel (rtm_tile.h), emulated phase by phase ----
 
     def _tiles_pass(self, phase, flags, rz, src_j=-1, wav=None, seis=None, snaps=None, record=False,
-                    interior=0, seg=0, qmode=0, q16=None, smax=None):
+                    interior=0, seg=0, qmode=0, q16=None, smax=None, block=0):
         op = self.op
         self.reset()
         nint = op.nx * op.nz
@@ -156,17 +160,30 @@ class DevSim:
         if q16 is None:
             q16 = np.zeros((op.nt // TS, op.nz, op.nx), np.uint16)
             smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
+        pairs = interior == IN_PAIRS
+        if pairs:               # two-step launches step four u buffers (u^n in u[n & 3])
+            self.u += [np.zeros((self.Z, self.P), self.dt), np.zeros((self.Z, self.P), self.dt)]
         arrs = [self.u[0], self.u[1], self.C, self.ax, self.bx, self.az, self.bz, self.psiX[0],
                 self.psiX[1], self.psiZ[0], self.psiZ[1], self.csiX, self.csiZ, snaps, I, Is, Ic, Isc,
-                q16, smax]
+                q16, smax] + (self.u[2:4] if pairs else [])
         ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
         hs = self.hs[:3].copy()
+        # launches as the executor issues them: with two-step launches the
+        # first of each pair steps the pair items at n and n+1, and `block`
+        # (when given) splits the run into blocks (an odd block ends on a
+        # launch whose second step is not part of it)
+        run_len = block if block else op.nt
         for n in range(op.nt):
+            t = n % run_len
+            left = min(run_len, op.nt - (n - t)) - t
             g = np.array([self.X, self.Z, self.P, op.t_left, op.t_right, op.t_up, op.t_down, op.nx,
                           op.nz, op.nt, TS, op.n_wavelet, op.n_receivers, phase, flags, src_j,
-                          self.rec_row, n, rz, seg, interior, qmode], np.int32)
+                          self.rec_row, n, rz, seg, interior, qmode, int(t % 2 == 0),
+                          int(left > 1)], np.int32)
             fn(_p(g), _p(hs), ptrs, _p(self.recmap), _p(wav), _p(seis_t), _p(seis_rec))
         self.cur = op.nt & 1
+        if pairs:
+            self.last = (self.u[(op.nt - 1) & 3], self.u[op.nt & 3])
         if qmode and qmode != Q_I16 and phase == 1 and (op.nt - 1) % TS == 0:
             c = (op.nt - 1) // TS
             qs = q16[c]
@@ -175,7 +192,7 @@ class DevSim:
                                          _p(smax[c]), C.c_int(qmode))
         return snaps, I, Is, seis_rec
 
-    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0, qmode=0):
+    def forward_tiled(self, shot_x, shot_z, wav, rz=4, record=False, interior=0, seg=0, qmode=0, block=0):
         """Forward pass; with qmode (1 FP16, 2 BF16, 3 int16 blocks) the
         snapshots are the 16-bit store and its max-abs slots (for int16
         blocks, the block exponents): returns (q16, smax), rec."""
@@ -189,17 +206,19 @@ class DevSim:
         else:
             smax = np.zeros((op.nt // TS, SNAP_WORDS), np.uint32)
         snaps, _, _, rec = self._tiles_pass(1, flags, rz, src_j=src_j, wav=wav, record=record,
-                                            interior=interior, seg=seg, qmode=qmode, q16=q16, smax=smax)
+                                            interior=interior, seg=seg, qmode=qmode, q16=q16, smax=smax,
+                                            block=block)
         if qmode:
             return (q16, smax), np.ascontiguousarray(rec.T)
         return snaps, np.ascontiguousarray(rec.T)
 
-    def backward_tiled(self, seis, snaps, rz=4, kahan=False, interior=0, seg=0, qmode=0):
+    def backward_tiled(self, seis, snaps, rz=4, kahan=False, interior=0, seg=0, qmode=0, block=0):
         flags = 8 | (1 if kahan else 0)
         if qmode:
             q16, smax = snaps
             _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, interior=interior, seg=seg, qmode=qmode,
                                            q16=q16, smax=smax)
         else:
-            _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, interior=interior, seg=seg)
+            _, I, Is, _ = self._tiles_pass(2, flags, rz, seis=seis, snaps=snaps, interior=interior, seg=seg,
+                                           block=block)
         return np.ascontiguousarray(I.T), np.ascontiguousarray(Is.T)

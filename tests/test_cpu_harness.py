@@ -350,3 +350,111 @@ def test_xcd_balance_keeps_every_tile_once(hx, nsims, interior):
                 for s in range(nsims):
                     mine = [(-int(r[3]), int(r[2]), int(r[1])) for r in rows if r[0] == s]
                     assert mine == sorted(mine)
+
+
+def _band_case():
+    """A grid whose receiver row and source lie in the 4-point band around
+    the two-step region (stepped by tiles, recomputed as margins by the pair
+    items): receivers at padded row 18 (region from row 20 at 16-row tiles,
+    from row 20 at 4-row tiles), source at padded column 66 (region from 68),
+    11 imaging steps over 55 time steps, an odd count."""
+    cfg, g = _case("wide")
+    cfg = dict(cfg, depthReceiver=0, firstLocationShot=46, depthShot=4, numberOfTimeStep=55,
+               sampligWavelet=55, firstLocationReceiver=0, spacingReceiver=2, numberOfReceivers=250)
+    rng = np.random.default_rng(5)
+    from rtmuq2d_amd import synth
+    seis = rng.standard_normal((1, 250, 55)).astype(np.float32)
+    return cfg, {"models": g["models"], "wavelet": synth.ricker(25.0, 0.001, 55), "seismograms": seis}
+
+
+@pytest.mark.parametrize("name", ["wide", "band"])
+@pytest.mark.parametrize("rz,seg,block", [(4, 0, 0), (1, 8, 0), (2, 12, 7), (4, 32, 5), (8, 0, 10)])
+def test_pair_launches_bitexact(hx, name, rz, seg, block):
+    """Two steps per launch on the inner image region (rtm_pair.h, the lane
+    exchanges as array shifts) beside one-step tiles elsewhere, over four u
+    buffers, in the executor's launch order: graph blocks of `block` steps
+    (an odd block ends on a launch whose second step is not part of it, and
+    the next block starts its pairs at an odd step).  Fields, snapshots,
+    recorded traces and images against the oracle, bit for bit."""
+    cfg, g = _band_case() if name == "band" else _case(name)
+    op = common.orc_params(cfg, time_slices=5 if name == "band" else 20)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = cfg["firstLocationShot"], cfg["depthShot"]
+    S, drec, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], record=True)
+    ds = DL.DevSim(hx, op, vpad)
+    items = _tiles(hx, hx.hx_inner_list, ds.X, ds.Z, op.t_left, op.t_right, op.t_up, op.t_down, rz,
+                   DL.IN_PAIRS, seg)
+    assert len(items) >= 2
+    S2, rec = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, record=True, interior=DL.IN_PAIRS, seg=seg, block=block)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu) and np.array_equal(b, lU)
+    assert np.array_equal(np.ascontiguousarray(S2.transpose(0, 2, 1)), S)
+    assert np.array_equal(rec, drec.astype(np.float32))
+    d = g["seismograms"][0]
+    I, Is, lu2, lU2 = O.backward(op, vpad, d, S)
+    I2, Is2 = ds.backward_tiled(d, S2, rz=rz, interior=DL.IN_PAIRS, seg=seg, block=block)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+
+
+def test_pair_launches_kahan_bitexact(hx):
+    """The compensated image through the pair items' imaging equals the tile
+    kernel's compensated image bit for bit (the tile kernel's is pinned
+    against FP64 by test_kahan_image_is_closer_to_fp64)."""
+    cfg, g = _case("wide")
+    op = common.orc_params(cfg, time_slices=4)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = cfg["firstLocationShot"], cfg["depthShot"]
+    S, _, _, _ = O.forward(op, sx, sz, vpad, g["wavelet"], record=False)
+    S2 = np.ascontiguousarray(S.transpose(0, 2, 1))
+    d = g["seismograms"][0]
+    ds = DL.DevSim(hx, op, vpad)
+    I, Is = ds.backward_tiled(d, S2, rz=2, kahan=True)
+    I2, Is2 = ds.backward_tiled(d, S2, rz=2, kahan=True, interior=DL.IN_PAIRS, seg=16)
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+    I3, _ = ds.backward_tiled(d, S2, rz=2, interior=DL.IN_PAIRS, seg=16)
+    assert not np.array_equal(I3, I)   # the compensation changes bits
+
+
+PAIR_GRIDS = GRIDS + [(1801, 451, 50, 50, 50, 50), (3481, 781, 40, 40, 40, 40), (1081, 381, 40, 40, 40, 40),
+                      (292, 169, 20, 22, 18, 21), (150, 90, 9, 9, 9, 9), (140, 60, 12, 12, 12, 12),
+                      (2001, 401, 100, 100, 100, 100), (80, 40, 8, 8, 8, 8)]
+
+
+@pytest.mark.parametrize("rz,seg", [(1, 0), (2, 16), (4, 0), (4, 40), (8, 24)])
+def test_pair_lists_cover_the_stepped_region_once(hx, rz, seg):
+    """Tiles plus pair items cover the stepped region exactly once; every
+    pair item keeps the bounds k_pairs relies on (u^n readable 8 points
+    around it, u^{n+1} computed 4 points around it without touching a CPML
+    strip or the unstepped border) and lies in the image region; the pair
+    region is a rectangle of whole tiles."""
+    import ctypes as C
+    for X, Z, TL, TR, TU, TD in PAIR_GRIDS:
+        TZ = 4 * rz
+        t = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, 4, seg)
+        cover = np.zeros((Z, X), np.int32)
+        for _, x0, z0, _ in t:
+            cover[z0:min(z0 + TZ, Z - 4), x0:min(x0 + 64, X - 4)] += 1
+        cap = 100000
+        buf = np.zeros(4 * cap, np.int32)
+        n = hx.hx_inner_list(X, Z, TL, TR, TU, TD, rz, 4, seg, buf.ctypes.data_as(C.c_void_p), cap)
+        assert n >= 0
+        items = buf[:4 * n].reshape(n, 4)
+        for packed, x0, zA, zB in items:
+            wp = int(packed) >> 24
+            w = 2 * wp
+            assert 1 <= wp <= 56 and x0 % 2 == 0 and (zB - zA) % 4 == 0 and zB > zA
+            assert x0 - 8 >= 0 and x0 + w + 8 <= X and zA - 8 >= 0 and zB + 8 <= Z
+            assert x0 >= TL and x0 + w <= X - TR and zA >= TU and zB <= Z - TD
+            # u^{n+1} margin: outside the strips and inside the stepped region
+            assert x0 - 4 >= max(TL - 4, 4) and x0 + w + 4 <= min(X - TR + 4, X - 4)
+            assert zA - 4 >= max(TU - 4, 4) and zB + 4 <= min(Z - TD + 4, Z - 4)
+            cover[zA:zB, x0:x0 + w] += 1
+        assert (cover[4:Z - 4, 4:X - 4] == 1).all(), (X, Z)
+        assert cover.sum() == (Z - 8) * (X - 8)
+        if n:
+            xs, xe = items[:, 1].min(), max(int(x) + 2 * (int(p) >> 24) for p, x, _, _ in items)
+            zs, ze = items[:, 2].min(), items[:, 3].max()
+            assert (xs - 4) % 64 == 0 and (xe - 4) % 64 == 0 and (zs - 4) % TZ == 0 and (ze - 4) % TZ == 0
+            assert (ze - zs) * (xe - xs) == sum((int(b) - int(a)) * 2 * (int(p) >> 24) for p, _, a, b in items)

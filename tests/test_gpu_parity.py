@@ -139,6 +139,126 @@ def test_marched_interior_bitexact(gpu_available, rz, interior, seg, dtype):
     assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
 
 
+def _band_case():
+    """The wide grid with the receivers on padded row 18 and the source at
+    padded column 66, both in the 4-point band around the two-step region
+    (which starts at row 20 and column 68): tiles own those points and the
+    pair items recompute them as margins.  55 steps, an odd count."""
+    cfg, model, wav, seis = _wide_case()
+    nt = 55
+    cfg = dict(cfg, depthReceiver=0, firstLocationShot=46, depthShot=4, numberOfTimeStep=nt, sampligWavelet=nt,
+               firstLocationReceiver=0, spacingReceiver=2, numberOfReceivers=250)
+    from rtmuq2d_amd import synth
+    seis = np.random.default_rng(5).standard_normal((250, nt)).astype(np.float32)
+    return cfg, model, synth.ricker(25.0, 0.001, nt), seis
+
+
+@pytest.mark.parametrize("case", ["wide", "band"])
+@pytest.mark.parametrize("rz,block,seg", [(4, 100, 0), (1, 7, 8), (2, 5, 12), (8, 10, 32), (4, 1, 16), (2, 100, 0)])
+def test_pair_launches_bitexact(gpu_available, case, rz, block, seg):
+    """Two time steps per launch on the inner image region (k_pairs) beside
+    one-step tiles (k_tiles over four u buffers), in graph blocks of `block`
+    steps (odd blocks end on a launch whose second step is cut, 1 = eager):
+    fields, recorded traces, snapshots and images bit-identical to the
+    oracle."""
+    say(f"[gpu] pair launches {case} rz={rz} block={block} seg={seg}")
+    cfg, model, wav, seis = _band_case() if case == "band" else _wide_case()
+    ts = 5 if case == "band" else 10
+    op = common.orc_params(cfg, time_slices=ts)
+    p = common.rtm_params(cfg, time_slices=ts)
+    vpad = O.extend(op, model)
+    sx, sz = cfg["firstLocationShot"], cfg["depthShot"]
+    S, d_orc, lu, lU = O.forward(op, sx, sz, vpad, wav, record=True)
+    I, Is, lu2, lU2 = O.backward(op, vpad, seis, S)
+    with R.Plan(p) as plan:
+        plan.set_tiling(rz, block, R.INTERIOR_PAIRS, seg)
+        assert plan.tiling() == (rz, block, R.INTERIOR_PAIRS, seg)
+        plan.set_velocity(vpad)
+        plan.set_wavelet(wav)
+        plan.forward(sx, sz, record=True)
+        a, b = plan.fields()
+        assert np.array_equal(a, lu) and np.array_equal(b, lU)
+        assert np.array_equal(plan.seismogram(), d_orc)
+        plan.set_seismogram(seis)
+        plan.zero_image()
+        plan.backward()
+        a, b = plan.fields()
+        assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
+        I2, Is2 = plan.image()
+        assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+        # back to one step per launch on the same plan: two buffers again
+        plan.set_tiling(rz, 100, 0, 0)
+        plan.zero_image()
+        plan.migrate(sx, sz)
+        I3, _ = plan.image()
+    assert np.array_equal(I3, I)
+
+
+def test_pair_launches_kahan_image_equals_tiles(gpu_available):
+    """The compensated image through the pair items equals the tile
+    kernel's compensated image bit for bit."""
+    say("[gpu] pair launches, Kahan image")
+    cfg, model, wav, seis = _wide_case()
+    p = common.rtm_params(cfg, time_slices=4, image_kahan=1)
+    vpad = O.extend(common.orc_params(cfg, time_slices=4), model)
+    sx, sz = cfg["firstLocationShot"], cfg["depthShot"]
+    out = []
+    for shape in [(2, 100, 0, 0), (2, 100, R.INTERIOR_PAIRS, 16)]:
+        with R.Plan(p) as plan:
+            plan.set_tiling(*shape)
+            plan.set_velocity(vpad)
+            plan.set_wavelet(wav)
+            plan.set_seismogram(seis)
+            plan.zero_image()
+            plan.migrate(sx, sz)
+            out.append(plan.image())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("group,block", [(0, 100), (2, 9)])
+def test_batch_pair_launches_bitexact(gpu_available, group, block):
+    """A batch stepped with pair items (one launch group, or groups of 2
+    with a ragged last group): every realization's image equals the
+    oracle's."""
+    say(f"[gpu] batch pair launches group={group} block={block}")
+    cfg, model, wav, seis = _wide_case()
+    op = common.orc_params(cfg, time_slices=10)
+    p = common.rtm_params(cfg, time_slices=10)
+    sx, sz = common.shot_positions(cfg)[0]
+    vpads = [O.extend(op, model * np.float32(1 + 0.03 * k)) for k in range(3)]
+    refs = [O.rtm_shot(op, sx, sz, v, wav, seis) for v in vpads]
+    with R.Batch(p, len(vpads)) as b:
+        b.set_group(group)
+        b.set_tiling(4, block, R.INTERIOR_PAIRS, 24)
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(wav)
+        b.set_seismogram(seis)
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k in range(len(vpads)):
+            I, Is = b.image(k)
+            assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
+
+
+def test_pair_shape_refusals(gpu_available):
+    """Pair items step FP32 fields with the FP32 store: an FP64 plan, a
+    16-bit store and the request server refuse the shape and keep theirs."""
+    say("[gpu] pair shape refusals")
+    cfg, _, _, _ = _wide_case()
+    with R.Plan(common.rtm_params(cfg), R.REAL_F64) as plan:
+        before = plan.tiling()
+        with pytest.raises(R.RtmError):
+            plan.set_tiling(4, 100, R.INTERIOR_PAIRS, 0)
+        assert plan.tiling() == before
+    with R.Plan(common.rtm_params(cfg, snapshot_mode=R.SNAP_FP16)) as plan:
+        with pytest.raises(R.RtmError):
+            plan.set_tiling(4, 100, R.INTERIOR_PAIRS, 0)
+    with R.Server(1 << 28, max_slots=2, snapshot_mode=R.SNAP_FP32) as srv:
+        with pytest.raises(R.RtmError):
+            srv.set_tiling(4, 100, R.INTERIOR_PAIRS, 0)
+
+
 @pytest.mark.parametrize("interior,seg", [(0, 0), (1, 16), (2, 0)])
 def test_batch_marched_bitexact(gpu_available, interior, seg):
     """A batch with a streaming interior kernel: every realization's image

@@ -70,23 +70,30 @@ def c1_inputs(nt):
     return _C1_INPUTS[nt]
 
 
-def test_c1_ts1_fp32_store_past_4gib_bitexact(gpu_available):
-    """The headline workload's path: c1 grid, FP32 snapshot every step, 64x8
-    tiles.  2000 steps make a 4.78 GB store, so the per-snapshot base
-    pointers pass 4 GiB; the image equals the oracle's over the same steps."""
+_C1_REF = {}
+
+
+@pytest.mark.parametrize("shape", [C1_SHAPE, (2, 100, R.INTERIOR_PAIRS, 0), (4, 100, R.INTERIOR_PAIRS, 64)])
+def test_c1_ts1_fp32_store_past_4gib_bitexact(gpu_available, shape):
+    """The plain c1 workload's path: c1 grid, FP32 snapshot every step, in
+    64x8 tiles and with two-step pair launches on the inner image region.
+    2000 steps make a 4.78 GB store, so the per-snapshot base pointers pass
+    4 GiB; the image equals the oracle's over the same steps."""
     nt = 2000
     p = c1(nt, 1)
     nint = p.nx * p.nz
     assert nt * nint * 4 > 4 << 30
-    say(f"[gpu] c1 TS=1 FP32 store, {nt} steps, store {nt * nint * 4 / 1e9:.2f} GB, shape {C1_SHAPE}")
+    say(f"[gpu] c1 TS=1 FP32 store, {nt} steps, store {nt * nint * 4 / 1e9:.2f} GB, shape {shape}")
     v_true, v_mig, w, d = c1_inputs(nt)
     op = O.from_rtm(p)
     vpad = O.extend(op, v_mig)
     sx, sz = p.first_location_shot, p.depth_shot
-    I_ref, Is_ref = O.rtm_shot(op, sx, sz, vpad, w, d)
+    if nt not in _C1_REF:
+        _C1_REF[nt] = O.rtm_shot(op, sx, sz, vpad, w, d)
+    I_ref, Is_ref = _C1_REF[nt]
     say("[gpu] c1 TS=1: oracle done")
     with R.Plan(p) as plan:
-        plan.set_tiling(*C1_SHAPE)
+        plan.set_tiling(*shape)
         plan.set_velocity(vpad)
         plan.set_wavelet(w)
         plan.set_seismogram(d)
@@ -111,7 +118,7 @@ def _c2_group_refs(nt, nsims):
     return _C2[(nt, nsims)]
 
 
-@pytest.mark.parametrize("shape", [(4, 100, 0, 0), (4, 100, 1, 32)])
+@pytest.mark.parametrize("shape", [(4, 100, 0, 0), (4, 100, 1, 32), (4, 100, 4, 0), (2, 99, 4, 48)])
 def test_c2_launch_group_of_13_on_c1_grid(gpu_available, shape):
     """One c2 launch group (13 realizations of the c1 grid, TS = 20, the
     size the group autotuner picks), steps shortened to 200: every image
@@ -142,32 +149,43 @@ def test_c2_launch_group_of_13_on_c1_grid(gpu_available, shape):
     assert np.allclose(var, stack.var(0, ddof=1), rtol=1e-10, atol=1e-12 * stack.var(0).max())
 
 
-def test_c2_full_ensemble_equals_plain_images(gpu_available):
+_C2_PLAIN = {}
+
+
+@pytest.mark.parametrize("shape", [None, (4, 100, R.INTERIOR_PAIRS, 0)])
+def test_c2_full_ensemble_equals_plain_images(gpu_available, shape):
     """The whole c2 workload (64 realizations, nt = 4000, TS = 20, launch
-    groups of 13): every batched image equals the plain plan's image of the
-    same realization, bit for bit (the plain path is bit-exact against the
-    oracle and the reference at this size)."""
+    groups of 13), in the default shape and with two-step pair launches:
+    every batched image equals the plain plan's image of the same
+    realization (64x16 tiles), bit for bit (the plain path is bit-exact
+    against the oracle and the reference at this size)."""
     nsims = 64
     p = c1(4000, 20)
     v_true, _, w = bench.c1_inputs(p)
-    say(f"[gpu] c2 full ensemble: {nsims} x nt {p.number_of_time_steps}, batch vs plain")
+    say(f"[gpu] c2 full ensemble: {nsims} x nt {p.number_of_time_steps}, batch vs plain, shape {shape}")
     sx, sz = p.first_location_shot, p.depth_shot
     vpads = [R.extend(p, synth.ensemble_velocity(v_true, 100 + k)) for k in range(nsims)]
-    plain = []
-    with R.Plan(p) as plan:
-        plan.set_velocity(R.extend(p, v_true))
-        plan.set_wavelet(w)
-        plan.forward(sx, sz, record=True)
-        d = plan.seismogram()
-        plan.set_seismogram(d)
-        for k, v in enumerate(vpads):
-            plan.set_velocity(v)
-            plan.zero_image()
-            plan.migrate(sx, sz)
-            plain.append(plan.image()[0])
-    say("[gpu] c2 full ensemble: plain images done")
+    if not _C2_PLAIN:
+        plain = []
+        with R.Plan(p) as plan:
+            plan.set_tiling(4, 100, 0, 0)
+            plan.set_velocity(R.extend(p, v_true))
+            plan.set_wavelet(w)
+            plan.forward(sx, sz, record=True)
+            d = plan.seismogram()
+            plan.set_seismogram(d)
+            for k, v in enumerate(vpads):
+                plan.set_velocity(v)
+                plan.zero_image()
+                plan.migrate(sx, sz)
+                plain.append(plan.image()[0])
+        _C2_PLAIN["d"], _C2_PLAIN["plain"] = d, plain
+        say("[gpu] c2 full ensemble: plain images done")
+    d, plain = _C2_PLAIN["d"], _C2_PLAIN["plain"]
     with R.Batch(p, nsims) as b:
         b.set_group(13)
+        if shape:
+            b.set_tiling(*shape)
         assert b.group() == (13, 5)
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
