@@ -321,6 +321,9 @@ static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
     if (ex.upload_descs(d.data(), N, q->stream)) return -1;
     if (ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
     const int nt = b0.p.number_of_time_steps;
+    // every realization of the group at the same step with the same store:
+    // the steps that neither store nor image run the light tile instance
+    ex.set_schedule(b0.qmode == rtm::Q_NONE && !b0.ringmode ? phase : 0, nt, b0.p.time_slices, d[0].flags);
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (ex.run(nt, q->graph_block, q->stream)) return -1;
     if (phase == rtm::PH_FORWARD)
@@ -328,6 +331,7 @@ static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
             if (q->sims[s0 + s].flush_last_snapshot(q->stream)) return -1;
     HIP_TRY(hipEventRecord(q->ev1, q->stream));
     HIP_TRY(hipEventSynchronize(q->ev1));
+    if (ex.check_err(q->stream)) return -1;
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
     (phase == rtm::PH_FORWARD ? q->fwd_ms : q->bwd_ms) += ms;

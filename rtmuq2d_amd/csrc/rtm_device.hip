@@ -317,6 +317,8 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
             d.ubuf[1] = (R *)b.sbuf[1];
         }
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
+        // steps that store no snapshot run the light tile instance
+        q->ex.set_schedule(b.qmode == rtm::Q_NONE && !b.ringmode ? rtm::PH_FORWARD : 0, nt, p.time_slices, d.flags);
         if (record && !(d.flags & rtm::F_RECORD)) {
             // receivers sharing a point or outside: record each one per step
             // (with pair items an even step's launch also steps them at n+1,
@@ -333,6 +335,7 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
             return -1;
         }
         if (b.flush_last_snapshot(q->stream)) return -1;
+        if (q->ex.check_err(q->stream)) return -1;
     } else {
         v1_forward<R>(q, src_j, src_inside, record);
     }
@@ -368,7 +371,10 @@ static int backward_t(rtm_plan *q)
         d.flags = rtm::F_IMAGE | (p.image_kahan ? rtm::F_KAHAN : 0) | (b.ringmode ? rtm::F_RING : 0);
         if (b.ringmode) d.src_j = q->fwd_src_j;   // only the reconstruction reads it in this phase
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
+        // steps that image nothing run the light tile instance
+        q->ex.set_schedule(b.qmode == rtm::Q_NONE && !b.ringmode ? rtm::PH_BACKWARD : 0, nt, p.time_slices, d.flags);
         if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
+        if (q->ex.check_err(q->stream)) return -1;
     } else if (b.qmode != rtm::Q_NONE || b.ringmode) {
         rtm_set_error("16-bit snapshot stores and boundary saving need every receiver inside the stepped region");
         return -1;

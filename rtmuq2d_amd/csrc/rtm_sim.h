@@ -82,21 +82,27 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
     }
 }
 
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1>
+// LIGHT: an instance without the snapshot store and the imaging, for the
+// steps the executor knows have neither (fewer registers, a higher
+// occupancy).  Meeting a step that stores or images sets *err, which the
+// executor reports as an error after the pass.
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
-                                         int x0, int z0, int tx, int ty)
+                                         int x0, int z0, int tx, int ty, int *err)
 {
     const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16, UM>(S, n);
     if (!V.active) return;
+    if (LIGHT && (V.snap || V.sfield || V.qsnap || V.smax) && threadIdx.x == 0) atomicOr(err, 1);
     rtm::TileRegs<R, RZ, NW> G;
-    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, !LIGHT>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS, NW>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
         rtm::tile_psi<R, RZ, CLS, NW>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW>(L, G, S, V, x0, z0, tx, ty);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW, !LIGHT>(L, G, S, V, x0, z0, tx, ty);
+    if (LIGHT) return;
     if (Q16 && V.smax) commit_snap_max_wg<NW>(V.smax, m, red);
     if (I16 && V.forward && V.qsnap) {   /* uniform: a store step */
         float rmax[RZ];
@@ -106,15 +112,15 @@ __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red,
     }
 }
 
-template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4, int UM = 1>
+template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
 __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
-                                             int4 tile, int tx, int ty)
+                                             int4 tile, int tx, int ty, int *err)
 {
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
-    default: run_tile<R, RZ, 3, Q16, I16, NW, UM>(L, red, S, n, tile.y, tile.z, tx, ty); break;
+    case 0: run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    default: run_tile<R, RZ, 3, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     }
 }
 
@@ -128,11 +134,12 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *
 // NW: wavefronts per workgroup, one row group each (tiles of 64 x NW*RZ).
 // UM: u buffer mask, 1 (two buffers, in-place update) or 3 (four buffers:
 // the tiles beside two-step pair items).
-template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4, int UM = 1>
+// LIGHT: the instance without store and imaging (run_tile); err: its flag.
+template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
 __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                    const int *__restrict__ n0,
                                                    const int4 *__restrict__ tiles,
-                                                   const int *__restrict__ ntiles_p, int s)
+                                                   const int *__restrict__ ntiles_p, int s, int *err)
 {
     __shared__ rtm::TileLds<R, RZ, NW> L;
     __shared__ float red[NW];
@@ -150,7 +157,7 @@ __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict_
     const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    run_tile_cls<R, RZ, Q16, I16, NW, UM>(L, red, S, n, tile, tx, ty);
+    run_tile_cls<R, RZ, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile, tx, ty, err);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run
@@ -180,9 +187,9 @@ __global__ void __launch_bounds__(256) k_tiles_mix(const SimDesc<R> *__restrict_
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (tile.w & rtm::T_SMALL) {
         tile.w &= 3;
-        run_tile_cls<R, 1, Q16>(L.small, red, S, n, tile, tx, ty);
+        run_tile_cls<R, 1, Q16>(L.small, red, S, n, tile, tx, ty, nullptr);
     } else {
-        run_tile<R, RZ, 0, Q16>(L.full, red, S, n, tile.y, tile.z, tx, ty);
+        run_tile<R, RZ, 0, Q16>(L.full, red, S, n, tile.y, tile.z, tx, ty, nullptr);
     }
 }
 
@@ -911,6 +918,7 @@ struct Exec {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int block = 0, ring_blocks = 0;
+        long key = 0;                   // the step schedule it was captured with (sched_key)
     };
     Captured cap[3];
     hipStream_t g_stream = nullptr;     // where the graphs were last launched
@@ -932,8 +940,21 @@ struct Exec {
     int run(int steps, int block, hipStream_t s);
     // the launch of step s_in_block of a block; with pair items (IN_PAIRS)
     // an even s_in_block also steps them at s_in_block + 1 when that is
-    // below `lim` (the block's length)
-    int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30);
+    // below `lim` (the block's length).  heavy = false: no simulation stores
+    // or images at this step, so the tiles run the light instance.
+    int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30, bool heavy = true);
+    // The step schedule of the coming pass (every simulation of the launch at
+    // the same step, FP32 store, no boundary saving): the forward steps
+    // n % ts == 0 store a snapshot (flags F_SNAP), the backward steps with
+    // (nt-1-n) % ts == 0 image (F_IMAGE); every other step runs the light tile
+    // instance.  phase 0 clears it (every step runs the full instance).
+    bool sched = false;
+    int sched_phase = 0, sched_nt = 0, sched_ts = 1, sched_flags = 0;
+    int *d_err = nullptr;               // set by a light launch that met a store or imaging step
+    void set_schedule(int phase, int nt, int ts, int flags);
+    bool heavy_step(int n) const;       // step n of the pass stores or images (always without a schedule)
+    long sched_key(int n0) const;       // the schedule of a block starting at step n0 (0: every step full)
+    int check_err(hipStream_t s);       // -1 with the error set if a light launch met a heavy step
     // every captured graph, once the stream they ran on has drained
     int drop_graph();
     // the boundary-saving kernel of the coming pass (0: none; its grid

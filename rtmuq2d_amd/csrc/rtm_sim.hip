@@ -563,6 +563,9 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
     HIP_TRY(hipMalloc((void **)&d_inner, sizeof(int4) * cap_inner));
     HIP_TRY(hipMalloc((void **)&d_ninner, sizeof(int)));
     HIP_TRY(hipMemset(d_ninner, 0, sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&d_err, sizeof(int)));
+    HIP_TRY(hipMemset(d_err, 0, sizeof(int)));
+    sched = false;
     ninner = 0;
     h_n0.assign(cap_sims, 0);
     HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -590,11 +593,12 @@ int Exec::drop_graph()
 void Exec::release()
 {
     drop_graph();
-    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles, d_inner, d_ninner};
+    void *all[] = {d_desc, d_n0, d_tiles, d_ntiles, d_inner, d_ninner, d_err};
     for (void *p : all)
         if (p) (void)hipFree(p);
     d_desc = nullptr; d_n0 = nullptr; d_tiles = nullptr; d_ntiles = nullptr;
-    d_inner = nullptr; d_ninner = nullptr;
+    d_inner = nullptr; d_ninner = nullptr; d_err = nullptr;
+    sched = false;
     ninner = ntiles = 0;
     if (side) {
         (void)hipStreamSynchronize(side);
@@ -653,8 +657,41 @@ int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
     return 0;
 }
 
+// The light tile instance (no snapshot store, no imaging) for FP32 fields
+// with the FP32 store in tile shapes; false when there is none.
+template <typename R, bool Q16, int UM = 1>
+static bool launch_light(const Exec &e, int s_in_block, hipStream_t ts)
+{
+    if constexpr (Q16 || sizeof(R) != 4) {
+        return false;
+    } else {
+        const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
+        const int grid = 8 * ((e.cap_tiles + 7) / 8);
+        const bool one = e.cap_sims == 1;
+        switch (e.rz) {
+        case 1:
+            if (one) hipLaunchKernelGGL((k_tiles<R, 1, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 1, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            break;
+        case 2:
+            if (one) hipLaunchKernelGGL((k_tiles<R, 2, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 2, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            break;
+        case 8:
+            if (one) hipLaunchKernelGGL((k_tiles<R, 8, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 8, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            break;
+        default:
+            if (one) hipLaunchKernelGGL((k_tiles<R, 4, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 4, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            break;
+        }
+        return true;
+    }
+}
+
 template <typename R, bool Q16>
-static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
+static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s, bool heavy)
 {
     const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
     // interior and strip tiles write disjoint points and read only u^n and
@@ -682,27 +719,29 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s)
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         const bool one = e.cap_sims == 1;
         if (e.rz == 2) {
-            if (one) hipLaunchKernelGGL((k_tiles<R, 2, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
-            else hipLaunchKernelGGL((k_tiles<R, 2, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 2, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 2, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
         } else {
-            if (one) hipLaunchKernelGGL((k_tiles<R, 4, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
-            else hipLaunchKernelGGL((k_tiles<R, 4, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 4, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 4, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
         }
+    } else if (e.ntiles && !heavy && launch_light<R, Q16>(e, s_in_block, ts)) {
+        // a step that neither stores nor images: the light instance
     } else if (e.ntiles && e.cap_sims == 1) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     } else if (e.ntiles) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<R, 1, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<R, 2, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<R, 8, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<R, 4, Q16>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     }
     if (fork) {
@@ -731,27 +770,27 @@ static void launch_tiles_i16(const Exec &e, int s_in_block, hipStream_t s)
     if (e.interior == rtm::IN_TILES8) {
         const bool one = e.cap_sims == 1;
         if (e.rz == 2) {
-            if (one) hipLaunchKernelGGL((k_tiles<float, 2, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
-            else hipLaunchKernelGGL((k_tiles<float, 2, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            if (one) hipLaunchKernelGGL((k_tiles<float, 2, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<float, 2, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
         } else {
-            if (one) hipLaunchKernelGGL((k_tiles<float, 4, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
-            else hipLaunchKernelGGL((k_tiles<float, 4, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block);
+            if (one) hipLaunchKernelGGL((k_tiles<float, 4, false, true, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<float, 4, false, false, true, 8>), dim3(grid), dim3(512), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
         }
         return;
     }
     if (e.cap_sims == 1) {
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     } else {
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, true>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     }
 }
@@ -782,7 +821,7 @@ int check_pair_real(int real, int interior)
 // Launch of step s of a block with pair items: the pair items at s and s+1
 // (an even s), then the tiles of the rest of the grid at s (k_tiles over
 // four u buffers).  FP32, FP32 store (check_store_shape, check_pair_real).
-static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s)
+static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s, bool heavy)
 {
     const SimDesc<float> *d = (const SimDesc<float> *)e.d_desc;
     const bool one = e.cap_sims == 1;
@@ -794,31 +833,32 @@ static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s)
         else hipLaunchKernelGGL((k_pairs<float, false, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block, lim);
     }
     if (!e.ntiles) return;
+    if (!heavy && launch_light<float, false, 3>(e, s_in_block, s)) return;
     const int grid = 8 * ((e.cap_tiles + 7) / 8);
     if (one) {
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, true, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     } else {
         switch (e.rz) {
-        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
-        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block); break;
+        case 1: hipLaunchKernelGGL((k_tiles<float, 1, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 2: hipLaunchKernelGGL((k_tiles<float, 2, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        case 8: hipLaunchKernelGGL((k_tiles<float, 8, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
+        default: hipLaunchKernelGGL((k_tiles<float, 4, false, false, false, 4, 3>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err); break;
         }
     }
 }
 
-int Exec::launch_one(int s_in_block, hipStream_t s, int lim)
+int Exec::launch_one(int s_in_block, hipStream_t s, int lim, bool heavy)
 {
-    if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s);
-    else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s);
+    if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s, heavy);
+    else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s, true);
     else if (i16) launch_tiles_i16(*this, s_in_block, s);
-    else if (q16) launch_tiles<float, true>(*this, s_in_block, s);
-    else launch_tiles<float, false>(*this, s_in_block, s);
+    else if (q16) launch_tiles<float, true>(*this, s_in_block, s, true);
+    else launch_tiles<float, false>(*this, s_in_block, s, heavy);
     // boundary saving: the ring save / reconstruction follows the step on the
     // same stream (running them beside the step on a stream of their own cost
     // more in graph edges than it hid)
@@ -827,6 +867,44 @@ int Exec::launch_one(int s_in_block, hipStream_t s, int lim)
         else launch_ring<float>(*this, s_in_block, s);
     }
     return 0;
+}
+
+void Exec::set_schedule(int phase, int nt, int ts, int flags)
+{
+    sched = phase == rtm::PH_FORWARD || phase == rtm::PH_BACKWARD;
+    sched_phase = sched ? phase : 0;
+    sched_nt = nt;
+    sched_ts = ts > 0 ? ts : 1;
+    sched_flags = flags;
+}
+
+bool Exec::heavy_step(int n) const
+{
+    if (!sched) return true;
+    if (n < 0 || n >= sched_nt) return false;   // inactive: the kernels skip the simulation
+    if (sched_phase == rtm::PH_FORWARD) return (sched_flags & rtm::F_SNAP) && n % sched_ts == 0;
+    return (sched_flags & rtm::F_IMAGE) && (sched_nt - 1 - n) % sched_ts == 0;
+}
+
+long Exec::sched_key(int n0) const
+{
+    // a captured block is replayed at n0 + k*block: its light and full steps
+    // repeat only when the block is a multiple of the snapshot interval
+    if (!sched) return 0;
+    return 1 + sched_phase + 4L * (sched_flags & (rtm::F_SNAP | rtm::F_IMAGE)) +
+           64L * ((sched_nt - 1) % sched_ts) + 64L * 65536 * (n0 % sched_ts) + 64L * 65536 * 65536 * sched_ts;
+}
+
+int Exec::check_err(hipStream_t s)
+{
+    int h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!h) return 0;
+    fill_zero(d_err, 1, s);
+    (void)hipStreamSynchronize(s);
+    rtm_set_error("internal: a light step launch met a step that stores or images (step schedule mismatch)");
+    return -1;
 }
 
 int Exec::set_ring_phase(int phase, int blocks)
@@ -844,21 +922,28 @@ int Exec::run(int steps, int block, hipStream_t s)
         return -1;
     }
     int done = 0;
+    const int n0 = h_n0[0];   // with a schedule every simulation is at this step
     if (block > 1 && steps >= block) {
-        Captured &c = cap[ring_phase];
+        // with a step schedule whose pattern repeats from block to block, the
+        // captured steps run the light or the full tile instance as the
+        // schedule says; otherwise every captured step runs the full one
+        const bool pattern = sched && block % sched_ts == 0;
+        const long key = pattern ? sched_key(n0) : 0;
+        Captured &c = cap[sched ? sched_phase : ring_phase];
         const bool lists_match = g_tiles == (ntiles > 0) && g_inner == (ninner > 0);
         if (!lists_match) drop_graph();
-        if (!c.exec || c.block != block || c.ring_blocks != ring_blocks) {
+        if (!c.exec || c.block != block || c.ring_blocks != ring_blocks || c.key != key) {
             if (c.exec && g_stream) (void)hipStreamSynchronize(g_stream);
             destroy_captured(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int t = 0; t < block; t++) launch_one(t, s, block);
+            for (int t = 0; t < block; t++) launch_one(t, s, block, pattern ? heavy_step(n0 + t) : true);
             hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
             hipError_t e = hipStreamEndCapture(s, &c.graph);
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
             HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
             c.block = block;
             c.ring_blocks = ring_blocks;
+            c.key = key;
             g_tiles = ntiles > 0;
             g_inner = ninner > 0;
         }
@@ -869,7 +954,7 @@ int Exec::run(int steps, int block, hipStream_t s)
         }
     }
     const int rem = steps - done;
-    for (int t = 0; t < rem; t++) launch_one(t, s, rem);
+    for (int t = 0; t < rem; t++) launch_one(t, s, rem, heavy_step(n0 + done + t));
     if (rem) hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, rem);
     HIP_TRY(hipGetLastError());
     for (int i = 0; i < cap_sims; i++) h_n0[i] += steps;
@@ -934,6 +1019,9 @@ int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hi
     }
     if (ex.upload_descs(descs.data(), (int)descs.size(), s)) return -1;
     if (ex.set_n0(std::vector<int>(descs.size(), 0), s)) return -1;
+    // the forward schedule of these steps, as a pass runs them
+    ex.set_schedule(descs.empty() || descs[0].qmode != rtm::Q_NONE ? 0 : rtm::PH_FORWARD,
+                    descs.empty() ? 0 : descs[0].nt, descs.empty() ? 1 : descs[0].TS, flags);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -949,6 +1037,7 @@ int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hi
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (!rc) rc = ex.check_err(s);
     *ms = t / (steps > 0 ? steps : 1);
     return rc;
 }

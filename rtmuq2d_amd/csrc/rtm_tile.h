@@ -72,7 +72,10 @@ template <typename T>
 RTM_HD void stg_stream(T *p, int i, T v)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_nontemporal_store(v, p + i);
+    /* through the global address space: a flat store would also count
+     * against the LDS counter and make the next LDS wait wait for it */
+    const unsigned off = (unsigned)i * (unsigned)sizeof(T);
+    __builtin_nontemporal_store(v, (RTM_GAS T *)((RTM_GAS char *)p + off));
 #else
     p[i] = v;
 #endif
@@ -329,6 +332,7 @@ struct TileLds {
     R u[TZ + 16][80];       /* rows z0-8 .. z0+TZ+7, columns x0-8 .. x0+71 */
     R px[TZ][72];           /* psiX^{n+1}: rows z0 .., columns x0-4 .. x0+67 */
     R pz[TZ + 8][64];       /* psiZ^{n+1}: rows z0-4 .., columns x0 .. x0+63 */
+    R az[TZ + 8], bz[TZ + 8];   /* z profiles of rows z0-4 .. z0+TZ+3 (z-strip tiles) */
 };
 
 /* Per-launch view of one sim at step n. */
@@ -472,12 +476,46 @@ struct TileRegs {
     R Icv[RZ], Iscv[RZ];        /* Kahan compensations                  */
     int recr;                   /* receiver at the own column, or -1    */
     R recv;                     /* its trace sample (backward)          */
+    R azt, bzt;                 /* z-strip tiles: the z profiles at row z0-4+thread (staged in LDS) */
     R uq[RZ];                   /* int16 block store: U at own image points, else 0 */
 };
 
-/* Phase 1a: every global load of the step.  tx in [0,64), ty in [0,4) (the
- * wavefront index: uniform within a wavefront).  CLS is the tile class. */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4>
+/* The imaging operands of the thread's points (backward, on imaging
+ * steps): the snapshot value and the image accumulators.  Part of phase 1,
+ * or loaded after the barriers (run_tile's LATE instances: their registers
+ * then do not add to phase 1's, which keeps the occupancy of the batched
+ * kernels up at the price of a second round trip on imaging steps). */
+template <typename R, int RZ, bool Q16 = false, bool I16 = false, int NW = 4>
+RTM_HD void tile_fetch_img(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
+                           int tx, int ty)
+{
+    const int i = x0 + tx;
+    const int ic = clampi(i, 0, S.X - 1);
+    if (!V.forward && ((Q16 || I16) ? V.qsnap != nullptr : (V.snap || V.sfield))) {
+        const int ii = clampi(i - S.TL, 0, S.nx - 1);
+        const int qb = clampi(i16_block(x0) - S.qb0, 0, S.nqb - 1);   /* the tile row's block (I16) */
+#pragma unroll
+        for (int rr = 0; rr < RZ; rr++) {
+            const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
+            const int q = kk * S.nx + ii;
+            if (I16) G.sv[rr] = R(dequant_i16((short)ldg(V.qsnap, q), ldg(V.qexp, kk * S.nqb + qb)));
+            else if (Q16) G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
+            else if (V.sfield) G.sv[rr] = ldg(V.sfield, clampi(z0 + ty * RZ + rr, 0, S.Z - 1) * S.P + ic);
+            else G.sv[rr] = ldg(V.snap, q);
+            G.Iv[rr] = ldg(S.I, q);
+            G.Isv[rr] = ldg(S.Is, q);
+            if (S.flags & F_KAHAN) {
+                G.Icv[rr] = ldg(S.Ic, q);
+                G.Iscv[rr] = ldg(S.Isc, q);
+            }
+        }
+    }
+}
+
+/* Phase 1a: every global load of the step (the imaging operands unless
+ * IMG_EARLY is false).  tx in [0,64), ty in [0,4) (the wavefront index:
+ * uniform within a wavefront).  CLS is the tile class. */
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool IMG_EARLY = true>
 RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
                        int tx, int ty)
 {
@@ -537,26 +575,14 @@ RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepVi
             const R v = ldg(V.psiZi, clampi(k, 0, S.Z - 1) * S.P + ic);
             G.pz[t] = (iok && z_strip(S, k)) ? v : R(0);
         }
+        /* the z profiles of the tile's rows and psi halo, read once here
+         * instead of after the barriers, row by row (thread ty*64+tx takes
+         * row z0-4+ty*64+tx; the rows past TZ+8 are not staged) */
+        const int kp = clampi(z0 - 4 + ty * 64 + tx, 0, S.Z - 1);
+        G.azt = ldg(S.az, kp);
+        G.bzt = ldg(S.bz, kp);
     }
-    if (!V.forward && ((Q16 || I16) ? V.qsnap != nullptr : (V.snap || V.sfield))) {
-        const int ii = clampi(i - S.TL, 0, S.nx - 1);
-        const int qb = clampi(i16_block(x0) - S.qb0, 0, S.nqb - 1);   /* the tile row's block (I16) */
-#pragma unroll
-        for (int rr = 0; rr < RZ; rr++) {
-            const int kk = clampi(z0 + ty * RZ + rr - S.TU, 0, S.nz - 1);
-            const int q = kk * S.nx + ii;
-            if (I16) G.sv[rr] = R(dequant_i16((short)ldg(V.qsnap, q), ldg(V.qexp, kk * S.nqb + qb)));
-            else if (Q16) G.sv[rr] = R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode));
-            else if (V.sfield) G.sv[rr] = ldg(V.sfield, clampi(z0 + ty * RZ + rr, 0, S.Z - 1) * S.P + ic);
-            else G.sv[rr] = ldg(V.snap, q);
-            G.Iv[rr] = ldg(S.I, q);
-            G.Isv[rr] = ldg(S.Is, q);
-            if (S.flags & F_KAHAN) {
-                G.Icv[rr] = ldg(S.Ic, q);
-                G.Iscv[rr] = ldg(S.Isc, q);
-            }
-        }
-    }
+    if (IMG_EARLY) tile_fetch_img<R, RZ, Q16, I16, NW>(G, S, V, x0, z0, tx, ty);
     G.recr = -1;
     G.recv = R(0);
     const int rrow = S.rec_row - (z0 + ty * RZ);
@@ -581,6 +607,10 @@ RTM_HD void tile_stage(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, int 
         const int r = ty + NW * t + 8 - HZ;
         L.u[r][tx + 8 - HX] = G.u0[t];
         L.u[r][cb] = G.u1[t];
+    }
+    if ((CLS & T_ZSTRIP) && ty == 0 && tx < NW * RZ + 8) {
+        L.az[tx] = G.azt;
+        L.bz[tx] = G.bzt;
     }
 }
 
@@ -618,7 +648,7 @@ RTM_HD void tile_psi(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, const 
                 const int lr = r + 4;
                 const R d1 = d1_val(L.u[lr + 1][lc], L.u[lr + 2][lc], L.u[lr + 3][lc], L.u[lr + 4][lc],
                                     L.u[lr - 1][lc], L.u[lr - 2][lc], L.u[lr - 3][lc], L.u[lr - 4][lc]);
-                const R pv = psi_val(ldg(S.az, k), ldg(S.bz, k), G.pz[t], d1, S.h1);
+                const R pv = psi_val(L.az[r], L.bz[r], G.pz[t], d1, S.h1);   /* row k = z0-4+r */
                 v = iok ? pv : R(0);
             }
             L.pz[r][tx] = v;
@@ -627,7 +657,9 @@ RTM_HD void tile_psi(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, const 
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4>
+/* XS: the instance carries the snapshot store and the imaging (false: a
+ * light instance for steps that have neither, with fewer registers). */
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool XS = true>
 RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
@@ -679,7 +711,7 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
                                   L.pz[pr - 1][tx], L.pz[pr - 2][tx], L.pz[pr - 3][tx], L.pz[pr - 4][tx])
                            * S.h1;
             const R d2 = d2_val(uc, zp1, zp2, zp3, zp4, zm1, zm2, zm3, zm4);
-            csz = csi_val(ldg(S.az, k), ldg(S.bz, k), G.csz[rr], d2, dpsi, S.h3);
+            csz = csi_val(L.az[kl + 4], L.bz[kl + 4], G.csz[rr], d2, dpsi, S.h3);   /* row k = z0+kl */
             corrZ = corr_val(Cj, S.h2, dpsi, csz);
         }
         /* additions in the reference's strip order: L, T, R, B */
@@ -706,7 +738,9 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
             if (V.forward && rec_row) stg(V.rec_row_out, G.recr, float(U));
             const int kk = k - S.TU;
             const bool img_pt = img_col && kk >= 0 && kk < S.nz;
-            if (I16 && V.forward) {
+            if (!XS) {
+                /* a light instance: no store and no imaging on this step */
+            } else if (I16 && V.forward) {
                 /* the int16 block store follows the update (tile_store_i16) */
                 if (img_pt) G.uq[rr] = U;
             } else if (Q16 && V.forward) {
@@ -731,7 +765,7 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
             }
         }
     }
-    if (Q16 && V.forward && V.qsnap && img_col) {
+    if (XS && Q16 && V.forward && V.qsnap && img_col) {
         /* 16-bit store of u^n (the previous step's snapshot) from the tile's
          * LDS copy, apart from the update loop: kept inside it, u^n and the
          * store's operands stayed live across the loop and slowed every step */
