@@ -120,9 +120,32 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *
     case 0: run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
-    default: run_tile<R, RZ, 3, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    default:
+        if constexpr (rtm::corner_tiles_small(RZ, NW)) {
+            // the lists cut these corner tiles into 64 x 4 tiles (its LDS
+            // holds one psi plane): flag the list as broken
+            if (threadIdx.x == 0 && err) atomicOr(err, 2);
+        } else {
+            run_tile<R, RZ, 3, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+        }
+        break;
     }
 }
+
+// A step kernel's LDS: the tile's, and, where corner tiles come as 64 x 4
+// tiles (rtm::corner_tiles_small), the small tile's in a union with it.
+template <typename R, int RZ, int NW, bool SMALL = rtm::corner_tiles_small(RZ, NW)>
+struct StepLds {
+    rtm::TileLds<R, RZ, NW> full;
+};
+
+template <typename R, int RZ, int NW>
+struct StepLds<R, RZ, NW, true> {
+    union {
+        rtm::TileLds<R, 1> small;
+        rtm::TileLds<R, RZ, NW> full;
+    };
+};
 
 // Q16: every sim of the launch keeps a 16-bit snapshot store (the store
 // kind is a property of the whole plan or batch, so the FP32-store kernels
@@ -141,7 +164,7 @@ __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict_
                                                    const int4 *__restrict__ tiles,
                                                    const int *__restrict__ ntiles_p, int s, int *err)
 {
-    __shared__ rtm::TileLds<R, RZ, NW> L;
+    __shared__ StepLds<R, RZ, NW> L;
     __shared__ float red[NW];
     const int ntiles = *ntiles_p;
     const int per = (ntiles + 7) >> 3;
@@ -157,7 +180,14 @@ __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict_
     const int n = n0[sim] + s;
     const int tx = threadIdx.x & 63;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row group per wavefront
-    run_tile_cls<R, RZ, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile, tx, ty, err);
+    if constexpr (rtm::corner_tiles_small(RZ, NW)) {
+        if (tile.w & rtm::T_SMALL) {   // a 64 x 4 piece of a corner tile: one row per thread
+            const int4 t4 = make_int4(tile.x, tile.y, tile.z, tile.w & 3);
+            run_tile_cls<R, 1, Q16, I16, 4, UM, LIGHT>(L.small, red, S, n, t4, tx, ty, err);
+            return;
+        }
+    }
+    run_tile_cls<R, RZ, Q16, I16, NW, UM, LIGHT>(L.full, red, S, n, tile, tx, ty, err);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run

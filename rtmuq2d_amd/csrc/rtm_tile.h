@@ -326,12 +326,35 @@ RTM_HD unsigned snap_max_bits(const unsigned *slots)
 #endif
 }
 
-template <typename R, int RZ, int NW = 4>
-struct TileLds {
+/* The psi^{n+1} planes of a tile: psiX for x-strip tiles, psiZ for z-strip
+ * tiles.  Only a corner tile (both strips) needs both at once; the work
+ * lists cut corner tiles of four row groups into 64 x 4 tiles
+ * (corner_tiles_small), so those tiles keep the two planes in one union and
+ * the LDS of a workgroup stays within an eighth of the CU's. */
+template <typename R, int RZ, int NW, bool UNI>
+struct TilePsiLds {
     static constexpr int TZ = NW * RZ;
-    R u[TZ + 16][80];       /* rows z0-8 .. z0+TZ+7, columns x0-8 .. x0+71 */
     R px[TZ][72];           /* psiX^{n+1}: rows z0 .., columns x0-4 .. x0+67 */
     R pz[TZ + 8][64];       /* psiZ^{n+1}: rows z0-4 .., columns x0 .. x0+63 */
+};
+
+template <typename R, int RZ, int NW>
+struct TilePsiLds<R, RZ, NW, true> {
+    static constexpr int TZ = NW * RZ;
+    union {
+        R px[TZ][72];
+        R pz[TZ + 8][64];
+    };
+};
+
+/* Tiles of four row groups and more than one row per thread never are corner
+ * tiles (the lists cut those into 64 x 4 tiles). */
+constexpr bool corner_tiles_small(int rz, int nw) { return rz > 1 && nw == 4; }
+
+template <typename R, int RZ, int NW = 4>
+struct TileLds : TilePsiLds<R, RZ, NW, corner_tiles_small(RZ, NW)> {
+    static constexpr int TZ = NW * RZ;
+    R u[TZ + 16][80];       /* rows z0-8 .. z0+TZ+7, columns x0-8 .. x0+71 */
     R az[TZ + 8], bz[TZ + 8];   /* z profiles of rows z0-4 .. z0+TZ+3 (z-strip tiles) */
 };
 

@@ -79,7 +79,10 @@ inline void build_work(std::vector<WorkItem> &tiles, std::vector<WorkItem> &inne
             if (r.zA >= 0 && z0 >= r.zA && z0 < r.zB) continue;
             if (!D2.empty() && in_d2(x0, z0)) continue;
             const int cls = tile_class(d, x0, z0, TZ);
-            if (interior == IN_MIXED && cls != 0) {
+            /* mixed lists cut every strip tile into 64 x 4 tiles; the other
+             * four-row-group lists cut corner tiles (corner_tiles_small) */
+            if ((interior == IN_MIXED && cls != 0) ||
+                (cls == (T_XSTRIP | T_ZSTRIP) && interior != IN_TILES8 && corner_tiles_small(rz, 4))) {
                 for (int zs = z0; zs < z0 + TZ && zs < Z - kHalf; zs += 4)
                     tiles.push_back({sim, x0, zs, tile_class(d, x0, zs, 4) | T_SMALL});
                 continue;

@@ -3,6 +3,8 @@
 // without a GPU.  Built by tests/test_cpu_harness.py with
 // g++ -O2 -ffp-contract=off; test infrastructure only.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstddef>
 #include <vector>
 
@@ -210,12 +212,12 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
     rtm::build_work(tiles, inner, 0, S.X, S.Z, S.TL, S.TR, S.TU, S.TD, RZ, interior, seg_rows);
     for (const auto &t : tiles) {
         // the LDS holds stale data from the previous tile, as on the GPU
-        if (t.d & rtm::T_SMALL) {   // a 64 x 4 tile of a mixed list (k_tiles_mix)
+        if (t.d & rtm::T_SMALL) {   // a 64 x 4 tile: of a mixed list (k_tiles_mix), or a corner piece
             switch (t.d & 3) {
-            case 0: run_tile<R, 1, 0, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
-            case 1: run_tile<R, 1, 1, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
-            case 2: run_tile<R, 1, 2, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
-            default: run_tile<R, 1, 3, Q16>(*Ls, Gs, S, V, t.b, t.c); break;
+            case 0: run_tile<R, 1, 0, Q16, I16>(*Ls, Gs, S, V, t.b, t.c); break;
+            case 1: run_tile<R, 1, 1, Q16, I16>(*Ls, Gs, S, V, t.b, t.c); break;
+            case 2: run_tile<R, 1, 2, Q16, I16>(*Ls, Gs, S, V, t.b, t.c); break;
+            default: run_tile<R, 1, 3, Q16, I16>(*Ls, Gs, S, V, t.b, t.c); break;
             }
             continue;
         }
@@ -232,7 +234,15 @@ static void run_tiles_q(const rtm::SimDesc<R> &S, int n, int interior, int seg_r
         case 0: run_tile<R, RZ, 0, Q16, I16>(*L, G, S, V, t.b, t.c); break;
         case 1: run_tile<R, RZ, 1, Q16, I16>(*L, G, S, V, t.b, t.c); break;
         case 2: run_tile<R, RZ, 2, Q16, I16>(*L, G, S, V, t.b, t.c); break;
-        default: run_tile<R, RZ, 3, Q16, I16>(*L, G, S, V, t.b, t.c); break;
+        default:
+            if constexpr (rtm::corner_tiles_small(RZ, 4)) {
+                // the kernels' LDS holds one psi plane here; the lists cut corners
+                fprintf(stderr, "harness: a %d-row corner tile in the list\n", 4 * RZ);
+                abort();
+            } else {
+                run_tile<R, RZ, 3, Q16, I16>(*L, G, S, V, t.b, t.c);
+            }
+            break;
         }
     }
     if (interior == rtm::IN_PAIRS) {

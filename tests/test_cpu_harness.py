@@ -434,8 +434,9 @@ def test_pair_lists_cover_the_stepped_region_once(hx, rz, seg):
         TZ = 4 * rz
         t = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, 4, seg)
         cover = np.zeros((Z, X), np.int32)
-        for _, x0, z0, _ in t:
-            cover[z0:min(z0 + TZ, Z - 4), x0:min(x0 + 64, X - 4)] += 1
+        for _, x0, z0, cls in t:
+            h = 4 if cls & 4 else TZ   # corner tiles come as 64 x 4 pieces
+            cover[z0:min(z0 + h, Z - 4), x0:min(x0 + 64, X - 4)] += 1
         cap = 100000
         buf = np.zeros(4 * cap, np.int32)
         n = hx.hx_inner_list(X, Z, TL, TR, TU, TD, rz, 4, seg, buf.ctypes.data_as(C.c_void_p), cap)
@@ -458,3 +459,20 @@ def test_pair_lists_cover_the_stepped_region_once(hx, rz, seg):
             zs, ze = items[:, 2].min(), items[:, 3].max()
             assert (xs - 4) % 64 == 0 and (xe - 4) % 64 == 0 and (zs - 4) % TZ == 0 and (ze - 4) % TZ == 0
             assert (ze - zs) * (xe - xs) == sum((int(b) - int(a)) * 2 * (int(p) >> 24) for p, _, a, b in items)
+
+
+@pytest.mark.parametrize("rz,interior", [(2, 0), (4, 0), (8, 0), (4, 1), (2, 4), (1, 0), (2, 3)])
+def test_corner_tiles_come_as_small_tiles(hx, rz, interior):
+    """Lists of 64 x 4rz tiles with rz > 1 and four row groups carry no
+    corner tile (both strips): those come as 64 x 4 pieces, since the step
+    kernel's LDS then holds one psi plane per tile; the pieces and the other
+    tiles cover the stepped region once.  Tiles of 8 row groups and 64 x 4
+    tiles keep their corner tiles whole."""
+    for X, Z, TL, TR, TU, TD in GRIDS:
+        t = _tiles(hx, hx.hx_work_list, X, Z, TL, TR, TU, TD, rz, interior, 16)
+        full = t[(t[:, 3] & 4) == 0]
+        cut = rz > 1 and interior != 3
+        assert ((full[:, 3] & 3) == 3).any() != cut
+        if cut:
+            small = t[(t[:, 3] & 4) != 0]
+            assert len(small) > 0 and ((small[:, 2] - 4) % 4 == 0).all()
