@@ -83,16 +83,16 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
 }
 
 // LIGHT: an instance without the snapshot store and the imaging, for the
-// steps the executor knows have neither (fewer registers, a higher
-// occupancy).  Meeting a step that stores or images sets *err, which the
-// executor reports as an error after the pass.
+// steps the executor's schedule says have neither (fewer registers, a
+// higher occupancy, and a prologue without the snapshot index).  The
+// schedule (Exec::heavy_step) and the store / imaging tests of
+// rtm::step_view_q are the same formulas.
 template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                          int x0, int z0, int tx, int ty, int *err)
 {
-    const rtm::StepView<R> V = rtm::step_view_q<R, Q16, I16, UM>(S, n);
+    const rtm::StepView<R> V = LIGHT ? rtm::step_view_light<R, UM>(S, n) : rtm::step_view_q<R, Q16, I16, UM>(S, n);
     if (!V.active) return;
-    if (LIGHT && (V.snap || V.sfield || V.qsnap || V.smax) && threadIdx.x == 0) atomicOr(err, 1);
     rtm::TileRegs<R, RZ, NW> G;
     rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, !LIGHT>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS, NW>(L, G, tx, ty);

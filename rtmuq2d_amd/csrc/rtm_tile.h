@@ -445,6 +445,36 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
     return v;
 }
 
+/* The view of a step that stores no snapshot and images nothing (the light
+ * tile instances): no snapshot index, so no division by TS in the
+ * prologue every wavefront runs. */
+template <typename R, int UM = 1>
+RTM_HD StepView<R> step_view_light(const SimDesc<R> &S, int n)
+{
+    StepView<R> v{};
+    v.n = n;
+    v.active = S.phase != PH_IDLE && n >= 0 && n < S.nt;
+    if (!v.active) return v;
+    v.forward = S.phase == PH_FORWARD;
+    v.u = S.ubuf[n & UM];
+    v.U = S.ubuf[(n + 1) & UM];
+    v.Uo = UM == 1 ? v.U : S.ubuf[(n - 1) & UM];
+    v.psiXi = S.psiX[n & 1];
+    v.psiXo = S.psiX[(n + 1) & 1];
+    v.psiZi = S.psiZ[n & 1];
+    v.psiZo = S.psiZ[(n + 1) & 1];
+    /* every field from selected values, no branch: fields stored on some
+     * paths only would put the whole view in scratch memory */
+    v.src = v.forward && n < S.nw && S.src_j >= 0;
+    const R w = R(ldg(S.wav, v.src ? n : 0));
+    v.src_w = v.src ? w : R(0);
+    float *rec = S.seis_rec + (size_t)n * S.nrec;
+    const float *seis = S.seis_t + (size_t)(S.nt - 1 - n) * S.nrec;
+    v.rec_row_out = (v.forward && (S.flags & F_RECORD)) ? rec : nullptr;
+    v.seis_row = v.forward ? nullptr : seis;
+    return v;
+}
+
 template <typename R>
 RTM_HD StepView<R> step_view(const SimDesc<R> &S, int n)
 {
