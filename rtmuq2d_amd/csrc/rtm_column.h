@@ -58,13 +58,43 @@ RTM_HD Pair<R> col_update(const Pair<R> *w, Pair<R> L1, Pair<R> L2, Pair<R> R1, 
     return U;
 }
 
+/* A lane's receivers for a column item, loaded before the march: the
+ * receiver at each of its two columns on the receiver row (-1: none) and,
+ * backward, the trace samples that replace u there.  `on_item`: the
+ * receiver row is one of the item's rows (uniform). */
+template <typename R>
+struct ColRec {
+    int ra, rb;
+    R va, vb;
+};
+
+template <typename R>
+RTM_HD ColRec<R> col_rec(const SimDesc<R> &S, const StepView<R> &V, int xl, bool ok, bool on_item)
+{
+    ColRec<R> r{-1, -1, R(0), R(0)};
+    if (!on_item || !(!V.forward || V.rec_row_out)) return r;   /* uniform */
+    const int xc = clampi(xl, 0, S.X - 2);
+    const int ra = ldg(S.recmap, xc), rb = ldg(S.recmap, xc + 1);
+    r.ra = ok ? ra : -1;
+    r.rb = ok ? rb : -1;
+    if (!V.forward) {
+        const R va = R(ldg(V.seis_row, r.ra > 0 ? r.ra : 0)), vb = R(ldg(V.seis_row, r.rb > 0 ? r.rb : 0));
+        r.va = va;
+        r.vb = vb;
+    }
+    return r;
+}
+
 /* Source add, receiver replacement or recording, snapshot store (FP32, or
  * the 16-bit store's max |U| and quantization of u^n) and imaging for the
  * lane's two points at row k (in the order of tile_points); the row tests
- * are uniform across the wavefront.  uc = u^n at the two points. */
-template <typename R, bool Q16 = false>
-RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl, bool ok, int src_row, Pair<R> &U,
-                       Pair<R> Cp, Pair<R> uc, float &umax)
+ * are uniform across the wavefront.  uc = u^n at the two points.  Only the
+ * imaging loads memory, and only the IMG instance images (the item's step
+ * is an imaging step): a load in a branch that some rows take would make
+ * the march wait for all its prefetched rows at every row after it. */
+template <typename R, bool Q16 = false, bool IMG = true>
+RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, const ColRec<R> &rec, int k, int xl, bool ok,
+                       int src_row, Pair<R> &U, Pair<R> Cp, Pair<R> uc, float &umax)
 {
     if (V.forward) {
         if (k == src_row) {
@@ -74,14 +104,12 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
         }
     }
     if (k == S.rec_row && (!V.forward || V.rec_row_out)) {
-        const int ra = ok ? ldg(S.recmap, xl) : -1;
-        const int rb = ok ? ldg(S.recmap, xl + 1) : -1;
         if (V.forward) {
-            if (ra >= 0) stg(V.rec_row_out, ra, float(U.a));
-            if (rb >= 0) stg(V.rec_row_out, rb, float(U.b));
+            if (rec.ra >= 0) stg(V.rec_row_out, rec.ra, float(U.a));
+            if (rec.rb >= 0) stg(V.rec_row_out, rec.rb, float(U.b));
         } else {
-            if (ra >= 0) U.a = R(ldg(V.seis_row, ra));
-            if (rb >= 0) U.b = R(ldg(V.seis_row, rb));
+            U.a = rec.ra >= 0 ? rec.va : U.a;
+            U.b = rec.rb >= 0 ? rec.vb : U.b;
         }
     }
     const int kk = k - S.TU;
@@ -101,7 +129,7 @@ RTM_HD void col_extras(const SimDesc<R> &S, const StepView<R> &V, int k, int xl,
                 if (V.smax) umax = fmaxf(umax, fabsf(float(vals[e])));
                 if (V.qsnap) stg_stream(V.qsnap, q, quant16(float(us[e]), V.qscale, S.qmode));
             }
-        } else {
+        } else if (IMG) {
             const R s = Q16        ? R(dequant16(ldg(V.qsnap, q), V.qscale, S.qmode))
                         : V.sfield ? ldg(V.sfield, k * S.P + xl + e)
                                    : ldg(V.snap, q);

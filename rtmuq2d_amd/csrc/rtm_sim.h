@@ -294,9 +294,10 @@ __device__ __forceinline__ void col_fetch(rtm::Pair<R> (&w)[16], ColSet<R> &st, 
 // Rows k0 .. k0+3; row k0+rr's window rows k-4 .. k+4 are slots
 // (B + rr + j) & 15, j = 0..8.  EXTRAS: some row of the chunk has a source,
 // receiver, snapshot or imaging term.
-template <typename R, int B, bool EXTRAS, bool Q16>
+template <typename R, int B, bool EXTRAS, bool Q16, bool IMG>
 __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const ColSet<R> &st, const ColCtx<R> &X,
-                                          const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, float &umax)
+                                          const SimDesc<R> &S, const rtm::StepView<R> &V,
+                                          const rtm::ColRec<R> &rec, int k0, float &umax)
 {
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
@@ -316,25 +317,25 @@ __device__ __forceinline__ void col_chunk(const rtm::Pair<R> (&w)[16], const Col
         R2.a = lane_shl1(R1.a, H.z);
         R2.b = lane_shl1(R1.b, H.w);
         rtm::Pair<R> U = rtm::col_update(win, L1, L2, R1, R2, st.C[rr], st.O[rr]);
-        if (EXTRAS) rtm::col_extras<R, Q16>(S, V, k, X.xl, X.ok, X.srow, U, st.C[rr], c, umax);
+        if (EXTRAS) rtm::col_extras<R, Q16, IMG>(S, V, rec, k, X.xl, X.ok, X.srow, U, st.C[rr], c, umax);
         if (X.ok) rtm::stg(X.U2 + k * X.Ph, X.xh, U);
     }
 }
 
 // Chunk c of the four-chunk cycle (rows k0 .. k0+3): prefetch the next
 // chunk into the other set, then compute this one.
-template <typename R, int CI, bool Q16>
+template <typename R, int CI, bool Q16, bool IMG>
 __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, ColSet<R> &nxt, const ColCtx<R> &X,
-                                         const SimDesc<R> &S, const rtm::StepView<R> &V, int k0, int zB,
-                                         int ex_lo, int ex_hi, float &umax)
+                                         const SimDesc<R> &S, const rtm::StepView<R> &V, const rtm::ColRec<R> &rec,
+                                         int k0, int zB, int ex_lo, int ex_hi, float &umax)
 {
     col_fetch<R, (4 * CI + 12) & 15>(w, nxt, X, k0 + 4, zB);
     // rows [ex_lo, ex_hi) carry snapshot/imaging terms; single rows the
     // source or receiver terms
     const bool extras = (k0 < ex_hi && k0 + 4 > ex_lo) || (X.srow >= k0 && X.srow < k0 + 4)
                         || (S.rec_row >= k0 && S.rec_row < k0 + 4 && (!V.forward || V.rec_row_out));
-    if (extras) col_chunk<R, 4 * CI, true, Q16>(w, cur, X, S, V, k0, umax);
-    else col_chunk<R, 4 * CI, false, Q16>(w, cur, X, S, V, k0, umax);
+    if (extras) col_chunk<R, 4 * CI, true, Q16, IMG>(w, cur, X, S, V, rec, k0, umax);
+    else col_chunk<R, 4 * CI, false, Q16, IMG>(w, cur, X, S, V, rec, k0, umax);
 }
 
 // The interior column kernel (rtm_column.h): four independent wavefronts
@@ -345,12 +346,10 @@ __device__ __forceinline__ void col_step(rtm::Pair<R> (&w)[16], ColSet<R> &cur, 
 // are remapped so that each XCD walks a contiguous run of items
 // (consecutive segments of a column share rows).
 // One column item (the body of k_columns), for a sim without (Q16 false)
-// or with a 16-bit snapshot store.
-template <typename R, bool Q16>
-__device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it, int item)
+// or with a 16-bit snapshot store; IMG: the instance for imaging steps.
+template <typename R, bool Q16, bool IMG>
+__device__ __forceinline__ void column_march(const SimDesc<R> &S, const rtm::StepView<R> &V, int4 it, int item)
 {
-    const rtm::StepView<R> V = rtm::step_view_q<R, Q16>(S, n);
-    if (!V.active) return;
     const int lane = threadIdx.x & 63;
     const int x0 = it.y, zA = it.z, zB = it.w;
     const int P = S.P;
@@ -370,6 +369,7 @@ __device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it,
     // imaging (backward, from the snapshot store or the rebuilt source field)
     const bool img = Q16 ? (V.qsnap || V.smax) : (V.snap || V.sfield);
     const int ex_lo = img ? S.TU : 0, ex_hi = img ? S.TU + S.nz : 0;
+    const rtm::ColRec<R> rec = rtm::col_rec(S, V, X.xl, X.ok, S.rec_row >= zA && S.rec_row < zB);
     float umax = 0.0f;
     rtm::Pair<R> w[16];
 #pragma unroll
@@ -377,16 +377,27 @@ __device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it,
     ColSet<R> A, Bs;
     col_fetch<R, 8>(w, A, X, zA, zB);
     for (int k0 = zA;; k0 += 16) {
-        col_step<R, 0, Q16>(w, A, Bs, X, S, V, k0, zB, ex_lo, ex_hi, umax);
+        col_step<R, 0, Q16, IMG>(w, A, Bs, X, S, V, rec, k0, zB, ex_lo, ex_hi, umax);
         if (k0 + 4 >= zB) break;
-        col_step<R, 1, Q16>(w, Bs, A, X, S, V, k0 + 4, zB, ex_lo, ex_hi, umax);
+        col_step<R, 1, Q16, IMG>(w, Bs, A, X, S, V, rec, k0 + 4, zB, ex_lo, ex_hi, umax);
         if (k0 + 8 >= zB) break;
-        col_step<R, 2, Q16>(w, A, Bs, X, S, V, k0 + 8, zB, ex_lo, ex_hi, umax);
+        col_step<R, 2, Q16, IMG>(w, A, Bs, X, S, V, rec, k0 + 8, zB, ex_lo, ex_hi, umax);
         if (k0 + 12 >= zB) break;
-        col_step<R, 3, Q16>(w, Bs, A, X, S, V, k0 + 12, zB, ex_lo, ex_hi, umax);
+        col_step<R, 3, Q16, IMG>(w, Bs, A, X, S, V, rec, k0 + 12, zB, ex_lo, ex_hi, umax);
         if (k0 + 16 >= zB) break;
     }
     if (Q16 && V.smax) commit_snap_max(V.smax, umax, item);
+}
+
+template <typename R, bool Q16>
+__device__ __forceinline__ void column_item(const SimDesc<R> &S, int n, int4 it, int item)
+{
+    const rtm::StepView<R> V = rtm::step_view_q<R, Q16>(S, n);
+    if (!V.active) return;
+    // imaging steps (backward, a snapshot or the rebuilt source field) run
+    // the instance that loads the imaging operands inside the march
+    if (!V.forward && (Q16 ? V.qsnap != nullptr : (V.snap || V.sfield))) column_march<R, Q16, true>(S, V, it, item);
+    else column_march<R, Q16, false>(S, V, it, item);
 }
 
 template <typename R, bool Q16>

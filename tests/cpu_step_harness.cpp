@@ -93,6 +93,7 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
     const int srow = rtm::col_src_row(S, V);
     float umax = 0.0f;
     auto at = [&](const R *f, int k, int x) { return f[(size_t)k * P + x]; };
+    const bool on_item = S.rec_row >= zA && S.rec_row < zB;
     for (int k = zA; k < zB; k++) {
         P2 c[64], L1[64], L2[64], R1[64], R2[64];
         rtm::Quad<R> H[64];
@@ -115,7 +116,8 @@ static void run_column_item(const rtm::SimDesc<R> &S, const rtm::StepView<R> &V,
             const P2 Op = {at(V.U, k, xc[l]), at(V.U, k, xc[l] + 1)};
             P2 U = rtm::col_update(w, L1[l], L2[l], R1[l], R2[l], Cp, Op);
             const bool ok = l < (half ? 32 : 64);
-            rtm::col_extras<R, Q16>(S, V, k, xl[l], ok, srow, U, Cp, w[4], umax);
+            rtm::col_extras<R, Q16>(S, V, rtm::col_rec(S, V, xl[l], ok, on_item), k, xl[l], ok, srow, U, Cp, w[4],
+                                    umax);
             if (ok) {
                 V.U[(size_t)k * P + xl[l]] = U.a;
                 V.U[(size_t)k * P + xl[l] + 1] = U.b;
