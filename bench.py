@@ -995,12 +995,14 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
                           us_per_step=round(t["ms_per_step"] * 1e3, 2)) for t in sweep],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "traffic_note": "not measured: under a rocprofv3 counter pass the batched kernels skip their "
-                                     "work (zero images, 3.6 us dispatches), so no counter reading describes them",
-                     "algorithmic_bytes_per_launch": BYTES_PER_UPDATE * X * Z * gsize_,
+                     "traffic_note": "not measured: under a rocprofv3 pass of L2 (TCC) counters the batched "
+                                     "kernels skip their work (zero images), so no L2 reading describes them; "
+                                     "SQ counter passes leave their work intact",
+                     "algorithmic_bytes_per_launch": round(BYTES_PER_UPDATE * X * Z * nsims / ngroups_),
                      "launch_us": round(step_ms * 1e3 / ngroups_, 3),
-                     "launch_note": f"one launch steps a group of {gsize_} realizations; {ngroups_} launches per "
-                                    "time step, the last group ragged"},
+                     "launch_note": f"{ngroups_} launches per time step, one per launch group of at most {gsize_} "
+                                    f"realizations ({nsims} in all); bytes and time are the means over a step's "
+                                    "launches"},
         "welford_seconds": round(welford_s, 4),
         "image0_relL2_vs_fp64": rel_l2(I0, I64),
         "meets_gate": gate(rel_l2(I0, I64)),

@@ -262,8 +262,11 @@ int rtm_batch_get_tiling(rtm_batch *batch, int *rows_per_thread, int *graph_bloc
  * rtm_plan_autotune; ms_per_step is the time per step of that group). */
 int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double *ms_per_step, int *n_out);
 /* Launch groups (no reference counterpart): the realizations run in groups
- * of sims_per_group (0 or >= the batch size: one group), each group's
- * forward and backward passes before the next group's.  Results do not
+ * of at most sims_per_group (0 or >= the batch size: one group), as few
+ * groups as that allows with sizes that differ by at most one (64
+ * realizations at sims_per_group 15: 13, 13, 13, 13, 12), each group's
+ * forward and backward passes before the next group's.  get_group reports
+ * the largest group's size and the number of groups.  Results do not
  * depend on the grouping.  A group whose per-step working set fits the
  * device's last-level cache steps faster than one that does not.  A new
  * batch starts with the group size its grids were last tuned to, else with
@@ -272,7 +275,8 @@ int rtm_batch_autotune(rtm_batch *batch, int steps, int cap, int *shapes, double
 int rtm_batch_set_group(rtm_batch *batch, int sims_per_group);
 int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
 /* Shape-tunes each candidate group size (1/2 to 2 times the size that fills
- * the 224 MiB budget, and the whole batch), timing forward steps that write
+ * the 224 MiB budget, each taken to the largest group of its balanced split,
+ * and the whole batch) on group 0, timing forward steps that write
  * the snapshot stores (overwritten by the next migration), and keeps the
  * smallest group within 2% of the least time per realization-step; row k =
  * group_sizes[k], us_per_sim_step[k]. */

@@ -114,9 +114,10 @@ unsigned char *na_scratch(int device)
 
 struct rtm_batch {
     std::vector<SimBuffers> sims;
-    // one executor per launch group; group k holds sims [k*group, k*group + ex[k].nsims)
+    // one executor per launch group; group k holds sims
+    // [group_first(k), group_first(k) + group_len(k))
     std::vector<Exec> ex;
-    int group = 0;                      // realizations per launch group (0: all)
+    int group = 0;                      // most realizations per launch group (0: all)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int real = RTM_REAL_F32;
@@ -137,11 +138,35 @@ static void batch_free(rtm_batch *q)
     if (q->stream) (void)hipStreamDestroy(q->stream);
 }
 
-static int group_size(const rtm_batch *q)
+// Launch groups: at most q->group realizations each (0: one group), as few
+// groups as that allows, their sizes differing by at most one (the larger
+// ones first), so that no group is a small remainder.
+static int group_limit(const rtm_batch *q)
 {
     const int N = (int)q->sims.size();
     return q->group > 0 && q->group < N ? q->group : N;
 }
+
+static int group_count(const rtm_batch *q)
+{
+    const int N = (int)q->sims.size(), g = group_limit(q);
+    return (N + g - 1) / g;
+}
+
+static int group_first(const rtm_batch *q, int k)
+{
+    const int N = (int)q->sims.size(), ng = group_count(q);
+    return k * (N / ng) + std::min(k, N % ng);
+}
+
+static int group_len(const rtm_batch *q, int k)
+{
+    const int N = (int)q->sims.size(), ng = group_count(q);
+    return N / ng + (k < N % ng ? 1 : 0);
+}
+
+// the size of group 0, the largest
+static int group_size(const rtm_batch *q) { return group_len(q, 0); }
 
 static void batch_drop_execs(rtm_batch *q)
 {
@@ -160,12 +185,12 @@ static int batch_set_tiling(rtm_batch *q, int rz, int block, int interior, int s
         return -1;
     for (auto &b : q->sims)
         if (b.set_quad(interior == rtm::IN_PAIRS)) return -1;
-    const int N = (int)q->sims.size(), g = group_size(q), ng = (N + g - 1) / g;
+    const int ng = group_count(q);
     batch_drop_execs(q);
     q->ex.resize(ng);       // never resized again before the next release
     auto fail = [&]() { batch_drop_execs(q); return -1; };
     for (int k = 0; k < ng; k++) {
-        const int s0 = k * g, n = std::min(g, N - s0);
+        const int s0 = group_first(q, k), n = group_len(q, k);
         std::vector<int4> tiles, inner;
         for (int s = 0; s < n; s++)
             append_work(tiles, inner, s, q->sims[s0 + s].p, rz, interior, seg_rows);
@@ -295,7 +320,7 @@ template <typename R>
 static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
 {
     Exec &ex = q->ex[k];
-    const int s0 = k * group_size(q), N = ex.cap_sims;
+    const int s0 = group_first(q, k), N = ex.cap_sims;
     std::vector<SimDesc<R>> d(N);
     for (int s = 0; s < N; s++) {
         SimBuffers &b = q->sims[s0 + s];
@@ -575,7 +600,9 @@ extern "C" int rtm_batch_get_group(rtm_batch *q, int *sims_per_group, int *group
 
 // Group sizes tried: the size whose per-step working set fills the cache
 // budget (kGroupBytes at 16 B per padded point), h, scaled by 1/2, 5/8, 3/4,
-// 7/8, 1, 5/4, 3/2 and 2, and the whole batch.  Each is shape-tuned with its
+// 7/8, 1, 5/4, 3/2 and 2, each taken to the largest group of the balanced
+// split it gives (64 realizations at most 15 a group: 5 groups, 13 at most),
+// and the whole batch.  Each is shape-tuned with its
 // snapshot stores written, as in a forward pass (a migration's stores and
 // image updates also take cache room, so the best group is smaller than one
 // timed on the fields alone); the smallest group within 2% of the least
@@ -589,10 +616,11 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     const int N = (int)q->sims.size();
     const size_t per = q->sims[0].nfield * (kStepBytesPerPoint / 4) * q->sims[0].esz;
     const int h = (int)std::max<size_t>(1, kGroupBytes / per);
+    // each candidate as the size of the largest of its balanced groups
     std::vector<int> cand;
     for (int f8 : {4, 5, 6, 7, 8, 10, 12, 16}) {
-        const int g = std::max(1, (h * f8 + 7) / 8);
-        if (g < N && (cand.empty() || cand.back() != g)) cand.push_back(g);
+        const int g = std::max(1, (h * f8 + 7) / 8), ng = (N + g - 1) / g, gb = (N + ng - 1) / ng;
+        if (gb < N && (cand.empty() || cand.back() != gb)) cand.push_back(gb);
     }
     cand.push_back(N);
     double best_t = 1e30;

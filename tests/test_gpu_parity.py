@@ -287,8 +287,9 @@ def test_batch_marched_bitexact(gpu_available, interior, seg):
 def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior):
     """A batch of realizations advanced in lockstep: each image bit-identical
     to the oracle's for that realization (alone or in the batch, in one
-    launch group or in groups of 2 or 4 with a ragged last group), and the
-    FP64 Welford mean/variance equal to numpy's over the same images."""
+    launch group or in groups of at most 2 or 4: 5 realizations run as
+    2, 2, 1 or as 3, 2), and the FP64 Welford mean/variance equal to numpy's
+    over the same images."""
     say(f"[gpu] batch ensemble rz={rz} group={group} interior={interior}")
     cfg = common.REF_CONFIGS["asym"]
     g = golden("asym")
@@ -304,7 +305,9 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior)
     with R.Batch(p, len(models)) as b:
         b.set_tiling(rz, 100, interior)
         b.set_group(group)
-        assert b.group() == ((group, (len(models) + group - 1) // group) if group else (len(models), 1))
+        ng = (len(models) + group - 1) // group if group else 1
+        # balanced groups: the largest holds ceil(N / groups)
+        assert b.group() == ((len(models) + ng - 1) // ng, ng)
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
         b.set_wavelet(g["wavelet"])
