@@ -86,22 +86,26 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
 // steps the executor's schedule says have neither (fewer registers, a
 // higher occupancy, and a prologue without the snapshot index).  The
 // schedule (Exec::heavy_step) and the store / imaging tests of
-// rtm::step_view_q are the same formulas.
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
+// rtm::step_view_q are the same formulas.  It has no early exit for an
+// inactive step (its loads stay in bounds and tile_points stores nothing),
+// so its descriptor reads issue together, ahead of any branch.
+// SAFE: an interior tile inside the stepped region (rtm::tile_safe).
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false,
+          bool SAFE = false>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                          int x0, int z0, int tx, int ty, int *err)
 {
     const rtm::StepView<R> V = LIGHT ? rtm::step_view_light<R, UM>(S, n) : rtm::step_view_q<R, Q16, I16, UM>(S, n);
-    if (!V.active) return;
+    if (!LIGHT && !V.active) return;
     rtm::TileRegs<R, RZ, NW> G;
-    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, !LIGHT>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, !LIGHT, SAFE>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS, NW>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
         rtm::tile_psi<R, RZ, CLS, NW>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW, !LIGHT>(L, G, S, V, x0, z0, tx, ty);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW, !LIGHT, SAFE>(L, G, S, V, x0, z0, tx, ty);
     if (LIGHT) return;
     if (Q16 && V.smax) commit_snap_max_wg<NW>(V.smax, m, red);
     if (I16 && V.forward && V.qsnap) {   /* uniform: a store step */
@@ -117,7 +121,12 @@ __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *
                                              int4 tile, int tx, int ty, int *err)
 {
     switch (tile.w) {
-    case 0: run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    case 0:
+        if (rtm::tile_safe(S, tile.y, tile.z, NW * RZ))
+            run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT, true>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+        else
+            run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+        break;
     case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     default:

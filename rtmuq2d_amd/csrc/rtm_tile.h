@@ -451,11 +451,16 @@ RTM_HD StepView<R> step_view_q(const SimDesc<R> &S, int n)
 template <typename R, int UM = 1>
 RTM_HD StepView<R> step_view_light(const SimDesc<R> &S, int n)
 {
+    /* every descriptor field read up front and combined without branches,
+     * so the reads issue together instead of one dependent round trip per
+     * condition; an inactive step (n outside [0, nt)) gets a view whose
+     * reads stay in bounds (trace row clamped, wavelet sample 0) and the
+     * caller stores nothing */
+    const int phase = S.phase, nt = S.nt, nw = S.nw, src_j = S.src_j, flags = S.flags, nrec = S.nrec;
     StepView<R> v{};
     v.n = n;
-    v.active = S.phase != PH_IDLE && n >= 0 && n < S.nt;
-    if (!v.active) return v;
-    v.forward = S.phase == PH_FORWARD;
+    v.active = (phase != PH_IDLE) & (n >= 0) & (n < nt);
+    v.forward = phase == PH_FORWARD;
     v.u = S.ubuf[n & UM];
     v.U = S.ubuf[(n + 1) & UM];
     v.Uo = UM == 1 ? v.U : S.ubuf[(n - 1) & UM];
@@ -465,12 +470,13 @@ RTM_HD StepView<R> step_view_light(const SimDesc<R> &S, int n)
     v.psiZo = S.psiZ[(n + 1) & 1];
     /* every field from selected values, no branch: fields stored on some
      * paths only would put the whole view in scratch memory */
-    v.src = v.forward && n < S.nw && S.src_j >= 0;
+    v.src = v.forward & (n < nw) & (src_j >= 0);
     const R w = R(ldg(S.wav, v.src ? n : 0));
     v.src_w = v.src ? w : R(0);
-    float *rec = S.seis_rec + (size_t)n * S.nrec;
-    const float *seis = S.seis_t + (size_t)(S.nt - 1 - n) * S.nrec;
-    v.rec_row_out = (v.forward && (S.flags & F_RECORD)) ? rec : nullptr;
+    const int nc = clampi(n, 0, nt - 1);
+    float *rec = S.seis_rec + (size_t)nc * nrec;
+    const float *seis = S.seis_t + (size_t)(nt - 1 - nc) * nrec;
+    v.rec_row_out = (v.forward & ((flags & F_RECORD) != 0)) ? rec : nullptr;
     v.seis_row = v.forward ? nullptr : seis;
     return v;
 }
@@ -508,6 +514,15 @@ RTM_HD int tile_class(const SimDesc<R> &S, int x0, int z0, int TZ)
     return cls;
 }
 
+/* An interior tile inside the stepped region, halo included: columns
+ * x0-4 .. x0+67 and rows z0-4 .. z0+TZ+3 lie in the grid and every point of
+ * the tile is stepped (i < X-4, k < Z-4), so its loads need no clamps and
+ * its points no bounds tests (the SAFE instances of the tile phases). */
+template <typename R>
+RTM_HD bool tile_safe(const SimDesc<R> &S, int x0, int z0, int TZ)
+{
+    return x0 >= kHalf && z0 >= kHalf && x0 + 64 + kHalf <= S.X && z0 + TZ + kHalf <= S.Z;
+}
 
 /* Per-thread values fetched from HBM before the first barrier, so that a
  * step costs one memory round trip: the thread's share of the u tile and the
@@ -567,32 +582,43 @@ RTM_HD void tile_fetch_img(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const St
 
 /* Phase 1a: every global load of the step (the imaging operands unless
  * IMG_EARLY is false).  tx in [0,64), ty in [0,4) (the wavefront index:
- * uniform within a wavefront).  CLS is the tile class. */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool IMG_EARLY = true>
+ * uniform within a wavefront).  CLS is the tile class.  SAFE: an interior
+ * tile whose halo lies inside the grid (tile_safe), read without clamps or
+ * masks. */
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool IMG_EARLY = true,
+          bool SAFE = false>
 RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepView<R> &V, int x0, int z0,
                        int tx, int ty)
 {
+    static_assert(!SAFE || CLS == 0, "only interior tiles take the unclamped path");
     constexpr int HX = (CLS & T_XSTRIP) ? 8 : 4, HZ = (CLS & T_ZSTRIP) ? 8 : 4;
     constexpr int NR = RZ + 2 * HZ / NW;             /* (TZ + 2 HZ) / NW rows per thread */
+    const int i = x0 + tx;
+    const int ic = SAFE ? i : clampi(i, 0, S.X - 1);
+    /* the receiver column first: the trace sample it selects is read after
+     * the field loads, and waiting for this load then leaves them in flight */
+    G.recr = -1;
+    G.recv = R(0);
+    const int rrow = S.rec_row - (z0 + ty * RZ);
+    const bool on_rec = rrow >= 0 && rrow < RZ && (!V.forward || V.rec_row_out);   /* uniform */
+    if (on_rec) G.recr = (SAFE || i < S.X - kHalf) ? ldg(S.recmap, ic) : -1;
     /* column blocks: 64 columns, then the 2*HX remaining ones (lanes repeat) */
     const int ia = x0 - HX + tx, ib = x0 - HX + 64 + (tx % (2 * HX));
-    const bool oka = ia >= 0 && ia < S.X, okb = ib < S.X;
-    const int ca = clampi(ia, 0, S.X - 1), cb = clampi(ib, 0, S.X - 1);
+    const bool oka = SAFE || (ia >= 0 && ia < S.X), okb = SAFE || ib < S.X;
+    const int ca = SAFE ? ia : clampi(ia, 0, S.X - 1), cb = SAFE ? ib : clampi(ib, 0, S.X - 1);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
         const int k = z0 - HZ + ty + NW * t;
-        const bool okk = k >= 0 && k < S.Z;
-        const int base = clampi(k, 0, S.Z - 1) * S.P;
+        const bool okk = SAFE || (k >= 0 && k < S.Z);
+        const int base = (SAFE ? k : clampi(k, 0, S.Z - 1)) * S.P;
         const R a = ldg(V.u, base + ca), b = ldg(V.u, base + cb);
         G.u0[t] = (okk && oka) ? a : R(0);
         G.u1[t] = (okk && okb) ? b : R(0);
     }
-    const int i = x0 + tx;
-    const int ic = clampi(i, 0, S.X - 1);
     const bool inX = i < S.TL - kHalf || (i >= S.X - S.TR + kHalf && i < S.X - kHalf);
 #pragma unroll
     for (int rr = 0; rr < RZ; rr++) {
-        const int k = clampi(z0 + ty * RZ + rr, 0, S.Z - 1);
+        const int k = SAFE ? z0 + ty * RZ + rr : clampi(z0 + ty * RZ + rr, 0, S.Z - 1);
         const int j = k * S.P + ic;
         G.C[rr] = ldg(S.C, j);
         G.Uold[rr] = ldg(V.Uo, j);
@@ -636,15 +662,9 @@ RTM_HD void tile_fetch(TileRegs<R, RZ, NW> &G, const SimDesc<R> &S, const StepVi
         G.bzt = ldg(S.bz, kp);
     }
     if (IMG_EARLY) tile_fetch_img<R, RZ, Q16, I16, NW>(G, S, V, x0, z0, tx, ty);
-    G.recr = -1;
-    G.recv = R(0);
-    const int rrow = S.rec_row - (z0 + ty * RZ);
-    if (rrow >= 0 && rrow < RZ && (!V.forward || V.rec_row_out)) {   /* uniform */
-        G.recr = i < S.X - kHalf ? ldg(S.recmap, ic) : -1;
-        if (!V.forward) {
-            const R v = R(ldg(V.seis_row, G.recr > 0 ? G.recr : 0));
-            G.recv = G.recr >= 0 ? v : R(0);
-        }
+    if (on_rec && !V.forward) {   /* uniform */
+        const R v = R(ldg(V.seis_row, G.recr > 0 ? G.recr : 0));
+        G.recv = G.recr >= 0 ? v : R(0);
     }
 }
 
@@ -711,8 +731,11 @@ RTM_HD void tile_psi(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, const 
 
 /* Phase 3: the fused update of this thread's RZ points. */
 /* XS: the instance carries the snapshot store and the imaging (false: a
- * light instance for steps that have neither, with fewer registers). */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool XS = true>
+ * light instance for steps that have neither, with fewer registers).
+ * SAFE: an interior tile inside the stepped region (tile_safe).  Nothing is
+ * stored on an inactive step (a light instance runs those through). */
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool XS = true,
+          bool SAFE = false>
 RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
 {
@@ -720,7 +743,7 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
     float umax = 0.0f;   /* max |U| over the thread's image points (16-bit store) */
     (void)HX;
     const int i = x0 + tx;
-    const bool iok = i < S.X - kHalf;
+    const bool iok = (SAFE || i < S.X - kHalf) && V.active;
     const int lc = tx + 8;
     /* this column's u from tile row ty*RZ-4 to ty*RZ+RZ+3 */
     R col[RZ + 8];
@@ -736,7 +759,7 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
     for (int rr = 0; rr < RZ; rr++) {
         const int kl = ty * RZ + rr;         /* tile-relative row (uniform) */
         const int k = z0 + kl;
-        if (k >= S.Z - kHalf) break;
+        if (!SAFE && k >= S.Z - kHalf) break;
         const int j = k * S.P + i;
         const R *lu = &L.u[kl + 8][lc];
         const R uc = col[rr + 4];

@@ -12,6 +12,10 @@
 #include "rtm_tile.h"
 #include "rtm_work.h"
 
+// interior tiles run through the clamped [0] and the unclamped (SAFE) [1]
+// phases since the last hx_interior_tiles call
+static long g_cls0_tiles[2] = {0, 0};
+
 template <typename R>
 static void run(const int *g, const R *hs, const R *u, R *U, const R *Cf, const R *ax, const R *bx,
                 const R *az, const R *bz, const R *pXi, R *pXo, R *cX, const R *pZi, R *pZo, R *cZ,
@@ -47,11 +51,17 @@ static void commit_max(const rtm::StepView<R> &V, float m)
 // phase, all 64*NW threads of a phase before the next (the barriers).
 template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4>
 static void run_tile(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, const rtm::SimDesc<R> &S,
-                     const rtm::StepView<R> &V, int x0, int z0)
+                     const rtm::StepView<R> &V, int x0, int z0);
+
+// An interior tile runs the unclamped (SAFE) phases when rtm::tile_safe
+// holds, as k_tiles dispatches it.
+template <typename R, int RZ, int CLS, bool Q16, bool I16, int NW, bool SAFE>
+static void run_tile_s(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, const rtm::SimDesc<R> &S,
+                       const rtm::StepView<R> &V, int x0, int z0)
 {
     for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, true, SAFE>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             rtm::tile_stage<R, RZ, CLS, NW>(L, G[ty * 64 + tx], tx, ty);
         }
     if (CLS)
@@ -61,7 +71,8 @@ static void run_tile(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, co
     float m = 0.0f;
     for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            const float t = rtm::tile_points<R, RZ, CLS, Q16, I16, NW>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            const float t =
+                rtm::tile_points<R, RZ, CLS, Q16, I16, NW, true, SAFE>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             m = t > m ? t : m;
         }
     commit_max(V, m);
@@ -79,6 +90,19 @@ static void run_tile(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, co
             for (int tx = 0; tx < 64; tx++)
                 rtm::tile_store_i16<R, RZ, NW>(G[ty * 64 + tx], S, V, rmax, x0, z0, tx, ty);
         }
+    }
+}
+
+template <typename R, int RZ, int CLS, bool Q16, bool I16, int NW>
+static void run_tile(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, const rtm::SimDesc<R> &S,
+                     const rtm::StepView<R> &V, int x0, int z0)
+{
+    if (CLS == 0 && rtm::tile_safe(S, x0, z0, NW * RZ)) {
+        g_cls0_tiles[1]++;
+        run_tile_s<R, RZ, 0, Q16, I16, NW, true>(L, G, S, V, x0, z0);
+    } else {
+        if (CLS == 0) g_cls0_tiles[0]++;
+        run_tile_s<R, RZ, CLS, Q16, I16, NW, false>(L, G, S, V, x0, z0);
     }
 }
 
@@ -316,6 +340,15 @@ static void tiles_entry(const int *g, const R *hs, R *const *ptr, const int *rec
 }
 
 extern "C" {
+/* interior tiles emulated through the clamped (out[0]) and the SAFE
+ * (out[1]) phases since the last call */
+void hx_interior_tiles(long *out)
+{
+    out[0] = g_cls0_tiles[0];
+    out[1] = g_cls0_tiles[1];
+    g_cls0_tiles[0] = g_cls0_tiles[1] = 0;
+}
+
 /* rtm::build_work's tile list for one grid as {sim, x0, z0, class}
  * quadruples (capacity `cap` items); returns its length or -1. */
 int hx_work_list(int X, int Z, int TL, int TR, int TU, int TD, int rz, int interior, int seg_rows, int *out, int cap)

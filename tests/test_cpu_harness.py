@@ -75,19 +75,54 @@ def _case(name):
     interior spans three tile columns (a 128-column item and a 64-column
     one) and several column segments (receivers on an interior row, source
     inside, 3 imaging steps)."""
-    if name != "wide":
+    if name not in ("wide", "thin"):
         return common.REF_CONFIGS[name], np.load(os.path.join(common.GOLDEN, f"{name}.npz"))
     cfg = dict(Nx=250, Nz=130, thicknessLeft=20, thicknessRight=22, thicknessUp=18, thicknessDown=21,
                spacingGrid=10.0, sampling=0.001, numberOfTimeStep=60, sampligWavelet=60,
                firstLocationShot=100, depthShot=30, spacingShot=1, numberOfShots=1,
                firstLocationReceiver=3, depthReceiver=35, spacingReceiver=3, numberOfReceivers=100,
                numberOfSimulations=1, zfp_tolerance=1e-4)
+    if name == "thin":
+        cfg.update(thicknessLeft=6, thicknessRight=6, thicknessUp=6, thicknessDown=6)
     rng = np.random.default_rng(21)
     model = rng.uniform(1800.0, 3200.0, (cfg["Nx"], cfg["Nz"])).astype(np.float32)
     from rtmuq2d_amd import synth
     wav = synth.ricker(25.0, 0.001, 60)
     seis = rng.standard_normal((1, cfg["numberOfReceivers"], 60)).astype(np.float32)
     return cfg, {"models": model[None], "wavelet": wav, "seismograms": seis}
+
+
+@pytest.mark.parametrize("name", ["wide", "thin"])
+@pytest.mark.parametrize("rz", [2, 4])
+def test_interior_tiles_clamped_and_unclamped_bitexact(hx, name, rz):
+    """Interior tiles whose halo lies inside the grid run the unclamped
+    (SAFE) phases, the others the clamped ones.  "wide" has only the first
+    kind; "thin" (CPML 6 wide, so interior tiles reach the grid's right and
+    bottom edges) has both.  Fields, snapshots, traces and images equal the
+    oracle's."""
+    cfg, g = _case(name)
+    op = common.orc_params(cfg)
+    vpad = O.extend(op, g["models"][0])
+    sx, sz = common.shot_positions(cfg)[0]
+    S, drec, lu, lU = O.forward(op, sx, sz, vpad, g["wavelet"], record=True)
+    ds = DL.DevSim(hx, op, vpad, np.float32)
+    counts = np.zeros(2, np.int64)
+    hx.hx_interior_tiles(counts.ctypes.data_as(DL.C.c_void_p))
+    S2, rec = ds.forward_tiled(sx, sz, g["wavelet"], rz=rz, record=True)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu) and np.array_equal(b, lU)
+    assert np.array_equal(np.ascontiguousarray(S2.transpose(0, 2, 1)), S)
+    assert np.array_equal(rec, drec.astype(np.float32))
+    d = g["seismograms"][0]
+    I, Is, lu2, lU2 = O.backward(op, vpad, d, S)
+    I2, Is2 = ds.backward_tiled(d, S2, rz=rz)
+    a, b = ds.last_fields()
+    assert np.array_equal(a, lu2) and np.array_equal(b, lU2)
+    assert np.array_equal(I2, I) and np.array_equal(Is2, Is)
+    hx.hx_interior_tiles(counts.ctypes.data_as(DL.C.c_void_p))
+    clamped, safe = int(counts[0]), int(counts[1])
+    assert safe > 0
+    assert (clamped > 0) == (name == "thin"), (clamped, safe)
 
 
 def _strips_to_full(op, P, psi, csi):
