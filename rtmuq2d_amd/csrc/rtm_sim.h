@@ -1000,11 +1000,11 @@ struct Exec {
     // instance.  phase 0 clears it (every step runs the full instance).
     bool sched = false;
     int sched_phase = 0, sched_nt = 0, sched_ts = 1, sched_flags = 0;
-    int *d_err = nullptr;               // set by a light launch that met a store or imaging step
+    int *d_err = nullptr;               // set by a step launch that met a tile its list should not hold
     void set_schedule(int phase, int nt, int ts, int flags);
     bool heavy_step(int n) const;       // step n of the pass stores or images (always without a schedule)
     long sched_key(int n0) const;       // the schedule of a block starting at step n0 (0: every step full)
-    int check_err(hipStream_t s);       // -1 with the error set if a light launch met a heavy step
+    int check_err(hipStream_t s);       // -1 with the error set if d_err was set (and clears it)
     // every captured graph, once the stream they ran on has drained
     int drop_graph();
     // the boundary-saving kernel of the coming pass (0: none; its grid

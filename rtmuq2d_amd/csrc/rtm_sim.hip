@@ -903,8 +903,7 @@ int Exec::check_err(hipStream_t s)
     if (!h) return 0;
     fill_zero(d_err, 1, s);
     (void)hipStreamSynchronize(s);
-    rtm_set_error("internal: %s", (h & 2) ? "a step launch met a corner tile that the work list should have cut"
-                                            : "a light step launch met a step that stores or images (step schedule mismatch)");
+    rtm_set_error("internal: a step launch met a corner tile that the work list should have cut (flags %d)", h);
     return -1;
 }
 
