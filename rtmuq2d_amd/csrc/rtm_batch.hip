@@ -348,7 +348,7 @@ static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
     const int nt = b0.p.number_of_time_steps;
     // every realization of the group at the same step with the same store:
     // the steps that neither store nor image run the light tile instance
-    ex.set_schedule(b0.qmode == rtm::Q_NONE && !b0.ringmode ? phase : 0, nt, b0.p.time_slices, d[0].flags);
+    ex.set_schedule(b0.light_steps() ? phase : 0, nt, b0.p.time_slices, d[0].flags);
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (ex.run(nt, q->graph_block, q->stream)) return -1;
     if (phase == rtm::PH_FORWARD)

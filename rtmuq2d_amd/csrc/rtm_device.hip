@@ -318,7 +318,7 @@ static int forward_t(rtm_plan *q, int shot_x, int shot_z, int record)
         }
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         // steps that store no snapshot run the light tile instance
-        q->ex.set_schedule(b.qmode == rtm::Q_NONE && !b.ringmode ? rtm::PH_FORWARD : 0, nt, p.time_slices, d.flags);
+        q->ex.set_schedule(b.light_steps() ? rtm::PH_FORWARD : 0, nt, p.time_slices, d.flags);
         if (record && !(d.flags & rtm::F_RECORD)) {
             // receivers sharing a point or outside: record each one per step
             // (with pair items an even step's launch also steps them at n+1,
@@ -372,7 +372,7 @@ static int backward_t(rtm_plan *q)
         if (b.ringmode) d.src_j = q->fwd_src_j;   // only the reconstruction reads it in this phase
         if (q->ex.upload_descs(&d, 1, q->stream) || q->ex.set_n0({0}, q->stream)) return -1;
         // steps that image nothing run the light tile instance
-        q->ex.set_schedule(b.qmode == rtm::Q_NONE && !b.ringmode ? rtm::PH_BACKWARD : 0, nt, p.time_slices, d.flags);
+        q->ex.set_schedule(b.light_steps() ? rtm::PH_BACKWARD : 0, nt, p.time_slices, d.flags);
         if (q->ex.run(nt, q->graph_block, q->stream)) return -1;
         if (q->ex.check_err(q->stream)) return -1;
     } else if (b.qmode != rtm::Q_NONE || b.ringmode) {

@@ -72,6 +72,7 @@ struct HipBackend {
     int rz = 4, interior = rtm::IN_TILES, seg_rows = 0;
     DeviceArena arena;
     std::vector<std::unique_ptr<Exec>> ex;      // one executor per launch group
+    std::vector<std::vector<int>> members_of;   // group g's request slots (as of its last prepare)
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -129,6 +130,9 @@ struct HipBackend {
             e.i16 = snapshot_mode == RTM_SNAP_I16;
         }
         if (e.set_tiles(tiles, stream) || e.set_inner(inner, stream)) return fail("work lists");
+        if ((int)members_of.size() <= g) members_of.resize(g + 1);
+        members_of[g].clear();
+        for (const SReq *r : members) members_of[g].push_back(r->slot);
         return 0;
     }
 
@@ -139,11 +143,13 @@ struct HipBackend {
             ex.back()->release();
             ex.pop_back();
         }
+        if ((int)members_of.size() > ngroups) members_of.resize(ngroups);
     }
 
     void reset(int g)
     {
         if (g < (int)ex.size()) ex[g]->release();
+        if (g < (int)members_of.size()) members_of[g].clear();
     }
 
     int upload(int g, const std::vector<const SReq *> &slots)
@@ -157,6 +163,17 @@ struct HipBackend {
         }
         if (ex[g]->upload_descs(d.data(), max_slots, stream) || ex[g]->set_n0(n0, stream))
             return fail("descriptors");
+        // the members' step schedules: a step at which none of them stores
+        // or images runs the light tile instance (the FP32 and int16 block
+        // stores write a snapshot only in the step that computes it)
+        std::vector<int4> gs;
+        if ((snapshot_mode == RTM_SNAP_FP32 || snapshot_mode == RTM_SNAP_I16) && g < (int)members_of.size()) {
+            gs.assign(max_slots, make_int4(0, 0, 1, 0));
+            for (int i : members_of[g])
+                if (i >= 0 && i < max_slots && slots[i])
+                    gs[i] = make_int4(d[i].phase, d[i].nt, d[i].TS, d[i].flags);
+        }
+        ex[g]->set_group_schedule(gs);
         return 0;
     }
 

@@ -881,6 +881,12 @@ struct SimBuffers {
     // step with four u buffers (pair shapes) or two; the extra buffers stay
     // allocated once made, and are zeroed by reset_state while in use
     int set_quad(bool on);
+    // a pass may run its steps that neither store nor image with the light
+    // tile instance: the FP32 and the int16 block stores write a snapshot
+    // only in the step that computes it (and boundary saving none in the
+    // tile launch); the FP16 / BF16 stores also fill the previous
+    // snapshot's slot in the step after it
+    bool light_steps() const { return qmode == rtm::Q_NONE || qmode == rtm::Q_I16; }
     int upload_profiles();
     int upload_receivers();
     // uploads; with wait == false the host data must stay alive until the
@@ -968,9 +974,13 @@ struct Exec {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int block = 0, ring_blocks = 0;
-        long key = 0;                   // the step schedule it was captured with (sched_key)
+        std::vector<char> heavy;        // per captured step: the full instance (empty: every step)
     };
-    Captured cap[3];
+    // by phase (plans, batches, boundary saving: 0 none, 1 forward, 2
+    // backward); 3..6 for the request server's schedules (a group's heavy
+    // steps change with its members' passes, and a few patterns recur)
+    Captured cap[7];
+    int gcap_next = 0;
     hipStream_t g_stream = nullptr;     // where the graphs were last launched
     bool g_tiles = false, g_inner = false;
     // the strip tiles run on `side` beside the interior kernel (they write
@@ -994,16 +1004,24 @@ struct Exec {
     // or images at this step, so the tiles run the light instance.
     int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30, bool heavy = true);
     // The step schedule of the coming pass (every simulation of the launch at
-    // the same step, FP32 store, no boundary saving): the forward steps
-    // n % ts == 0 store a snapshot (flags F_SNAP), the backward steps with
-    // (nt-1-n) % ts == 0 image (F_IMAGE); every other step runs the light tile
-    // instance.  phase 0 clears it (every step runs the full instance).
+    // the same step, a store that writes snapshots only in the step that
+    // computes them): the forward steps n % ts == 0 store a snapshot (flags
+    // F_SNAP), the backward steps with (nt-1-n) % ts == 0 image (F_IMAGE);
+    // every other step runs the light tile instance.  phase 0 clears it
+    // (every step runs the full instance).
     bool sched = false;
     int sched_phase = 0, sched_nt = 0, sched_ts = 1, sched_flags = 0;
+    // The request server's form: one schedule {phase, nt, ts, flags} per
+    // simulation slot, each at its own step (h_n0); phase 0 for the slots
+    // outside the launch.  A step runs the light instance when no member
+    // stores or images at it.
+    std::vector<int4> gsched;
     int *d_err = nullptr;               // set by a step launch that met a tile its list should not hold
     void set_schedule(int phase, int nt, int ts, int flags);
+    void set_group_schedule(const std::vector<int4> &per_sim);
     bool heavy_step(int n) const;       // step n of the pass stores or images (always without a schedule)
-    long sched_key(int n0) const;       // the schedule of a block starting at step n0 (0: every step full)
+    bool heavy_at(int t) const;         // step t from the current counters stores or images for some sim
+    bool pattern(int block) const;      // the heavy steps repeat from block to block
     int check_err(hipStream_t s);       // -1 with the error set if d_err was set (and clears it)
     // every captured graph, once the stream they ran on has drained
     int drop_graph();

@@ -658,7 +658,8 @@ int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
 }
 
 // The light tile instance (no snapshot store, no imaging) for FP32 fields
-// with the FP32 store in tile shapes; false when there is none.
+// in tile shapes (the same for every store: a light step stores nothing);
+// false when there is none.
 template <typename R, bool Q16, int UM = 1>
 static bool launch_light(const Exec &e, int s_in_block, hipStream_t ts)
 {
@@ -856,7 +857,11 @@ int Exec::launch_one(int s_in_block, hipStream_t s, int lim, bool heavy)
 {
     if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s, heavy);
     else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s, true);
-    else if (i16) launch_tiles_i16(*this, s_in_block, s);
+    else if (i16) {
+        // the light instance is the FP32 one: a light step stores nothing
+        if (heavy || interior != rtm::IN_TILES || !launch_light<float, false>(*this, s_in_block, s))
+            launch_tiles_i16(*this, s_in_block, s);
+    }
     else if (q16) launch_tiles<float, true>(*this, s_in_block, s, true);
     else launch_tiles<float, false>(*this, s_in_block, s, heavy);
     // boundary saving: the ring save / reconstruction follows the step on the
@@ -876,6 +881,43 @@ void Exec::set_schedule(int phase, int nt, int ts, int flags)
     sched_nt = nt;
     sched_ts = ts > 0 ? ts : 1;
     sched_flags = flags;
+    gsched.clear();
+}
+
+void Exec::set_group_schedule(const std::vector<int4> &per_sim)
+{
+    sched = false;
+    gsched = per_sim;
+    for (auto &g : gsched) g.z = g.z > 0 ? g.z : 1;
+}
+
+// a step of a simulation's pass stores or images; the pattern is periodic
+// in n (also past the pass's end, where the kernels skip the simulation), so
+// a group's heavy steps repeat from block to block
+static bool heavy_of(const int4 &g, int n)
+{
+    const auto mod = [](int a, int m) { return ((a % m) + m) % m; };
+    if (g.x == rtm::PH_FORWARD) return (g.w & rtm::F_SNAP) && mod(n, g.z) == 0;
+    if (g.x == rtm::PH_BACKWARD) return (g.w & rtm::F_IMAGE) && mod(g.y - 1 - n, g.z) == 0;
+    return false;
+}
+
+bool Exec::heavy_at(int t) const
+{
+    if (sched) return heavy_step(h_n0[0] + t);
+    if (gsched.empty()) return true;
+    for (int i = 0; i < (int)gsched.size() && i < cap_sims; i++)
+        if (heavy_of(gsched[i], h_n0[i] + t)) return true;
+    return false;
+}
+
+bool Exec::pattern(int block) const
+{
+    if (sched) return block % sched_ts == 0;
+    if (gsched.empty()) return false;
+    for (const auto &g : gsched)
+        if (g.x && block % g.z) return false;
+    return true;
 }
 
 bool Exec::heavy_step(int n) const
@@ -886,14 +928,6 @@ bool Exec::heavy_step(int n) const
     return (sched_flags & rtm::F_IMAGE) && (sched_nt - 1 - n) % sched_ts == 0;
 }
 
-long Exec::sched_key(int n0) const
-{
-    // a captured block is replayed at n0 + k*block: its light and full steps
-    // repeat only when the block is a multiple of the snapshot interval
-    if (!sched) return 0;
-    return 1 + sched_phase + 4L * (sched_flags & (rtm::F_SNAP | rtm::F_IMAGE)) +
-           64L * ((sched_nt - 1) % sched_ts) + 64L * 65536 * (n0 % sched_ts) + 64L * 65536 * 65536 * sched_ts;
-}
 
 int Exec::check_err(hipStream_t s)
 {
@@ -922,28 +956,38 @@ int Exec::run(int steps, int block, hipStream_t s)
         return -1;
     }
     int done = 0;
-    const int n0 = h_n0[0];   // with a schedule every simulation is at this step
     if (block > 1 && steps >= block) {
         // with a step schedule whose pattern repeats from block to block, the
         // captured steps run the light or the full tile instance as the
         // schedule says; otherwise every captured step runs the full one
-        const bool pattern = sched && block % sched_ts == 0;
-        const long key = pattern ? sched_key(n0) : 0;
-        Captured &c = cap[sched ? sched_phase : ring_phase];
+        std::vector<char> heavy;
+        if (pattern(block)) {
+            heavy.resize(block);
+            for (int t = 0; t < block; t++) heavy[t] = heavy_at(t);
+        }
+        int slot = sched ? sched_phase : ring_phase;
+        if (!sched && !gsched.empty()) {
+            slot = -1;
+            for (int k = 3; k < 7 && slot < 0; k++)
+                if (cap[k].exec && cap[k].block == block && cap[k].ring_blocks == ring_blocks && cap[k].heavy == heavy)
+                    slot = k;
+            if (slot < 0) slot = 3 + (gcap_next++ & 3);
+        }
+        Captured &c = cap[slot];
         const bool lists_match = g_tiles == (ntiles > 0) && g_inner == (ninner > 0);
         if (!lists_match) drop_graph();
-        if (!c.exec || c.block != block || c.ring_blocks != ring_blocks || c.key != key) {
+        if (!c.exec || c.block != block || c.ring_blocks != ring_blocks || c.heavy != heavy) {
             if (c.exec && g_stream) (void)hipStreamSynchronize(g_stream);
             destroy_captured(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int t = 0; t < block; t++) launch_one(t, s, block, pattern ? heavy_step(n0 + t) : true);
+            for (int t = 0; t < block; t++) launch_one(t, s, block, heavy.empty() ? true : heavy[t] != 0);
             hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
             hipError_t e = hipStreamEndCapture(s, &c.graph);
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }
             HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
             c.block = block;
             c.ring_blocks = ring_blocks;
-            c.key = key;
+            c.heavy = heavy;
             g_tiles = ntiles > 0;
             g_inner = ninner > 0;
         }
@@ -954,7 +998,7 @@ int Exec::run(int steps, int block, hipStream_t s)
         }
     }
     const int rem = steps - done;
-    for (int t = 0; t < rem; t++) launch_one(t, s, rem, heavy_step(n0 + done + t));
+    for (int t = 0; t < rem; t++) launch_one(t, s, rem, heavy_at(done + t));
     if (rem) hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, rem);
     HIP_TRY(hipGetLastError());
     for (int i = 0; i < cap_sims; i++) h_n0[i] += steps;
@@ -1020,7 +1064,8 @@ int time_steps(Exec &ex, std::vector<SimDesc<R>> descs, int steps, int block, hi
     if (ex.upload_descs(descs.data(), (int)descs.size(), s)) return -1;
     if (ex.set_n0(std::vector<int>(descs.size(), 0), s)) return -1;
     // the forward schedule of these steps, as a pass runs them
-    ex.set_schedule(descs.empty() || descs[0].qmode != rtm::Q_NONE ? 0 : rtm::PH_FORWARD,
+    ex.set_schedule(descs.empty() || (descs[0].qmode != rtm::Q_NONE && descs[0].qmode != rtm::Q_I16) ? 0
+                                                                                                   : rtm::PH_FORWARD,
                     descs.empty() ? 0 : descs[0].nt, descs.empty() ? 1 : descs[0].TS, flags);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));

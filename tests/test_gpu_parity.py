@@ -1103,12 +1103,16 @@ def _sub(req):
 
 
 @pytest.mark.parametrize("group_bytes", [None, 1])
-def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group_bytes):
+@pytest.mark.parametrize("block", [50, 100])
+def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group_bytes, block):
     """The request server with the int16 block store: requests of three grid
     sizes arriving while others are in flight, each image equal to the
     oracle's for that request alone under the store's rule, in one launch
-    group and with every request in a group of its own."""
-    say(f"[gpu] request server, int16 block store, group_bytes {group_bytes}")
+    group and with every request in a group of its own.  With blocks of 100
+    steps (a multiple of the snapshot interval 4) the steps at which no
+    member of a launch group stores or images run the light tile instance;
+    with 50 every step runs the full one."""
+    say(f"[gpu] request server, int16 block store, group_bytes {group_bytes} block {block}")
     reqs = _server_requests()
     refs = []
     for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
@@ -1118,7 +1122,7 @@ def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group
     budget = int(sorted(fp)[-1] + sorted(fp)[-2] + (1 << 20))
     got = {}
     with R.Server(budget, max_slots=4, snapshot_mode=R.SNAP_I16) as srv:
-        srv.set_tiling(4, 50, 0, 0)
+        srv.set_tiling(4, block, 0, 0)
         with pytest.raises(R.RtmError, match="tile shapes only"):
             srv.set_tiling(4, 50, 1, 16)
         if group_bytes is not None:
@@ -1136,11 +1140,14 @@ def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group
         assert np.array_equal(got[rid], refs[k]), k
 
 
-def test_server_fp32_store_matches_the_oracle(gpu_available):
+@pytest.mark.parametrize("group_bytes,block", [(1, 100), (None, 100), (None, 30)])
+def test_server_fp32_store_matches_the_oracle(gpu_available, group_bytes, block):
     """A server with the FP32 snapshot store (the reference's scheme): every
     image equals the oracle's for that request alone, bit for bit, with all
-    requests admitted at once and each in a launch group of its own."""
-    say("[gpu] request server, FP32 store")
+    requests admitted at once, each in a launch group of its own or all in
+    one (forward and backward members in the same launches).  Blocks of 100
+    steps run the light tile instance where no member stores or images."""
+    say(f"[gpu] request server, FP32 store, group_bytes {group_bytes} block {block}")
     reqs = _server_requests()[:3]
     refs = []
     for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
@@ -1149,7 +1156,9 @@ def test_server_fp32_store_matches_the_oracle(gpu_available):
     got = {}
     with R.Server(sum(R.request_footprint(r[2].copy().update(snapshot_mode=0)) for r in reqs) + (8 << 20),
                   max_slots=4, snapshot_mode=0) as srv:
-        srv.set_group_bytes(1)
+        srv.set_tiling(4, block, 0, 0)
+        if group_bytes is not None:
+            srv.set_group_bytes(group_bytes)
         ids = [srv.submit(*_sub(r)) for r in reqs]
         while srv.tick() > 0:
             for rid, I in srv.done():
