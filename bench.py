@@ -379,6 +379,8 @@ def main() -> None:
                     help="profile workload: only the c2 launch group in 64x16 tiles (counter passes)")
     ap.add_argument("--profile-batch-shapes", default=None,
                     help="profile workload: the launch group's kernel shapes, 'rz,block,interior,seg;...'")
+    ap.add_argument("--profile-lanes", type=int, default=1,
+                    help="profile workload: the batch as that many launch groups in flight at once")
     ap.add_argument("--no-c0", dest="c0", action="store_false",
                     help="skip the c0 leg (two 600x200 simulations through rtm_routine, GPU vs CPU)")
     args = ap.parse_args()
@@ -396,7 +398,7 @@ def main() -> None:
         if args.profile_batch_shapes:
             shapes = [tuple(int(v) for v in sh.split(",")) for sh in args.profile_batch_shapes.split(";")]
         run_profile(R, args.profile_nt, args.profile_sims, batch_only=args.profile_batch_only,
-                    block=args.profile_block, shapes=shapes)
+                    block=args.profile_block, shapes=shapes, lanes=args.profile_lanes)
         return
     if args.only_serve:
         print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
@@ -564,7 +566,7 @@ def main() -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None):
+def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None, lanes=1):
     """Short, fixed workload for counter profiles (rocprofv3 --pmc runs one
     pass per counter group over the same dispatches):
       1. a 1 GiB device-to-device copy (torch), the calibration of the
@@ -577,7 +579,8 @@ def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None):
     step-kernel dispatch of the run is the launch group's.  block: steps per
     captured graph of step 3 (1: every step launched on its own).  shapes:
     the batch's kernel shapes for step 3 instead (rows_per_thread,
-    graph_block, interior, seg_rows each), one run per shape."""
+    graph_block, interior, seg_rows each), one run per shape.  lanes: step
+    3's realizations as that many launch groups in flight at once."""
     import torch
     x = torch.rand(1 << 28, device="cuda")
     y = torch.empty_like(x)
@@ -599,6 +602,8 @@ def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None):
             log("[profile] plain", plan.last_timing(), "image norm", float(np.linalg.norm(plan.image()[0])))
     pb = p.copy().update(time_slices=20)
     with R.Batch(pb, nsims) as b:
+        b.set_group((nsims + lanes - 1) // lanes if lanes > 1 else 0)
+        b.set_lanes(lanes)
         vpad = R.extend(pb, v_mig)
         for k in range(nsims):
             b.set_velocity(k, vpad)
@@ -610,7 +615,7 @@ def run_profile(R, nt=400, nsims=13, batch_only=False, block=100, shapes=None):
             b.set_tiling(*shape)
             b.zero_images()
             b.migrate(pb.first_location_shot, pb.depth_shot)
-            log(f"[profile] batch{nsims} {tiling_dict(shape)}", b.last_timing(),
+            log(f"[profile] batch{nsims} lanes {lanes} {tiling_dict(shape)}", b.last_timing(),
                 "image norms", [round(float(np.linalg.norm(b.image(k)[0])), 6) for k in (0, nsims - 1)])
     print(json.dumps({"profile": "done", "nt": nt}), flush=True)
 
@@ -943,10 +948,17 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
     for t in sweep:
         log(f"[bench] batch autotune {tiling_dict((t['rows_per_thread'], t['graph_block'], t['interior'], t['seg_rows']))}: "
             f"{t['ms_per_step'] * 1e3:.1f} us/step")
+    # launch groups in flight at once: one lane at the tuned size, then two
+    # lanes of smaller groups, each timed as a wave of groups at that shape
+    lsweep = b.autotune_lanes(200)
+    for t in lsweep:
+        log(f"[bench] batch lanes {t['lanes']} x groups of {t['group']}: {t['us_per_sim_step']:.3f} us per "
+            "realization-step")
     bbest = (None, b.tiling())
     gsize, ngroups = b.group()
+    lanes = b.lanes()
     b_group = (gsize, ngroups)
-    log(f"[bench] batch using {tiling_dict(bbest[1])}, {ngroups} launch groups of {gsize}")
+    log(f"[bench] batch using {tiling_dict(bbest[1])}, {ngroups} launch groups of {gsize}, {lanes} in flight")
     for i in range(warmup):
         b.zero_images()
         b.migrate(sx, sz)
@@ -973,6 +985,7 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
     step_ms = (f_ms + b_ms) / 2.0          # one time step of every launch group
     achieved = BYTES_PER_UPDATE * X * Z * nsims / (step_ms * 1e-3) / 1e9
     gsize_, ngroups_ = b_group
+    lanes_ = lanes
     plain_iph = 3600.0 / plain_secs
     batch_iph = nsims * 3600.0 / batch_secs
     out = {
@@ -988,9 +1001,11 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
         "batched_over_plain": round(batch_iph / plain_iph, 3),
         "step_kernel_us": {"forward": round(f_ms * 1e3, 2), "backward": round(b_ms * 1e3, 2)},
         "tiling": tiling_dict(bbest[1]),
-        "launch_groups": {"sims_per_group": gsize, "groups": ngroups,
+        "launch_groups": {"sims_per_group": gsize, "groups": ngroups, "in_flight": lanes,
                           "autotune_us_per_sim_step": {str(t["group"]): round(t["us_per_sim_step"], 3)
-                                                       for t in gsweep}},
+                                                       for t in gsweep},
+                          "autotune_lanes_us_per_sim_step": {f"{t['lanes']}x{t['group']}": round(t["us_per_sim_step"], 3)
+                                                             for t in lsweep}},
         "autotune": [dict(tiling_dict((t["rows_per_thread"], t["graph_block"], t["interior"], t["seg_rows"])),
                           us_per_step=round(t["ms_per_step"] * 1e3, 2)) for t in sweep],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -1001,8 +1016,9 @@ def run_batch(R, p1, v_true, w, seis, nsims, ts, steps=3, warmup=1):
                      "algorithmic_bytes_per_launch": round(BYTES_PER_UPDATE * X * Z * nsims / ngroups_),
                      "launch_us": round(step_ms * 1e3 / ngroups_, 3),
                      "launch_note": f"{ngroups_} launches per time step, one per launch group of at most {gsize_} "
-                                    f"realizations ({nsims} in all); bytes and time are the means over a step's "
-                                    "launches"},
+                                    f"realizations ({nsims} in all), {lanes_} in flight at once on streams of their "
+                                    "own; bytes are the mean per launch and time the step's span over the number "
+                                    "of launches"},
         "welford_seconds": round(welford_s, 4),
         "image0_relL2_vs_fp64": rel_l2(I0, I64),
         "meets_gate": gate(rel_l2(I0, I64)),

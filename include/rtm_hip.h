@@ -279,9 +279,25 @@ int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
  * and the whole batch) on group 0, timing forward steps that write
  * the snapshot stores (overwritten by the next migration), and keeps the
  * smallest group within 2% of the least time per realization-step; row k =
- * group_sizes[k], us_per_sim_step[k]. */
+ * group_sizes[k], us_per_sim_step[k].  It runs one group at a time (lanes 1). */
 int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                              int *n_out);
+/* Launch groups in flight at once (no reference counterpart): with lanes > 1
+ * (at most 4) group k runs on lane k % lanes, each lane a stream of its own,
+ * so one group's steps fill the gaps between another's step launches; the
+ * group count becomes a multiple of the lanes.  A migration then reports the
+ * span of all its passes, half as forward and half as backward time
+ * (rtm_batch_last_timing).  Results do not depend on the lanes. */
+int rtm_batch_set_lanes(rtm_batch *batch, int lanes);
+int rtm_batch_get_lanes(rtm_batch *batch, int *lanes);
+/* Times one lane at the current group size, then two lanes at group sizes
+ * of 1/4 to 1/2 of the size that fills the 224 MiB budget (balanced over an
+ * even number of groups), each as a wave of its first groups in flight at
+ * the current shape, stepping forward steps that write the snapshot stores;
+ * keeps the first within 2% of the least time per realization-step.  Row k
+ * = lanes[k], group_sizes[k] (the largest group), us_per_sim_step[k]. */
+int rtm_batch_autotune_lanes(rtm_batch *batch, int steps, int cap, int *lanes, int *group_sizes,
+                             double *us_per_sim_step, int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */
 int rtm_batch_set_wavelet(rtm_batch *batch, const float *w, int n);
 int rtm_batch_set_seismogram(rtm_batch *batch, const float *d);             /* [nrec][nt] */

@@ -133,6 +133,9 @@ def lib() -> C.CDLL:
         "rtm_batch_set_group": (C.c_int, [plan_p, C.c_int]),
         "rtm_batch_get_group": (C.c_int, [plan_p, i32p, i32p]),
         "rtm_batch_autotune_group": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, f64p, i32p]),
+        "rtm_batch_set_lanes": (C.c_int, [plan_p, C.c_int]),
+        "rtm_batch_get_lanes": (C.c_int, [plan_p, i32p]),
+        "rtm_batch_autotune_lanes": (C.c_int, [plan_p, C.c_int, C.c_int, i32p, i32p, f64p, i32p]),
         "rtm_plan_set_velocity": (C.c_int, [plan_p, f32p]),
         "rtm_plan_set_wavelet": (C.c_int, [plan_p, f32p, C.c_int]),
         "rtm_plan_set_seismogram": (C.c_int, [plan_p, f32p]),
@@ -502,6 +505,28 @@ class Batch:
         _check(lib().rtm_batch_autotune_group(self.h, int(steps), cap, iptr(gs), dptr(us), C.byref(n)),
                "autotune_group")
         return [dict(group=int(g), us_per_sim_step=float(t)) for g, t in zip(gs[:n.value], us[:n.value])]
+
+    def set_lanes(self, lanes: int) -> None:
+        """rtm_batch_set_lanes: launch groups in flight at once (1 to 4)."""
+        _check(lib().rtm_batch_set_lanes(self.h, int(lanes)), "set_lanes")
+
+    def lanes(self) -> int:
+        n = C.c_int()
+        _check(lib().rtm_batch_get_lanes(self.h, C.byref(n)), "get_lanes")
+        return n.value
+
+    def autotune_lanes(self, steps: int = 200):
+        """rtm_batch_autotune_lanes: one lane at the current group size, then
+        two lanes at smaller groups; keeps the fastest (2% rule)."""
+        cap = 16
+        ls = np.zeros(cap, np.int32)
+        gs = np.zeros(cap, np.int32)
+        us = np.zeros(cap, np.float64)
+        n = C.c_int()
+        _check(lib().rtm_batch_autotune_lanes(self.h, int(steps), cap, iptr(ls), iptr(gs), dptr(us), C.byref(n)),
+               "autotune_lanes")
+        return [dict(lanes=int(a), group=int(g), us_per_sim_step=float(t))
+                for a, g, t in zip(ls[:n.value], gs[:n.value], us[:n.value])]
 
     def set_velocity(self, sim: int, vpad: np.ndarray) -> None:
         v = np.ascontiguousarray(vpad, dtype=np.float32)

@@ -112,12 +112,21 @@ unsigned char *na_scratch(int device)
 
 }  // namespace
 
+// launch groups in flight at once, at most
+constexpr int kMaxLanes = 4;
+
 struct rtm_batch {
     std::vector<SimBuffers> sims;
     // one executor per launch group; group k holds sims
     // [group_first(k), group_first(k) + group_len(k))
     std::vector<Exec> ex;
     int group = 0;                      // most realizations per launch group (0: all)
+    // launch groups in flight at once: group k runs on lane k % lanes, each
+    // lane a stream of its own (lane 0 is `stream`), so one group's steps
+    // fill the gaps between another's step launches
+    int lanes = 1;
+    hipStream_t lane_s[kMaxLanes] = {};
+    hipEvent_t lane_ev[kMaxLanes] = {};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int real = RTM_REAL_F32;
@@ -127,12 +136,19 @@ struct rtm_batch {
 };
 
 static void batch_drop_execs(rtm_batch *q);
+static int batch_set_lanes(rtm_batch *q, int lanes);
 
 static void batch_free(rtm_batch *q)
 {
     batch_drop_execs(q);
     for (auto &s : q->sims) s.release();
     q->sims.clear();
+    for (int j = 1; j < kMaxLanes; j++) {
+        if (q->lane_s[j]) (void)hipStreamDestroy(q->lane_s[j]);
+        if (q->lane_ev[j]) (void)hipEventDestroy(q->lane_ev[j]);
+        q->lane_s[j] = nullptr;
+        q->lane_ev[j] = nullptr;
+    }
     if (q->ev0) (void)hipEventDestroy(q->ev0);
     if (q->ev1) (void)hipEventDestroy(q->ev1);
     if (q->stream) (void)hipStreamDestroy(q->stream);
@@ -140,7 +156,9 @@ static void batch_free(rtm_batch *q)
 
 // Launch groups: at most q->group realizations each (0: one group), as few
 // groups as that allows, their sizes differing by at most one (the larger
-// ones first), so that no group is a small remainder.
+// ones first), so that no group is a small remainder.  With lanes > 1 the
+// count is a multiple of the lanes (while there are realizations for it),
+// so every wave of groups in flight is whole.
 static int group_limit(const rtm_batch *q)
 {
     const int N = (int)q->sims.size();
@@ -149,9 +167,12 @@ static int group_limit(const rtm_batch *q)
 
 static int group_count(const rtm_batch *q)
 {
-    const int N = (int)q->sims.size(), g = group_limit(q);
-    return (N + g - 1) / g;
+    const int N = (int)q->sims.size(), g = group_limit(q), L = q->lanes;
+    const int ng = (N + g - 1) / g;
+    return L > 1 ? std::min(N, (ng + L - 1) / L * L) : ng;
 }
+
+static hipStream_t lane_stream(const rtm_batch *q, int j) { return j == 0 ? q->stream : q->lane_s[j]; }
 
 static int group_first(const rtm_batch *q, int k)
 {
@@ -254,7 +275,11 @@ extern "C" int rtm_batch_create(const rtm_params *p, int nsims, int real, rtm_ba
     sh.group = (int)std::max<size_t>(1, kGroupBytes / per);
     if (!(env && *env)) tuner_lookup(batch_key(q), &sh);
     q->group = sh.group >= nsims ? 0 : sh.group;
-    if (batch_set_tiling(q, sh.rz, 100, sh.interior, sh.seg)) return fail();
+    q->rz = sh.rz;
+    q->interior = sh.interior;
+    q->seg_rows = sh.seg;
+    if (sh.lanes > 1 ? batch_set_lanes(q, sh.lanes) : batch_set_tiling(q, sh.rz, 100, sh.interior, sh.seg))
+        return fail();
     *out = q;
     return 0;
 }
@@ -314,18 +339,22 @@ extern "C" int rtm_batch_zero_images(rtm_batch *q)
     return 0;
 }
 
-// One pass of launch group k: every step advances that group's realizations.
-// The pass's event-timed duration is added to fwd_ms or bwd_ms.
+// One pass of launch group k on lane `lane`: every step advances that
+// group's realizations.  With `timed`, the pass waits for its end and its
+// event-timed duration is added to fwd_ms or bwd_ms; otherwise it returns
+// with its steps queued on the lane's stream (the caller joins the lanes and
+// checks the executors' error flags).
 template <typename R>
-static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
+static int batch_pass(rtm_batch *q, int k, int phase, int src_j, int lane = 0, bool timed = true)
 {
     Exec &ex = q->ex[k];
+    const hipStream_t st = lane_stream(q, lane);
     const int s0 = group_first(q, k), N = ex.cap_sims;
     std::vector<SimDesc<R>> d(N);
     for (int s = 0; s < N; s++) {
         SimBuffers &b = q->sims[s0 + s];
-        b.reset_state(q->stream, phase == rtm::PH_FORWARD && b.ringmode);
-        if (phase == rtm::PH_FORWARD) b.reset_snap_scales(q->stream);
+        b.reset_state(st, phase == rtm::PH_FORWARD && b.ringmode);
+        if (phase == rtm::PH_FORWARD) b.reset_snap_scales(st);
         d[s] = b.desc<R>();
         d[s].phase = phase;
         if (phase == rtm::PH_FORWARD) {
@@ -343,12 +372,20 @@ static int batch_pass(rtm_batch *q, int k, int phase, int src_j)
     const SimBuffers &b0 = q->sims[s0];
     ex.set_ring_phase(b0.ringmode ? phase : 0,
                       (int)((phase == rtm::PH_FORWARD ? b0.NR : b0.nint + b0.NR) + 255) / 256);
-    if (ex.upload_descs(d.data(), N, q->stream)) return -1;
-    if (ex.set_n0(std::vector<int>(N, 0), q->stream)) return -1;
+    if (ex.upload_descs(d.data(), N, st)) return -1;
+    if (ex.set_n0(std::vector<int>(N, 0), st)) return -1;
     const int nt = b0.p.number_of_time_steps;
     // every realization of the group at the same step with the same store:
     // the steps that neither store nor image run the light tile instance
     ex.set_schedule(b0.light_steps() ? phase : 0, nt, b0.p.time_slices, d[0].flags);
+    q->last_steps = nt;
+    if (!timed) {
+        if (ex.run(nt, q->graph_block, st)) return -1;
+        if (phase == rtm::PH_FORWARD)
+            for (int s = 0; s < N; s++)
+                if (q->sims[s0 + s].flush_last_snapshot(st)) return -1;
+        return 0;
+    }
     HIP_TRY(hipEventRecord(q->ev0, q->stream));
     if (ex.run(nt, q->graph_block, q->stream)) return -1;
     if (phase == rtm::PH_FORWARD)
@@ -391,12 +428,42 @@ extern "C" int rtm_batch_migrate(rtm_batch *q, int shot_x, int shot_z)
     const int src_j = zs * b0.P + xs;
     q->fwd_ms = q->bwd_ms = 0.0;
     int rc = 0;
-    for (int k = 0; k < (int)q->ex.size() && rc == 0; k++) {
-        rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, k, rtm::PH_FORWARD, src_j)
-                                     : batch_pass<float>(q, k, rtm::PH_FORWARD, src_j);
-        if (rc == 0)
-            rc = q->real == RTM_REAL_F64 ? batch_pass<double>(q, k, rtm::PH_BACKWARD, src_j)
-                                         : batch_pass<float>(q, k, rtm::PH_BACKWARD, src_j);
+    const int ng = (int)q->ex.size(), L = std::min(q->lanes, ng);
+    auto pass = [&](int k, int phase, int lane, bool timed) {
+        return q->real == RTM_REAL_F64 ? batch_pass<double>(q, k, phase, src_j, lane, timed)
+                                       : batch_pass<float>(q, k, phase, src_j, lane, timed);
+    };
+    if (L <= 1) {
+        for (int k = 0; k < ng && rc == 0; k++) {
+            rc = pass(k, rtm::PH_FORWARD, 0, true);
+            if (rc == 0) rc = pass(k, rtm::PH_BACKWARD, 0, true);
+        }
+    } else {
+        // waves of L groups, group k0 + j on lane j; each pass's uploads wait
+        // for its own lane only, so the other lanes keep stepping meanwhile
+        HIP_TRY(hipEventRecord(q->ev0, q->stream));
+        for (int j = 1; j < L; j++) HIP_TRY(hipStreamWaitEvent(q->lane_s[j], q->ev0, 0));
+        for (int k0 = 0; k0 < ng && rc == 0; k0 += L) {
+            for (int j = 0; j < L && k0 + j < ng && rc == 0; j++) rc = pass(k0 + j, rtm::PH_FORWARD, j, false);
+            for (int j = 0; j < L && k0 + j < ng && rc == 0; j++) rc = pass(k0 + j, rtm::PH_BACKWARD, j, false);
+        }
+        // join every lane (also after a failure: nothing may still run on
+        // them when the caller frees or reuses the batch)
+        for (int j = 1; j < L; j++) {
+            (void)hipEventRecord(q->lane_ev[j], q->lane_s[j]);
+            (void)hipStreamWaitEvent(q->stream, q->lane_ev[j], 0);
+        }
+        const hipError_t e1 = hipEventRecord(q->ev1, q->stream);
+        const hipError_t e2 = e1 == hipSuccess ? hipEventSynchronize(q->ev1) : e1;
+        if (rc == 0 && e2 != hipSuccess) {
+            rtm_set_error("rtm_batch_migrate: %s", hipGetErrorString(e2));
+            rc = -1;
+        }
+        for (int k = 0; k < ng && rc == 0; k++) rc = q->ex[k].check_err(q->stream);
+        float ms = 0.f;
+        if (rc == 0) HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
+        // the passes of different groups overlap: each half of the span
+        q->fwd_ms = q->bwd_ms = ms / 2.0;
     }
     for (auto &s : q->sims) s.have_snaps = true;
     return rc;
@@ -598,6 +665,41 @@ extern "C" int rtm_batch_get_group(rtm_batch *q, int *sims_per_group, int *group
     return 0;
 }
 
+static int batch_set_lanes(rtm_batch *q, int lanes)
+{
+    if (lanes < 1 || lanes > kMaxLanes) {
+        rtm_set_error("launch groups in flight must be 1 to %d (got %d)", kMaxLanes, lanes);
+        return -1;
+    }
+    for (int j = 1; j < lanes; j++) {
+        if (!q->lane_s[j] && hipStreamCreateWithFlags(&q->lane_s[j], hipStreamNonBlocking) != hipSuccess) {
+            rtm_set_error("rtm_batch_set_lanes: stream creation failed");
+            return -1;
+        }
+        if (!q->lane_ev[j] && hipEventCreateWithFlags(&q->lane_ev[j], hipEventDisableTiming) != hipSuccess) {
+            rtm_set_error("rtm_batch_set_lanes: event creation failed");
+            return -1;
+        }
+    }
+    q->lanes = lanes;
+    return batch_set_tiling(q, q->rz, q->graph_block, q->interior, q->seg_rows);
+}
+
+extern "C" int rtm_batch_set_lanes(rtm_batch *q, int lanes)
+{
+    if (!q) { rtm_set_error("rtm_batch_set_lanes: null batch"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    return batch_set_lanes(q, lanes);
+}
+
+extern "C" int rtm_batch_get_lanes(rtm_batch *q, int *lanes)
+{
+    if (!q || !lanes) { rtm_set_error("rtm_batch_get_lanes: bad argument"); return -1; }
+    *lanes = q->lanes;
+    return 0;
+}
+
 // Group sizes tried: the size whose per-step working set fills the cache
 // budget (kGroupBytes at 16 B per padded point), h, scaled by 1/2, 5/8, 3/4,
 // 7/8, 1, 5/4, 3/2 and 2, each taken to the largest group of the balanced
@@ -613,6 +715,7 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune_group: bad argument"); return -1; }
     HIP_TRY(hipSetDevice(q->sims[0].p.device));
     HIP_TRY(hipStreamSynchronize(q->stream));
+    q->lanes = 1;   // the sizes of groups run one at a time (rtm_batch_autotune_lanes tries more)
     const int N = (int)q->sims.size();
     const size_t per = q->sims[0].nfield * (kStepBytesPerPoint / 4) * q->sims[0].esz;
     const int h = (int)std::max<size_t>(1, kGroupBytes / per);
@@ -661,6 +764,118 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
     if (batch_set_tiling(q, best_shape.rz, q->graph_block, best_shape.interior, best_shape.seg)) return -1;
     best_shape.group = q->group;
     tuner_store(batch_key(q), best_shape);
+    return 0;
+}
+
+// Times a wave of launch groups 0 .. L-1 in flight at once on their lanes,
+// each stepping `steps` forward steps that write its snapshot stores (as a
+// pass does; the next forward pass rewrites them); *us: the wave's span per
+// step and realization.
+template <typename R>
+static int time_wave(rtm_batch *q, int L, int steps, double *us)
+{
+    const int flags = q->sims[0].ringmode ? 0 : rtm::F_SNAP;
+    int nsim = 0;
+    for (int j = 0; j < L; j++) {
+        Exec &ex = q->ex[j];
+        const hipStream_t st = lane_stream(q, j);
+        const int s0 = group_first(q, j), n = group_len(q, j);
+        std::vector<SimDesc<R>> d;
+        for (int s = 0; s < n; s++) {
+            SimDesc<R> x = q->sims[s0 + s].desc<R>();
+            x.phase = rtm::PH_FORWARD;
+            x.flags = flags;
+            x.src_j = -1;
+            d.push_back(x);
+        }
+        nsim += n;
+        if (ex.upload_descs(d.data(), n, st) || ex.set_n0(std::vector<int>(n, 0), st)) return -1;
+        const SimBuffers &b0 = q->sims[s0];
+        ex.set_schedule(b0.light_steps() ? rtm::PH_FORWARD : 0, b0.p.number_of_time_steps, b0.p.time_slices, flags);
+        const int warm = q->graph_block > 1 ? q->graph_block : 10;   // and capture
+        if (ex.run(warm, q->graph_block, st) || ex.set_n0(std::vector<int>(n, 0), st)) return -1;
+    }
+    HIP_TRY(hipEventRecord(q->ev0, q->stream));
+    for (int j = 1; j < L; j++) HIP_TRY(hipStreamWaitEvent(q->lane_s[j], q->ev0, 0));
+    int rc = 0;
+    for (int j = 0; j < L && rc == 0; j++) rc = q->ex[j].run(steps, q->graph_block, lane_stream(q, j));
+    for (int j = 1; j < L; j++) {
+        (void)hipEventRecord(q->lane_ev[j], q->lane_s[j]);
+        (void)hipStreamWaitEvent(q->stream, q->lane_ev[j], 0);
+    }
+    HIP_TRY(hipEventRecord(q->ev1, q->stream));
+    HIP_TRY(hipEventSynchronize(q->ev1));
+    if (rc) return -1;
+    for (int j = 0; j < L; j++)
+        if (q->ex[j].check_err(q->stream)) return -1;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, q->ev0, q->ev1));
+    *us = ms * 1e3 / ((double)(steps > 0 ? steps : 1) * nsim);
+    return 0;
+}
+
+// Launch groups in flight at once: one lane at the current group size, then
+// two lanes at group sizes of 1/4 to 1/2 of the size that fills the cache
+// budget (each balanced over an even number of groups), each timed as a wave
+// of its first groups in flight at the current shape; the first within 2%
+// of the least time per realization-step wins.
+extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *lanes, int *group_sizes,
+                                        double *us_per_sim_step, int *n_out)
+{
+    if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune_lanes: bad argument"); return -1; }
+    HIP_TRY(hipSetDevice(q->sims[0].p.device));
+    HIP_TRY(hipStreamSynchronize(q->stream));
+    const int N = (int)q->sims.size();
+    const size_t per = q->sims[0].nfield * (kStepBytesPerPoint / 4) * q->sims[0].esz;
+    const int h = (int)std::max<size_t>(1, kGroupBytes / per);
+    struct Cand {
+        int L, g;
+        double t;
+    };
+    std::vector<Cand> cand;
+    if (q->lanes != 1 && batch_set_lanes(q, 1)) return -1;
+    cand.push_back({1, group_size(q), 0.0});
+    for (int f8 : {2, 3, 4, 5}) {
+        const int g = std::max(1, (h * f8 + 7) / 8);
+        int ng = (N + g - 1) / g;
+        ng = std::min(N, (ng + 1) / 2 * 2);
+        const int gb = (N + ng - 1) / ng;
+        if (ng < 2) continue;
+        bool seen = false;
+        for (const Cand &c : cand) seen = seen || (c.L == 2 && c.g == gb);
+        if (!seen) cand.push_back({2, gb, 0.0});
+    }
+    const int g1 = q->group;
+    int k = 0;
+    for (Cand &c : cand) {
+        q->group = c.L == 1 ? g1 : (c.g >= N ? 0 : c.g);
+        if (batch_set_lanes(q, c.L)) return -1;
+        const int L = std::min(c.L, (int)q->ex.size());
+        const int rc = q->real == RTM_REAL_F64 ? time_wave<double>(q, L, steps, &c.t)
+                                               : time_wave<float>(q, L, steps, &c.t);
+        if (rc) return -1;
+        if (k < cap) {
+            if (lanes) lanes[k] = c.L;
+            if (group_sizes) group_sizes[k] = group_size(q);
+            if (us_per_sim_step) us_per_sim_step[k] = c.t;
+        }
+        k++;
+    }
+    if (n_out) *n_out = k < cap ? k : cap;
+    double best_t = 1e30;
+    for (const Cand &c : cand) best_t = std::min(best_t, c.t);
+    const Cand *win = &cand[0];
+    for (const Cand &c : cand)
+        if (c.t <= best_t * 1.02) {
+            win = &c;
+            break;
+        }
+    q->group = win->L == 1 ? g1 : (win->g >= N ? 0 : win->g);
+    if (batch_set_lanes(q, win->L)) return -1;
+    Shape sh{q->rz, q->interior, q->seg_rows};
+    sh.group = q->group;
+    sh.lanes = q->lanes;
+    tuner_store(batch_key(q), sh);
     return 0;
 }
 

@@ -1045,6 +1045,7 @@ constexpr size_t kGroupBytes = size_t(224) << 20;
 struct Shape {
     int rz, interior, seg;
     int group = 0;      // batches: realizations per launch group (0: all)
+    int lanes = 1;      // batches: launch groups in flight at once
 };
 
 // the shapes tried: tile heights 4..32 rows, and the column kernel beside

@@ -283,14 +283,17 @@ def test_batch_marched_bitexact(gpu_available, interior, seg):
             assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
 
 
-@pytest.mark.parametrize("rz,group,interior", [(2, 0, 0), (8, 0, 0), (4, 2, 0), (2, 4, 0), (2, 4, 3), (4, 0, 3)])
-def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior):
+@pytest.mark.parametrize("rz,group,interior,lanes", [(2, 0, 0, 1), (8, 0, 0, 1), (4, 2, 0, 1), (2, 4, 0, 1),
+                                                     (2, 4, 3, 1), (4, 0, 3, 1), (4, 2, 0, 2), (4, 0, 0, 2),
+                                                     (2, 1, 0, 3), (4, 2, 3, 4)])
+def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior, lanes):
     """A batch of realizations advanced in lockstep: each image bit-identical
     to the oracle's for that realization (alone or in the batch, in one
     launch group or in groups of at most 2 or 4: 5 realizations run as
-    2, 2, 1 or as 3, 2), and the FP64 Welford mean/variance equal to numpy's
-    over the same images."""
-    say(f"[gpu] batch ensemble rz={rz} group={group} interior={interior}")
+    2, 2, 1 or as 3, 2; with 2 to 4 groups in flight at once on lanes of
+    their own, the group count a multiple of the lanes), and the FP64
+    Welford mean/variance equal to numpy's over the same images."""
+    say(f"[gpu] batch ensemble rz={rz} group={group} interior={interior} lanes={lanes}")
     cfg = common.REF_CONFIGS["asym"]
     g = golden("asym")
     op = common.orc_params(cfg)
@@ -305,7 +308,11 @@ def test_batch_ensemble_bitexact_and_welford(gpu_available, rz, group, interior)
     with R.Batch(p, len(models)) as b:
         b.set_tiling(rz, 100, interior)
         b.set_group(group)
+        b.set_lanes(lanes)
+        assert b.lanes() == lanes
         ng = (len(models) + group - 1) // group if group else 1
+        if lanes > 1:
+            ng = min(len(models), (ng + lanes - 1) // lanes * lanes)
         # balanced groups: the largest holds ceil(N / groups)
         assert b.group() == ((len(models) + ng - 1) // ng, ng)
         for k, v in enumerate(vpads):
@@ -358,6 +365,41 @@ def test_batch_group_autotune_keeps_results(gpu_available):
             assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
     with R.Batch(p, len(vpads)) as b2:
         assert b2.group() == (gsize, ngroups) and b2.tiling() == tiling
+
+
+def test_batch_lanes_autotune_keeps_results(gpu_available):
+    """The lanes autotuner times one lane at the current group size, then two
+    lanes; it keeps the first row within 2% of the fastest, the images stay
+    bit-identical to the oracle's, and a new batch of the same grids starts
+    with the tuned lanes and group size."""
+    say("[gpu] batch lanes autotune")
+    cfg = common.REF_CONFIGS["asym"]
+    g = golden("asym")
+    op = common.orc_params(cfg)
+    p = common.rtm_params(cfg)
+    sx, sz = common.shot_positions(cfg)[0]
+    d = g["seismograms"][0]
+    vpads = [O.extend(op, g["models"][0] * np.float32(1 + 0.04 * k)) for k in range(9)]
+    refs = [O.rtm_shot(op, sx, sz, v, g["wavelet"], d) for v in vpads[:3]]
+    with R.Batch(p, len(vpads)) as b:
+        for k, v in enumerate(vpads):
+            b.set_velocity(k, v)
+        b.set_wavelet(g["wavelet"])
+        b.set_seismogram(d)
+        rows = b.autotune_lanes(20)
+        assert rows[0]["lanes"] == 1 and len(rows) >= 2 and all(r["lanes"] == 2 for r in rows[1:])
+        assert all(r["us_per_sim_step"] > 0 for r in rows)
+        fastest = min(r["us_per_sim_step"] for r in rows)
+        win = next(r for r in rows if r["us_per_sim_step"] <= fastest * 1.02)
+        assert b.lanes() == win["lanes"] and b.group()[0] == win["group"]
+        state = (b.lanes(), b.group(), b.tiling())
+        b.zero_images()
+        b.migrate(sx, sz)
+        for k in range(3):
+            I, Is = b.image(k)
+            assert np.array_equal(I, refs[k][0]) and np.array_equal(Is, refs[k][1]), k
+    with R.Batch(p, len(vpads)) as b2:
+        assert (b2.lanes(), b2.group(), b2.tiling()) == state
 
 
 def test_refused_tiling_keeps_the_current_shape(gpu_available):

@@ -152,13 +152,14 @@ def test_c2_launch_group_of_13_on_c1_grid(gpu_available, shape):
 _C2_PLAIN = {}
 
 
-@pytest.mark.parametrize("shape", [None, (4, 100, R.INTERIOR_PAIRS, 0)])
-def test_c2_full_ensemble_equals_plain_images(gpu_available, shape):
-    """The whole c2 workload (64 realizations, nt = 4000, TS = 20, launch
-    groups of 13), in the default shape and with two-step pair launches:
-    every batched image equals the plain plan's image of the same
-    realization (64x16 tiles), bit for bit (the plain path is bit-exact
-    against the oracle and the reference at this size)."""
+@pytest.mark.parametrize("shape,group,lanes", [(None, 13, 1), ((4, 100, R.INTERIOR_PAIRS, 0), 13, 1), (None, 6, 2)])
+def test_c2_full_ensemble_equals_plain_images(gpu_available, shape, group, lanes):
+    """The whole c2 workload (64 realizations, nt = 4000, TS = 20): launch
+    groups of 13 in the default shape and with two-step pair launches, and
+    groups of at most 6 two at a time (12 groups on two lanes): every
+    batched image equals the plain plan's image of the same realization
+    (64x16 tiles), bit for bit (the plain path is bit-exact against the
+    oracle and the reference at this size)."""
     nsims = 64
     p = c1(4000, 20)
     v_true, _, w = bench.c1_inputs(p)
@@ -183,10 +184,11 @@ def test_c2_full_ensemble_equals_plain_images(gpu_available, shape):
         say("[gpu] c2 full ensemble: plain images done")
     d, plain = _C2_PLAIN["d"], _C2_PLAIN["plain"]
     with R.Batch(p, nsims) as b:
-        b.set_group(13)
+        b.set_group(group)
+        b.set_lanes(lanes)
         if shape:
             b.set_tiling(*shape)
-        assert b.group() == (13, 5)
+        assert b.group() == ((13, 5) if lanes == 1 else (6, 12))
         for k, v in enumerate(vpads):
             b.set_velocity(k, v)
         b.set_wavelet(w)
