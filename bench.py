@@ -361,6 +361,9 @@ def main() -> None:
     ap.add_argument("--serve-arena-gb", type=float, default=200.0, help="HBM arena of the request server")
     ap.add_argument("--serve-store", choices=("i16", "fp16"), default="i16",
                     help="request server's snapshot store: int16 blocks or FP16 with a per-snapshot scale")
+    ap.add_argument("--serve-lanes", type=int, default=2, help="launch groups in flight at once in the server")
+    ap.add_argument("--serve-group-mib", type=int, default=112,
+                    help="launch-group budget of the server in MiB (0: the library's default, 224)")
     ap.add_argument("--serve-tiling", default="4,100,0,0",
                     help="request server kernel shape: rows_per_thread,graph_block,interior,seg_rows")
     ap.add_argument("--tiling", default=None,
@@ -403,7 +406,8 @@ def main() -> None:
     if args.only_serve:
         print(json.dumps(run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
                                      tuple(int(v) for v in args.serve_tiling.split(",")),
-                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16)), flush=True)
+                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16,
+                                     lanes=args.serve_lanes, group_mib=args.serve_group_mib)), flush=True)
         return
     p = c1_params()
     X, Z, nt = p.X, p.Z, p.number_of_time_steps
@@ -558,7 +562,8 @@ def main() -> None:
     if args.serve > 0:
         out["serving"] = run_serving(R, args.serve, args.serve_rate, args.serve_arena_gb,
                                      tuple(int(v) for v in args.serve_tiling.split(",")),
-                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16)
+                                     store=R.SNAP_I16 if args.serve_store == "i16" else R.SNAP_FP16,
+                                     lanes=args.serve_lanes, group_mib=args.serve_group_mib)
     if not args.no_cpu:
         log("[bench] timing the reference CPU path on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(p, v_mig, w, seis)
@@ -705,7 +710,8 @@ def serve_params(R, cls, store=None):
         number_of_simulations=1, zfp_tolerance=1e-4, time_slices=4, snapshot_mode=store, image_kahan=1)
 
 
-def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11, store=None):
+def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11, store=None, lanes=1,
+                group_mib=0):
     """Config c4: a Poisson stream of `nreq` migration requests, grid class
     drawn uniformly from SERVE_CLASSES, each with its own seeded folded /
     faulted velocity realization and its own source position along the
@@ -787,6 +793,9 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11, s
 
     with R.Server(int(arena_gb * 1e9), max_slots=48, snapshot_mode=store) as srv:
         srv.set_tiling(*shape)
+        srv.set_lanes(lanes)
+        if group_mib:
+            srv.set_group_bytes(group_mib << 20)
         res = serve.run(srv, reqs, on_done=keep,
                         progress=lambda n, t: log(f"[bench] c4 {n} requests served at {t:.1f} s"))
         # capacity: the same requests all waiting at t = 0, in launch groups
@@ -805,6 +814,7 @@ def run_serving(R, nreq, rate_factor, arena_gb, shape=(4, 100, 0, 0), seed=11, s
                      + f"; {STORE_NAMES[store]} snapshots every 4 steps, Kahan image"),
         "snapshot_store": STORE_NAMES[store],
         "requests": nreq, "arrival_rate_per_s": round(rate, 3), "server_tiling": tiling_dict(shape),
+        "launch_groups_in_flight": lanes,
         "class_counts": [int((classes == k).sum()) for k in range(len(SERVE_CLASSES))],
         "latency_s": {k: round(v, 4) for k, v in res["latency_s"].items()},
         "images_per_hour": round(res["images_per_hour"], 1),

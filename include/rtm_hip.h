@@ -291,8 +291,9 @@ int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_si
 int rtm_batch_set_lanes(rtm_batch *batch, int lanes);
 int rtm_batch_get_lanes(rtm_batch *batch, int *lanes);
 /* Times one lane at the current group size, then two lanes at group sizes
- * of 1/4 to 1/2 of the size that fills the 224 MiB budget (balanced over an
- * even number of groups), each as a wave of its first groups in flight at
+ * of 1/8 to 5/8 of the size that fills the 224 MiB budget and three lanes at
+ * 1/8 and 1/4 (balanced over a multiple of the lanes), each as a wave of its
+ * first groups in flight at
  * the current shape, stepping forward steps that write the snapshot stores;
  * keeps the first within 2% of the least time per realization-step.  Row k
  * = lanes[k], group_sizes[k] (the largest group), us_per_sim_step[k]. */
@@ -374,6 +375,11 @@ int rtm_server_failures(rtm_server *server);
  * last-level cache.  Results do not depend on the grouping. */
 int rtm_server_set_group_bytes(rtm_server *server, size_t bytes);
 int rtm_server_get_group_bytes(rtm_server *server, size_t *bytes, int *groups);
+/* Launch groups in flight at once (1 to 4; default 1): group g of a tick
+ * steps on lane g % lanes, a stream of its own, so the tick's groups run
+ * together.  Results do not depend on the lanes. */
+int rtm_server_set_lanes(rtm_server *server, int lanes);
+int rtm_server_get_lanes(rtm_server *server, int *lanes);
 
 #ifdef __cplusplus
 }

@@ -179,6 +179,8 @@ def lib() -> C.CDLL:
         "rtm_server_stats": (C.c_int, [plan_p, i32p, i32p, P(C.c_size_t), P(C.c_longlong)]),
         "rtm_server_set_group_bytes": (C.c_int, [plan_p, C.c_size_t]),
         "rtm_server_get_group_bytes": (C.c_int, [plan_p, P(C.c_size_t), i32p]),
+        "rtm_server_set_lanes": (C.c_int, [plan_p, C.c_int]),
+        "rtm_server_get_lanes": (C.c_int, [plan_p, i32p]),
         "rtm_server_failures": (C.c_int, [plan_p]),
     }
     for name, (res, args) in sig.items():
@@ -635,6 +637,15 @@ class Server:
         b, n = C.c_size_t(), C.c_int()
         _check(lib().rtm_server_get_group_bytes(self.h, C.byref(b), C.byref(n)), "get_group_bytes")
         return b.value, n.value
+
+    def set_lanes(self, lanes: int) -> None:
+        """rtm_server_set_lanes: launch groups in flight at once (1 to 4)."""
+        _check(lib().rtm_server_set_lanes(self.h, int(lanes)), "set_lanes")
+
+    def lanes(self) -> int:
+        n = C.c_int()
+        _check(lib().rtm_server_get_lanes(self.h, C.byref(n)), "get_lanes")
+        return n.value
 
     def submit(self, p: RtmParams, vpad: np.ndarray, wavelet: np.ndarray, seis: np.ndarray, shot_x: int,
                shot_z: int) -> int:

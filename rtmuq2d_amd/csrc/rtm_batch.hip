@@ -815,10 +815,10 @@ static int time_wave(rtm_batch *q, int L, int steps, double *us)
 }
 
 // Launch groups in flight at once: one lane at the current group size, then
-// two lanes at group sizes of 1/4 to 1/2 of the size that fills the cache
-// budget (each balanced over an even number of groups), each timed as a wave
-// of its first groups in flight at the current shape; the first within 2%
-// of the least time per realization-step wins.
+// two lanes at group sizes of 1/8 to 5/8 of the size that fills the cache
+// budget and three at 1/8 and 1/4 (each balanced over a multiple of the
+// lanes), each timed as a wave of its first groups in flight at the current
+// shape; the first within 2% of the least time per realization-step wins.
 extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *lanes, int *group_sizes,
                                         double *us_per_sim_step, int *n_out)
 {
@@ -835,15 +835,17 @@ extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *l
     std::vector<Cand> cand;
     if (q->lanes != 1 && batch_set_lanes(q, 1)) return -1;
     cand.push_back({1, group_size(q), 0.0});
-    for (int f8 : {2, 3, 4, 5}) {
-        const int g = std::max(1, (h * f8 + 7) / 8);
+    // (lanes, eighths of h): two lanes of h/8 .. 5h/8, three of h/8 .. h/4
+    const int tries[][2] = {{2, 1}, {2, 2}, {2, 3}, {2, 4}, {2, 5}, {3, 1}, {3, 2}};
+    for (const auto &t : tries) {
+        const int L = t[0], g = std::max(1, (h * t[1] + 7) / 8);
         int ng = (N + g - 1) / g;
-        ng = std::min(N, (ng + 1) / 2 * 2);
+        ng = std::min(N, (ng + L - 1) / L * L);
         const int gb = (N + ng - 1) / ng;
-        if (ng < 2) continue;
+        if (ng < L) continue;
         bool seen = false;
-        for (const Cand &c : cand) seen = seen || (c.L == 2 && c.g == gb);
-        if (!seen) cand.push_back({2, gb, 0.0});
+        for (const Cand &c : cand) seen = seen || (c.L == L && c.g == gb);
+        if (!seen) cand.push_back({L, gb, 0.0});
     }
     const int g1 = q->group;
     int k = 0;

@@ -62,8 +62,14 @@ SimDesc<float> backward_desc(const SReq &r)
     return d;
 }
 
+constexpr int kServerLanes = 4;
+
 // The device side of the server: the arena, one executor per launch group,
-// one stream.  Every call runs under the core's tick lock.
+// one stream, and up to three more lanes: with lanes > 1 launch group g steps
+// on lane g % lanes, so the groups of a tick run at once and one group's
+// steps fill the gaps between another's launches.  Every other call first
+// joins the lanes into the stream (join), so nothing on the stream overtakes
+// a group's steps.  Every call runs under the core's tick lock.
 struct HipBackend {
     using State = ReqState;
     int device = 0;
@@ -74,7 +80,23 @@ struct HipBackend {
     std::vector<std::unique_ptr<Exec>> ex;      // one executor per launch group
     std::vector<std::vector<int>> members_of;   // group g's request slots (as of its last prepare)
     hipStream_t stream = nullptr;
+    int lanes = 1;
+    hipStream_t lane_s[kServerLanes] = {};      // lane 0 is `stream`
+    hipEvent_t lane_ev[kServerLanes] = {}, fork_ev = nullptr;
+    bool lanes_busy = false;
     std::string err;
+
+    // the stream waits for every lane's queued steps
+    void join()
+    {
+        if (!lanes_busy) return;
+        for (int j = 1; j < kServerLanes; j++)
+            if (lane_s[j]) {
+                (void)hipEventRecord(lane_ev[j], lane_s[j]);
+                (void)hipStreamWaitEvent(stream, lane_ev[j], 0);
+            }
+        lanes_busy = false;
+    }
 
     std::string error() const { return err; }
     int fail(const char *what)
@@ -91,6 +113,7 @@ struct HipBackend {
     int admit(SReq &r, int slot)
     {
         (void)slot;
+        join();
         if (rtm_server_request_footprint(&r.p) > arena.size - arena.in_use) return 1;
         r.st.b.arena = &arena;
         if (r.st.b.alloc(r.p, RTM_REAL_F32)) {   // no block large enough: wait or fail
@@ -110,12 +133,17 @@ struct HipBackend {
         return 0;
     }
 
-    void release(SReq &r) { r.st.b.release(); }
+    void release(SReq &r)
+    {
+        join();
+        r.st.b.release();
+    }
 
     size_t step_bytes(const SReq &r) const { return r.st.b.nfield * kStepBytesPerPoint; }
 
     int prepare(int g, const std::vector<const SReq *> &members)
     {
+        join();
         while ((int)ex.size() <= g) ex.emplace_back(new Exec());
         std::vector<int4> tiles, inner;
         for (const SReq *r : members) append_work(tiles, inner, r->slot, r->p, rz, interior, seg_rows);
@@ -139,6 +167,7 @@ struct HipBackend {
     // the caller has synchronised the stream: no launch reads these executors
     void trim(int ngroups)
     {
+        join();
         while ((int)ex.size() > ngroups) {
             ex.back()->release();
             ex.pop_back();
@@ -148,12 +177,14 @@ struct HipBackend {
 
     void reset(int g)
     {
+        join();
         if (g < (int)ex.size()) ex[g]->release();
         if (g < (int)members_of.size()) members_of[g].clear();
     }
 
     int upload(int g, const std::vector<const SReq *> &slots)
     {
+        join();
         std::vector<SimDesc<float>> d(max_slots, SimDesc<float>{});
         std::vector<int> n0(max_slots, 0);   // a request's counter: steps done in its current pass
         for (int i = 0; i < max_slots; i++) {
@@ -180,14 +211,56 @@ struct HipBackend {
     int run(int g, int steps)
     {
         (void)hipGetLastError();   // an earlier group's error is that group's
-        if (ex[g]->run(steps, steps, stream)) return fail("launch group");
+        const int j = lanes > 1 ? g % lanes : 0;
+        hipStream_t s = stream;
+        if (j > 0) {
+            // the lane starts after everything queued on the stream so far
+            hipError_t e = hipEventRecord(fork_ev, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(lane_s[j], fork_ev, 0);
+            if (e != hipSuccess) return hip_fail("lane", e);
+            s = lane_s[j];
+            lanes_busy = true;
+        }
+        if (ex[g]->run(steps, steps, s)) return fail("launch group");
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail("launch group", e);
         return 0;
     }
 
+    int set_lanes(int n)
+    {
+        join();
+        if (hipStreamSynchronize(stream) != hipSuccess) return fail("lanes");
+        for (int j = 1; j < n; j++) {
+            if (!lane_s[j] && hipStreamCreateWithFlags(&lane_s[j], hipStreamNonBlocking) != hipSuccess)
+                return fail("lane stream");
+            if (!lane_ev[j] && hipEventCreateWithFlags(&lane_ev[j], hipEventDisableTiming) != hipSuccess)
+                return fail("lane event");
+        }
+        if (!fork_ev && hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess)
+            return fail("lane event");
+        lanes = n;
+        return 0;
+    }
+
+    void destroy_lanes()
+    {
+        for (int j = 1; j < kServerLanes; j++) {
+            if (lane_s[j]) (void)hipStreamSynchronize(lane_s[j]);
+            if (lane_s[j]) (void)hipStreamDestroy(lane_s[j]);
+            if (lane_ev[j]) (void)hipEventDestroy(lane_ev[j]);
+            lane_s[j] = nullptr;
+            lane_ev[j] = nullptr;
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        fork_ev = nullptr;
+        lanes_busy = false;
+        lanes = 1;
+    }
+
     int end_forward(SReq &r)
     {
+        join();
         if (r.st.b.flush_last_snapshot(stream)) return fail("last snapshot");
         r.st.b.reset_state(stream);
         return 0;
@@ -195,6 +268,7 @@ struct HipBackend {
 
     int finish(SReq &r)
     {
+        join();
         const int nx = r.p.nx, nz = r.p.nz;
         std::vector<float> h(r.st.b.nint);
         hipError_t e = hipMemcpyAsync(h.data(), r.st.b.I, sizeof(float) * r.st.b.nint, hipMemcpyDeviceToHost, stream);
@@ -208,6 +282,7 @@ struct HipBackend {
 
     int sync()
     {
+        join();
         const hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return hip_fail("stream", e);
         return 0;
@@ -231,12 +306,14 @@ extern "C" size_t rtm_server_request_footprint(const rtm_params *p)
 
 static void server_free(rtm_server *s)
 {
+    s->be.join();
     if (s->be.stream) (void)hipStreamSynchronize(s->be.stream);
     for (auto &r : s->core.slots)
         if (r) s->be.release(*r);
     s->core.slots.clear();
     s->be.trim(0);
     s->be.arena.release();
+    s->be.destroy_lanes();
     if (s->be.stream) (void)hipStreamDestroy(s->be.stream);
     s->be.stream = nullptr;
 }
@@ -359,6 +436,28 @@ extern "C" int rtm_server_set_group_bytes(rtm_server *s, size_t bytes)
     std::lock_guard<std::mutex> l(s->core.tick_mu);
     s->core.group_bytes = bytes;
     s->core.lists_dirty = true;
+    return 0;
+}
+
+extern "C" int rtm_server_set_lanes(rtm_server *s, int lanes)
+{
+    if (!s || lanes < 1 || lanes > kServerLanes) {
+        rtm_set_error("rtm_server_set_lanes: launch groups in flight must be 1 to %d", kServerLanes);
+        return -1;
+    }
+    std::lock_guard<std::mutex> l(s->core.tick_mu);
+    HIP_TRY(hipSetDevice(s->be.device));
+    if (s->be.set_lanes(lanes)) {
+        rtm_set_error("%s", s->be.error().c_str());
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int rtm_server_get_lanes(rtm_server *s, int *lanes)
+{
+    if (!s || !lanes) { rtm_set_error("rtm_server_get_lanes: bad argument"); return -1; }
+    *lanes = s->be.lanes;
     return 0;
 }
 

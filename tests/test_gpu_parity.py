@@ -387,7 +387,7 @@ def test_batch_lanes_autotune_keeps_results(gpu_available):
         b.set_wavelet(g["wavelet"])
         b.set_seismogram(d)
         rows = b.autotune_lanes(20)
-        assert rows[0]["lanes"] == 1 and len(rows) >= 2 and all(r["lanes"] == 2 for r in rows[1:])
+        assert rows[0]["lanes"] == 1 and len(rows) >= 2 and all(r["lanes"] in (2, 3) for r in rows[1:])
         assert all(r["us_per_sim_step"] > 0 for r in rows)
         fastest = min(r["us_per_sim_step"] for r in rows)
         win = next(r for r in rows if r["us_per_sim_step"] <= fastest * 1.02)
@@ -1144,17 +1144,18 @@ def _sub(req):
     return p, vpad, wav, seis, sx, sz
 
 
-@pytest.mark.parametrize("group_bytes", [None, 1])
+@pytest.mark.parametrize("group_bytes,lanes", [(None, 1), (1, 1), (1, 2), (1, 3)])
 @pytest.mark.parametrize("block", [50, 100])
-def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group_bytes, block):
+def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group_bytes, lanes, block):
     """The request server with the int16 block store: requests of three grid
     sizes arriving while others are in flight, each image equal to the
     oracle's for that request alone under the store's rule, in one launch
     group and with every request in a group of its own.  With blocks of 100
     steps (a multiple of the snapshot interval 4) the steps at which no
     member of a launch group stores or images run the light tile instance;
-    with 50 every step runs the full one."""
-    say(f"[gpu] request server, int16 block store, group_bytes {group_bytes} block {block}")
+    with 50 every step runs the full one.  With each request in a group of
+    its own, 2 or 3 lanes step the tick's groups at once."""
+    say(f"[gpu] request server, int16 block store, group_bytes {group_bytes} lanes {lanes} block {block}")
     reqs = _server_requests()
     refs = []
     for cfg, op, p, vpad, wav, seis, sx, sz in reqs:
@@ -1165,6 +1166,8 @@ def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group
     got = {}
     with R.Server(budget, max_slots=4, snapshot_mode=R.SNAP_I16) as srv:
         srv.set_tiling(4, block, 0, 0)
+        srv.set_lanes(lanes)
+        assert srv.lanes() == lanes
         with pytest.raises(R.RtmError, match="tile shapes only"):
             srv.set_tiling(4, 50, 1, 16)
         if group_bytes is not None:
@@ -1182,8 +1185,8 @@ def test_server_int16_block_store_same_alone_and_in_the_mix(gpu_available, group
         assert np.array_equal(got[rid], refs[k]), k
 
 
-@pytest.mark.parametrize("group_bytes,block", [(1, 100), (None, 100), (None, 30)])
-def test_server_fp32_store_matches_the_oracle(gpu_available, group_bytes, block):
+@pytest.mark.parametrize("group_bytes,block,lanes", [(1, 100, 1), (None, 100, 1), (None, 30, 1), (1, 100, 2)])
+def test_server_fp32_store_matches_the_oracle(gpu_available, group_bytes, block, lanes):
     """A server with the FP32 snapshot store (the reference's scheme): every
     image equals the oracle's for that request alone, bit for bit, with all
     requests admitted at once, each in a launch group of its own or all in
@@ -1199,6 +1202,7 @@ def test_server_fp32_store_matches_the_oracle(gpu_available, group_bytes, block)
     with R.Server(sum(R.request_footprint(r[2].copy().update(snapshot_mode=0)) for r in reqs) + (8 << 20),
                   max_slots=4, snapshot_mode=0) as srv:
         srv.set_tiling(4, block, 0, 0)
+        srv.set_lanes(lanes)
         if group_bytes is not None:
             srv.set_group_bytes(group_bytes)
         ids = [srv.submit(*_sub(r)) for r in reqs]
