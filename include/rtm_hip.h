@@ -370,9 +370,11 @@ int rtm_server_failures(rtm_server *server);
  * once no other thread calls into the server. */
 /* Launch groups: the active requests advance in groups whose per-step
  * working sets (16 B per padded grid point) sum to at most `bytes`, group
- * after group, each a block of steps; 0 puts every active request in one
- * group.  The default, 224 MiB, keeps a group within the device's 256 MB
- * last-level cache.  Results do not depend on the grouping. */
+ * after group, each a block of steps; a group holds requests of one pass
+ * (forward or backward), so its launches have one pass's store or imaging
+ * steps.  0 puts every active request in one group.  The default, 224 MiB,
+ * keeps a group within the device's 256 MB last-level cache.  Results do
+ * not depend on the grouping. */
 int rtm_server_set_group_bytes(rtm_server *server, size_t bytes);
 int rtm_server_get_group_bytes(rtm_server *server, size_t *bytes, int *groups);
 /* Launch groups in flight at once (1 to 4; default 1): group g of a tick

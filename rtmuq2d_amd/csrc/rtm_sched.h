@@ -185,7 +185,11 @@ struct Core {
     }
 
     // launch groups: greedily in slot order, per-step working sets summing
-    // to at most group_bytes; each group's executor gets its work lists
+    // to at most group_bytes; each group's executor gets its work lists.
+    // With a budget, the forward-pass requests come first and a group never
+    // holds both passes: a launch then has one pass's store or imaging steps
+    // (one step in TS, not two), and the other steps run the light tile
+    // instance.
     void regroup()
     {
         // the last tick's launches may still read the executors' lists
@@ -199,16 +203,20 @@ struct Core {
         }
         groups.clear();
         size_t sum = 0;
-        for (int i = 0; i < max_slots; i++) {
-            if (!slots[i]) continue;
-            const size_t w = be.step_bytes(*slots[i]);
-            if (groups.empty() || (group_bytes && sum + w > group_bytes)) {
-                groups.emplace_back();
-                sum = 0;
+        int last_pass = -1;
+        for (int pass : {(int)P_FORWARD, (int)P_BACKWARD})
+            for (int i = 0; i < max_slots; i++) {
+                if (!slots[i] || (group_bytes && slots[i]->pass != pass) || (!group_bytes && pass != P_FORWARD))
+                    continue;
+                const size_t w = be.step_bytes(*slots[i]);
+                if (groups.empty() || (group_bytes && (sum + w > group_bytes || slots[i]->pass != last_pass))) {
+                    groups.emplace_back();
+                    sum = 0;
+                }
+                groups.back().push_back(i);
+                sum += w;
+                last_pass = slots[i]->pass;
             }
-            groups.back().push_back(i);
-            sum += w;
-        }
         be.trim((int)groups.size());
         for (int g = 0; g < (int)groups.size(); g++) {
             std::vector<const R *> m;
@@ -262,6 +270,7 @@ struct Core {
                     r->pass = P_BACKWARD;
                     r->n = 0;
                     descs_dirty = true;
+                    if (group_bytes) lists_dirty = true;   // groups hold one pass each
                 } else {
                     const int rc = be.finish(*r);
                     std::unique_ptr<R> fin = std::move(slots[i]);

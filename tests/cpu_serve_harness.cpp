@@ -57,6 +57,8 @@ struct Fake {
     std::set<int> in_flight;               // request ids whose device work may still run
     bool lists_in_flight = false;
     int violations = 0;
+    int mixed_pass_groups = 0;             // prepared groups holding both passes
+    std::set<int> passes_grouped;          // passes seen in prepared groups
     std::string err;
 
     std::string error() const { return err; }
@@ -92,6 +94,13 @@ struct Fake {
     int prepare(int g, const std::vector<const FReq *> &m)
     {
         if (lists_in_flight) violations++;
+        for (const FReq *r : m) {
+            passes_grouped.insert(r->pass);
+            if (r->pass != m.front()->pass) {
+                mixed_pass_groups++;
+                break;
+            }
+        }
         if ((int)group_ids.size() <= g) group_ids.resize(g + 1);
         group_ids[g].clear();
         for (const FReq *r : m) group_ids[g].push_back(r->id);
@@ -233,6 +242,31 @@ void end_checks(const Core &c, const Fake &f)
 }
 
 // ---------------------------------------------------------------- scenarios
+
+// with a group budget, a launch group holds requests of one pass only (the
+// groups are rebuilt when a request switches to its backward pass); with
+// budget 0 every active request shares one group
+void s_pass_groups()
+{
+    for (size_t budget : {(size_t)1 << 40, (size_t)0}) {
+        Fake f;
+        Core c(f, 4);
+        c.block = 100;
+        c.group_bytes = budget;
+        std::vector<int> ids, nts;
+        for (int k = 0; k < 8; k++) {
+            nts.push_back(200 + 100 * k);
+            ids.push_back(submit(c, 30, 20, nts.back()));
+        }
+        const Outcome o = drain(c, f);
+        EXPECT(o.failed.empty());
+        for (int k = 0; k < 8; k++) EXPECT(done_ok(o, ids[k], nts[k], 100));
+        EXPECT(f.passes_grouped.size() == 2);
+        if (budget) EXPECT(f.mixed_pass_groups == 0);
+        else EXPECT(f.mixed_pass_groups > 0);
+        end_checks(c, f);
+    }
+}
 
 void s_basic()
 {
@@ -406,7 +440,7 @@ int main(int argc, char **argv)
     const std::map<std::string, void (*)()> sc = {
         {"basic", s_basic}, {"group_failure", s_group_failure}, {"upload_failure", s_upload_failure},
         {"no_fit", s_no_fit}, {"prepare_failure", s_prepare_failure}, {"pass_failures", s_pass_failures},
-        {"sticky", s_sticky}, {"threads", s_threads}};
+        {"sticky", s_sticky}, {"threads", s_threads}, {"pass_groups", s_pass_groups}};
     if (argc < 2 || !sc.count(argv[1])) {
         std::fprintf(stderr, "usage: %s SCENARIO\n", argv[0]);
         return 2;

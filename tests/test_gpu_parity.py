@@ -1133,7 +1133,9 @@ def test_server_each_result_same_alone_and_in_the_mix(gpu_available, shape, grou
                 got[rid] = I
         for rid, I in srv.done():
             got[rid] = I
-    assert most_groups == (2 if group_bytes == 1 else 1)
+    # a group per request (budget 1; the arena holds two), or at most one
+    # group per pass (the default budget: a group holds one pass)
+    assert most_groups == 2 if group_bytes == 1 else most_groups in (1, 2)
     assert sorted(got) == sorted(ids)
     for k, rid in enumerate(ids):
         assert np.array_equal(got[rid], refs[k]), k

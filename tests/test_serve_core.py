@@ -21,7 +21,7 @@ CSRC = os.path.join(common.REPO, "rtmuq2d_amd", "csrc")
 INC = os.path.join(common.REPO, "include")
 SRC = os.path.join(common.REPO, "tests", "cpu_serve_harness.cpp")
 SCENARIOS = ["basic", "group_failure", "upload_failure", "no_fit", "prepare_failure", "pass_failures",
-             "sticky", "threads"]
+             "sticky", "threads", "pass_groups"]
 
 
 def _build(tmp, flags, name):
