@@ -82,30 +82,41 @@ __device__ __forceinline__ void commit_snap_max_wg(unsigned *slots, float m, flo
     }
 }
 
-// LIGHT: an instance without the snapshot store and the imaging, for the
-// steps the executor's schedule says have neither (fewer registers, a
-// higher occupancy, and a prologue without the snapshot index).  The
-// schedule (Exec::heavy_step) and the store / imaging tests of
-// rtm::step_view_q are the same formulas.  It has no early exit for an
-// inactive step (its loads stay in bounds and tile_points stores nothing),
-// so its descriptor reads issue together, ahead of any branch.
+// KIND: which steps the instance serves.
+//   K_LIGHT  steps that neither store nor image (no store or imaging code:
+//            fewer registers, a higher occupancy, and a prologue without
+//            the snapshot index).  It has no early exit for an inactive
+//            step (its loads stay in bounds and tile_points stores
+//            nothing), so its descriptor reads issue together.
+//   K_STORE  forward steps that store a snapshot (no imaging code or
+//            imaging operands); meeting an imaging step sets err bit 1.
+//   K_FULL   every step.
+// The executor's schedule (Exec::heavy_step) and the store / imaging tests
+// of rtm::step_view_q are the same formulas.
 // SAFE: an interior tile inside the stepped region (rtm::tile_safe).
-template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false,
+enum StepKind : int { K_LIGHT = 0, K_STORE = 1, K_FULL = 2 };
+
+template <typename R, int RZ, int CLS, bool Q16, bool I16 = false, int NW = 4, int UM = 1, int KIND = K_FULL,
           bool SAFE = false>
 __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                          int x0, int z0, int tx, int ty, int *err)
 {
+    constexpr bool LIGHT = KIND == K_LIGHT;
     const rtm::StepView<R> V = LIGHT ? rtm::step_view_light<R, UM>(S, n) : rtm::step_view_q<R, Q16, I16, UM>(S, n);
     if (!LIGHT && !V.active) return;
+    if (KIND == K_STORE && !V.forward && (V.snap || V.sfield || V.qsnap)) {   // uniform: an imaging step
+        if (threadIdx.x == 0 && err) atomicOr(err, 1);
+        return;
+    }
     rtm::TileRegs<R, RZ, NW> G;
-    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, !LIGHT, SAFE>(G, S, V, x0, z0, tx, ty);
+    rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, KIND == K_FULL, SAFE>(G, S, V, x0, z0, tx, ty);
     rtm::tile_stage<R, RZ, CLS, NW>(L, G, tx, ty);
     __syncthreads();
     if (CLS) {
         rtm::tile_psi<R, RZ, CLS, NW>(L, G, S, x0, z0, tx, ty);
         __syncthreads();
     }
-    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW, !LIGHT, SAFE>(L, G, S, V, x0, z0, tx, ty);
+    const float m = rtm::tile_points<R, RZ, CLS, Q16, I16, NW, KIND, SAFE>(L, G, S, V, x0, z0, tx, ty);
     if (LIGHT) return;
     if (Q16 && V.smax) commit_snap_max_wg<NW>(V.smax, m, red);
     if (I16 && V.forward && V.qsnap) {   /* uniform: a store step */
@@ -116,26 +127,26 @@ __device__ __forceinline__ void run_tile(rtm::TileLds<R, RZ, NW> &L, float *red,
     }
 }
 
-template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
+template <typename R, int RZ, bool Q16, bool I16 = false, int NW = 4, int UM = 1, int KIND = K_FULL>
 __device__ __forceinline__ void run_tile_cls(rtm::TileLds<R, RZ, NW> &L, float *red, const SimDesc<R> &S, int n,
                                              int4 tile, int tx, int ty, int *err)
 {
     switch (tile.w) {
     case 0:
         if (rtm::tile_safe(S, tile.y, tile.z, NW * RZ))
-            run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT, true>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+            run_tile<R, RZ, 0, Q16, I16, NW, UM, KIND, true>(L, red, S, n, tile.y, tile.z, tx, ty, err);
         else
-            run_tile<R, RZ, 0, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+            run_tile<R, RZ, 0, Q16, I16, NW, UM, KIND>(L, red, S, n, tile.y, tile.z, tx, ty, err);
         break;
-    case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
-    case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    case 1: run_tile<R, RZ, 1, Q16, I16, NW, UM, KIND>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
+    case 2: run_tile<R, RZ, 2, Q16, I16, NW, UM, KIND>(L, red, S, n, tile.y, tile.z, tx, ty, err); break;
     default:
         if constexpr (rtm::corner_tiles_small(RZ, NW)) {
             // the lists cut these corner tiles into 64 x 4 tiles (its LDS
             // holds one psi plane): flag the list as broken
             if (threadIdx.x == 0 && err) atomicOr(err, 2);
         } else {
-            run_tile<R, RZ, 3, Q16, I16, NW, UM, LIGHT>(L, red, S, n, tile.y, tile.z, tx, ty, err);
+            run_tile<R, RZ, 3, Q16, I16, NW, UM, KIND>(L, red, S, n, tile.y, tile.z, tx, ty, err);
         }
         break;
     }
@@ -166,8 +177,8 @@ struct StepLds<R, RZ, NW, true> {
 // NW: wavefronts per workgroup, one row group each (tiles of 64 x NW*RZ).
 // UM: u buffer mask, 1 (two buffers, in-place update) or 3 (four buffers:
 // the tiles beside two-step pair items).
-// LIGHT: the instance without store and imaging (run_tile); err: its flag.
-template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4, int UM = 1, bool LIGHT = false>
+// KIND: the steps the instance serves (run_tile); err: its flags.
+template <typename R, int RZ, bool Q16, bool ONE = false, bool I16 = false, int NW = 4, int UM = 1, int KIND = K_FULL>
 __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict__ sims,
                                                    const int *__restrict__ n0,
                                                    const int4 *__restrict__ tiles,
@@ -192,11 +203,11 @@ __global__ void __launch_bounds__(64 * NW) k_tiles(const SimDesc<R> *__restrict_
     if constexpr (rtm::corner_tiles_small(RZ, NW)) {
         if (tile.w & rtm::T_SMALL) {   // a 64 x 4 piece of a corner tile: one row per thread
             const int4 t4 = make_int4(tile.x, tile.y, tile.z, tile.w & 3);
-            run_tile_cls<R, 1, Q16, I16, 4, UM, LIGHT>(L.small, red, S, n, t4, tx, ty, err);
+            run_tile_cls<R, 1, Q16, I16, 4, UM, KIND>(L.small, red, S, n, t4, tx, ty, err);
             return;
         }
     }
-    run_tile_cls<R, RZ, Q16, I16, NW, UM, LIGHT>(L.full, red, S, n, tile, tx, ty, err);
+    run_tile_cls<R, RZ, Q16, I16, NW, UM, KIND>(L.full, red, S, n, tile, tx, ty, err);
 }
 
 // A mixed tile list (rtm::IN_MIXED): 64 x 4 tiles (T_SMALL) of any class run
@@ -974,7 +985,7 @@ struct Exec {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int block = 0, ring_blocks = 0;
-        std::vector<char> heavy;        // per captured step: the full instance (empty: every step)
+        std::vector<char> heavy;        // per captured step: its StepKind (empty: every step K_FULL)
     };
     // by phase (plans, batches, boundary saving: 0 none, 1 forward, 2
     // backward); 3..6 for the request server's schedules (a group's heavy
@@ -1000,9 +1011,10 @@ struct Exec {
     int run(int steps, int block, hipStream_t s);
     // the launch of step s_in_block of a block; with pair items (IN_PAIRS)
     // an even s_in_block also steps them at s_in_block + 1 when that is
-    // below `lim` (the block's length).  heavy = false: no simulation stores
-    // or images at this step, so the tiles run the light instance.
-    int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30, bool heavy = true);
+    // below `lim` (the block's length).  kind (StepKind): K_LIGHT, no
+    // simulation stores or images at this step; K_STORE, some store a
+    // snapshot and none images; K_FULL, anything.
+    int launch_one(int s_in_block, hipStream_t s, int lim = 1 << 30, int kind = K_FULL);
     // The step schedule of the coming pass (every simulation of the launch at
     // the same step, a store that writes snapshots only in the step that
     // computes them): the forward steps n % ts == 0 store a snapshot (flags
@@ -1019,8 +1031,8 @@ struct Exec {
     int *d_err = nullptr;               // set by a step launch that met a tile its list should not hold
     void set_schedule(int phase, int nt, int ts, int flags);
     void set_group_schedule(const std::vector<int4> &per_sim);
-    bool heavy_step(int n) const;       // step n of the pass stores or images (always without a schedule)
-    bool heavy_at(int t) const;         // step t from the current counters stores or images for some sim
+    int heavy_step(int n) const;        // the StepKind step n of the pass needs (K_FULL without a schedule)
+    int heavy_at(int t) const;          // the StepKind step t from the current counters needs
     bool pattern(int block) const;      // the heavy steps repeat from block to block
     int check_err(hipStream_t s);       // -1 with the error set if d_err was set (and clears it)
     // every captured graph, once the stream they ran on has drained

@@ -657,11 +657,11 @@ int Exec::set_n0(const std::vector<int> &n0, hipStream_t s)
     return 0;
 }
 
-// The light tile instance (no snapshot store, no imaging) for FP32 fields
-// in tile shapes (the same for every store: a light step stores nothing);
-// false when there is none.
-template <typename R, bool Q16, int UM = 1>
-static bool launch_light(const Exec &e, int s_in_block, hipStream_t ts)
+// The light (KIND K_LIGHT: no snapshot store, no imaging) or the store
+// (K_STORE: the snapshot store, no imaging; I16: the int16 block store)
+// tile instance for FP32 fields in tile shapes; false when there is none.
+template <typename R, bool Q16, int UM, int KIND, bool I16 = false>
+static bool launch_kind(const Exec &e, int s_in_block, hipStream_t ts)
 {
     if constexpr (Q16 || sizeof(R) != 4) {
         return false;
@@ -671,20 +671,20 @@ static bool launch_light(const Exec &e, int s_in_block, hipStream_t ts)
         const bool one = e.cap_sims == 1;
         switch (e.rz) {
         case 1:
-            if (one) hipLaunchKernelGGL((k_tiles<R, 1, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
-            else hipLaunchKernelGGL((k_tiles<R, 1, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 1, false, true, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 1, false, false, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
             break;
         case 2:
-            if (one) hipLaunchKernelGGL((k_tiles<R, 2, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
-            else hipLaunchKernelGGL((k_tiles<R, 2, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 2, false, true, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 2, false, false, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
             break;
         case 8:
-            if (one) hipLaunchKernelGGL((k_tiles<R, 8, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
-            else hipLaunchKernelGGL((k_tiles<R, 8, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 8, false, true, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 8, false, false, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
             break;
         default:
-            if (one) hipLaunchKernelGGL((k_tiles<R, 4, false, true, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
-            else hipLaunchKernelGGL((k_tiles<R, 4, false, false, false, 4, UM, true>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            if (one) hipLaunchKernelGGL((k_tiles<R, 4, false, true, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
+            else hipLaunchKernelGGL((k_tiles<R, 4, false, false, I16, 4, UM, KIND>), dim3(grid), dim3(256), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
             break;
         }
         return true;
@@ -692,7 +692,7 @@ static bool launch_light(const Exec &e, int s_in_block, hipStream_t ts)
 }
 
 template <typename R, bool Q16>
-static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s, bool heavy)
+static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s, int kind)
 {
     const SimDesc<R> *d = (const SimDesc<R> *)e.d_desc;
     // interior and strip tiles write disjoint points and read only u^n and
@@ -726,8 +726,10 @@ static void launch_tiles(const Exec &e, int s_in_block, hipStream_t s, bool heav
             if (one) hipLaunchKernelGGL((k_tiles<R, 4, Q16, true, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
             else hipLaunchKernelGGL((k_tiles<R, 4, Q16, false, false, 8>), dim3(grid), dim3(512), 0, ts, d, e.d_n0, e.d_tiles, e.d_ntiles, s_in_block, e.d_err);
         }
-    } else if (e.ntiles && !heavy && launch_light<R, Q16>(e, s_in_block, ts)) {
+    } else if (e.ntiles && kind == K_LIGHT && launch_kind<R, Q16, 1, K_LIGHT>(e, s_in_block, ts)) {
         // a step that neither stores nor images: the light instance
+    } else if (e.ntiles && kind == K_STORE && launch_kind<R, Q16, 1, K_STORE>(e, s_in_block, ts)) {
+        // a forward step that stores a snapshot: the store instance
     } else if (e.ntiles && e.cap_sims == 1) {
         const int grid = 8 * ((e.cap_tiles + 7) / 8);
         switch (e.rz) {
@@ -834,7 +836,7 @@ static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s, 
         else hipLaunchKernelGGL((k_pairs<float, false, 1>), dim3(grid), dim3(256), 0, s, d, e.d_n0, e.d_inner, e.d_ninner, s_in_block, lim);
     }
     if (!e.ntiles) return;
-    if (!heavy && launch_light<float, false, 3>(e, s_in_block, s)) return;
+    if (!heavy && launch_kind<float, false, 3, K_LIGHT>(e, s_in_block, s)) return;
     const int grid = 8 * ((e.cap_tiles + 7) / 8);
     if (one) {
         switch (e.rz) {
@@ -853,17 +855,21 @@ static void launch_pairs(const Exec &e, int s_in_block, int lim, hipStream_t s, 
     }
 }
 
-int Exec::launch_one(int s_in_block, hipStream_t s, int lim, bool heavy)
+int Exec::launch_one(int s_in_block, hipStream_t s, int lim, int kind)
 {
-    if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s, heavy);
-    else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s, true);
+    if (interior == rtm::IN_PAIRS) launch_pairs(*this, s_in_block, lim, s, kind != K_LIGHT);
+    else if (real == RTM_REAL_F64) launch_tiles<double, false>(*this, s_in_block, s, K_FULL);
     else if (i16) {
-        // the light instance is the FP32 one: a light step stores nothing
-        if (heavy || interior != rtm::IN_TILES || !launch_light<float, false>(*this, s_in_block, s))
-            launch_tiles_i16(*this, s_in_block, s);
+        // the light instance is the FP32 one (a light step stores nothing);
+        // the store instance carries the int16 block store
+        const bool done = interior == rtm::IN_TILES &&
+                          (kind == K_LIGHT   ? launch_kind<float, false, 1, K_LIGHT>(*this, s_in_block, s)
+                           : kind == K_STORE ? launch_kind<float, false, 1, K_STORE, true>(*this, s_in_block, s)
+                                             : false);
+        if (!done) launch_tiles_i16(*this, s_in_block, s);
     }
-    else if (q16) launch_tiles<float, true>(*this, s_in_block, s, true);
-    else launch_tiles<float, false>(*this, s_in_block, s, heavy);
+    else if (q16) launch_tiles<float, true>(*this, s_in_block, s, K_FULL);
+    else launch_tiles<float, false>(*this, s_in_block, s, kind);
     // boundary saving: the ring save / reconstruction follows the step on the
     // same stream (running them beside the step on a stream of their own cost
     // more in graph edges than it hid)
@@ -891,24 +897,25 @@ void Exec::set_group_schedule(const std::vector<int4> &per_sim)
     for (auto &g : gsched) g.z = g.z > 0 ? g.z : 1;
 }
 
-// a step of a simulation's pass stores or images; the pattern is periodic
-// in n (also past the pass's end, where the kernels skip the simulation), so
-// a group's heavy steps repeat from block to block
-static bool heavy_of(const int4 &g, int n)
+// the instance a step of a simulation's pass needs (StepKind): a forward
+// snapshot step the store one, a backward imaging step the full one; the
+// pattern is periodic in n (also past the pass's end, where the kernels
+// skip the simulation), so a group's steps repeat from block to block
+static int kind_of(const int4 &g, int n)
 {
     const auto mod = [](int a, int m) { return ((a % m) + m) % m; };
-    if (g.x == rtm::PH_FORWARD) return (g.w & rtm::F_SNAP) && mod(n, g.z) == 0;
-    if (g.x == rtm::PH_BACKWARD) return (g.w & rtm::F_IMAGE) && mod(g.y - 1 - n, g.z) == 0;
-    return false;
+    if (g.x == rtm::PH_FORWARD) return (g.w & rtm::F_SNAP) && mod(n, g.z) == 0 ? K_STORE : K_LIGHT;
+    if (g.x == rtm::PH_BACKWARD) return (g.w & rtm::F_IMAGE) && mod(g.y - 1 - n, g.z) == 0 ? K_FULL : K_LIGHT;
+    return K_LIGHT;
 }
 
-bool Exec::heavy_at(int t) const
+int Exec::heavy_at(int t) const
 {
     if (sched) return heavy_step(h_n0[0] + t);
-    if (gsched.empty()) return true;
-    for (int i = 0; i < (int)gsched.size() && i < cap_sims; i++)
-        if (heavy_of(gsched[i], h_n0[i] + t)) return true;
-    return false;
+    if (gsched.empty()) return K_FULL;
+    int k = K_LIGHT;
+    for (int i = 0; i < (int)gsched.size() && i < cap_sims; i++) k = std::max(k, kind_of(gsched[i], h_n0[i] + t));
+    return k;
 }
 
 bool Exec::pattern(int block) const
@@ -920,12 +927,12 @@ bool Exec::pattern(int block) const
     return true;
 }
 
-bool Exec::heavy_step(int n) const
+int Exec::heavy_step(int n) const
 {
-    if (!sched) return true;
-    if (n < 0 || n >= sched_nt) return false;   // inactive: the kernels skip the simulation
-    if (sched_phase == rtm::PH_FORWARD) return (sched_flags & rtm::F_SNAP) && n % sched_ts == 0;
-    return (sched_flags & rtm::F_IMAGE) && (sched_nt - 1 - n) % sched_ts == 0;
+    if (!sched) return K_FULL;
+    if (n < 0 || n >= sched_nt) return K_LIGHT;   // inactive: the kernels skip the simulation
+    if (sched_phase == rtm::PH_FORWARD) return (sched_flags & rtm::F_SNAP) && n % sched_ts == 0 ? K_STORE : K_LIGHT;
+    return (sched_flags & rtm::F_IMAGE) && (sched_nt - 1 - n) % sched_ts == 0 ? K_FULL : K_LIGHT;
 }
 
 
@@ -937,7 +944,8 @@ int Exec::check_err(hipStream_t s)
     if (!h) return 0;
     fill_zero(d_err, 1, s);
     (void)hipStreamSynchronize(s);
-    rtm_set_error("internal: a step launch met a corner tile that the work list should have cut (flags %d)", h);
+    rtm_set_error("internal: %s", (h & 2) ? "a step launch met a corner tile that the work list should have cut"
+                                            : "a snapshot-store step launch met an imaging step (step schedule mismatch)");
     return -1;
 }
 
@@ -980,7 +988,7 @@ int Exec::run(int steps, int block, hipStream_t s)
             if (c.exec && g_stream) (void)hipStreamSynchronize(g_stream);
             destroy_captured(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int t = 0; t < block; t++) launch_one(t, s, block, heavy.empty() ? true : heavy[t] != 0);
+            for (int t = 0; t < block; t++) launch_one(t, s, block, heavy.empty() ? (int)K_FULL : (int)heavy[t]);
             hipLaunchKernelGGL(k_advance, dim3((cap_sims + 255) / 256), dim3(256), 0, s, d_n0, cap_sims, block);
             hipError_t e = hipStreamEndCapture(s, &c.graph);
             if (e != hipSuccess) { rtm_set_error("graph capture failed: %s", hipGetErrorString(e)); return -1; }

@@ -730,11 +730,12 @@ RTM_HD void tile_psi(TileLds<R, RZ, NW> &L, const TileRegs<R, RZ, NW> &G, const 
 }
 
 /* Phase 3: the fused update of this thread's RZ points. */
-/* XS: the instance carries the snapshot store and the imaging (false: a
- * light instance for steps that have neither, with fewer registers).
+/* XS: what the instance carries: 2 the snapshot store and the imaging, 1
+ * the snapshot store only (forward store steps), 0 neither (a light
+ * instance for steps that have neither, with fewer registers).
  * SAFE: an interior tile inside the stepped region (tile_safe).  Nothing is
  * stored on an inactive step (a light instance runs those through). */
-template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, bool XS = true,
+template <typename R, int RZ, int CLS, bool Q16 = false, bool I16 = false, int NW = 4, int XS = 2,
           bool SAFE = false>
 RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, const SimDesc<R> &S,
                          const StepView<R> &V, int x0, int z0, int tx, int ty)
@@ -827,16 +828,18 @@ RTM_HD float tile_points(const TileLds<R, RZ, NW> &L, TileRegs<R, RZ, NW> &G, co
                 const int q = kk * S.nx + (i - S.TL);
                 if (V.forward) {
                     if (V.snap) stg_stream(V.snap, q, U);
-                } else if (S.flags & F_KAHAN) {
-                    R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
-                    const R s = G.sv[rr];
-                    kahan_add(I, Ic, s * U);
-                    kahan_add(Is, Isc, s * s);
-                    stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
-                } else {
-                    const R s = G.sv[rr];
-                    stg(S.I, q, G.Iv[rr] + s * U);
-                    stg(S.Is, q, G.Isv[rr] + s * s);
+                } else if constexpr (XS == 2) {
+                    if (S.flags & F_KAHAN) {
+                        R I = G.Iv[rr], Ic = G.Icv[rr], Is = G.Isv[rr], Isc = G.Iscv[rr];
+                        const R s = G.sv[rr];
+                        kahan_add(I, Ic, s * U);
+                        kahan_add(Is, Isc, s * s);
+                        stg(S.I, q, I); stg(S.Ic, q, Ic); stg(S.Is, q, Is); stg(S.Isc, q, Isc);
+                    } else {
+                        const R s = G.sv[rr];
+                        stg(S.I, q, G.Iv[rr] + s * U);
+                        stg(S.Is, q, G.Isv[rr] + s * s);
+                    }
                 }
             }
         }

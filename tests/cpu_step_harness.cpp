@@ -59,9 +59,12 @@ template <typename R, int RZ, int CLS, bool Q16, bool I16, int NW, bool SAFE>
 static void run_tile_s(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, const rtm::SimDesc<R> &S,
                        const rtm::StepView<R> &V, int x0, int z0)
 {
+    // forward steps as the store instance steps them (no imaging operands,
+    // no imaging code), backward steps as the full one
     for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
-            rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, true, SAFE>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            if (V.forward) rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, false, SAFE>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+            else rtm::tile_fetch<R, RZ, CLS, Q16, I16, NW, true, SAFE>(G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             rtm::tile_stage<R, RZ, CLS, NW>(L, G[ty * 64 + tx], tx, ty);
         }
     if (CLS)
@@ -72,7 +75,9 @@ static void run_tile_s(rtm::TileLds<R, RZ, NW> &L, rtm::TileRegs<R, RZ, NW> *G, 
     for (int ty = 0; ty < NW; ty++)
         for (int tx = 0; tx < 64; tx++) {
             const float t =
-                rtm::tile_points<R, RZ, CLS, Q16, I16, NW, true, SAFE>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
+                V.forward
+                    ? rtm::tile_points<R, RZ, CLS, Q16, I16, NW, 1, SAFE>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty)
+                    : rtm::tile_points<R, RZ, CLS, Q16, I16, NW, 2, SAFE>(L, G[ty * 64 + tx], S, V, x0, z0, tx, ty);
             m = t > m ? t : m;
         }
     commit_max(V, m);
