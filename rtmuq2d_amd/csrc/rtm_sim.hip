@@ -568,7 +568,9 @@ int Exec::init(int real_type, int max_sims, int max_tiles, int rows_per_thread, 
     sched = false;
     ninner = 0;
     h_n0.assign(cap_sims, 0);
-    HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    // the side stream is made when an interior list first needs it
+    // (set_inner): every stream of the process takes a share of the
+    // device's few hardware queues, and idle ones only crowd them
     HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     return 0;
@@ -617,6 +619,11 @@ int Exec::set_inner(const std::vector<int4> &items, hipStream_t s)
         return -1;
     }
     ninner = (int)items.size();
+    if (ninner && !side && interior == rtm::IN_COLUMNS) {
+        HIP_TRY(hipStreamSynchronize(s));
+        drop_graph();   // graphs captured without the fork
+        HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    }
     if (ninner) HIP_TRY(hipMemcpyAsync(d_inner, items.data(), sizeof(int4) * ninner, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ninner, &ninner, sizeof(int), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
