@@ -290,10 +290,11 @@ int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_si
  * (rtm_batch_last_timing).  Results do not depend on the lanes. */
 int rtm_batch_set_lanes(rtm_batch *batch, int lanes);
 int rtm_batch_get_lanes(rtm_batch *batch, int *lanes);
-/* Times one lane at the current group size, then two lanes at group sizes
- * of 1/8 to 5/8 of the size that fills the 224 MiB budget and three lanes at
- * 1/8 and 1/4 (balanced over a multiple of the lanes), each as a wave of its
- * first groups in flight at
+/* Times one lane at the current group size, then two lanes at each distinct
+ * balanced group size from 1/8 to 5/8 of the size that fills the 224 MiB
+ * budget and three lanes from 1/8 to 1/4 (at most about 13 sizes per lane
+ * count), each as a wave of its first groups in flight, on the host clock
+ * in two sweeps over the candidates (the slowest reading counts), at
  * the current shape, stepping forward steps that write the snapshot stores;
  * keeps the first within 2% of the least time per realization-step.  Row k
  * = lanes[k], group_sizes[k] (the largest group), us_per_sim_step[k]. */

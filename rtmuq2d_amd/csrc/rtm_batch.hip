@@ -797,8 +797,7 @@ static int time_wave(rtm_batch *q, int L, int steps, double *us)
         if (ex.run(warm, q->graph_block, st) || ex.set_n0(std::vector<int>(n, 0), st)) return -1;
     }
     // two timed rounds, the slower kept; each measured on the host clock
-    // from an idle device to the joined lanes' end (a wave's span can be
-    // read too short from events, once seen at two thirds of the real time)
+    // from an idle device to the joined lanes' end
     double worst = 0.0;
     for (int round = 0; round < 2; round++) {
         for (int j = 0; j < L; j++)
@@ -846,10 +845,19 @@ extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *l
     std::vector<Cand> cand;
     if (q->lanes != 1 && batch_set_lanes(q, 1)) return -1;
     cand.push_back({1, group_size(q), 0.0});
-    // (lanes, eighths of h): two lanes of h/8 .. 5h/8, three of h/8 .. h/4
-    const int tries[][2] = {{2, 1}, {2, 2}, {2, 3}, {2, 4}, {2, 5}, {3, 1}, {3, 2}};
+    // the distinct balanced group sizes for two lanes of h/8 .. 5h/8 and
+    // three lanes of h/8 .. h/4, in steps of 1 up to ranges of 13 sizes (two
+    // lanes of 5 and of 6 can differ by a few percent on 64 realizations of
+    // 1801 x 451), coarser beyond
+    std::vector<std::pair<int, int>> tries;
+    const int lo = std::max(1, (h + 7) / 8);
+    for (const int L : {2, 3}) {
+        const int hi = std::max(lo, L == 2 ? (5 * h + 7) / 8 : (h + 3) / 4);
+        const int stride = std::max(1, (hi - lo + 11) / 12);
+        for (int g = lo; g <= hi; g += stride) tries.push_back({L, g});
+    }
     for (const auto &t : tries) {
-        const int L = t[0], g = std::max(1, (h * t[1] + 7) / 8);
+        const int L = t.first, g = t.second;
         int ng = (N + g - 1) / g;
         ng = std::min(N, (ng + L - 1) / L * L);
         const int gb = (N + ng - 1) / ng;
@@ -859,14 +867,26 @@ extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *l
         if (!seen) cand.push_back({L, gb, 0.0});
     }
     const int g1 = q->group;
-    int k = 0;
-    for (Cand &c : cand) {
+    // two sweeps, the second in reverse order, each candidate's slower visit
+    // kept: a wave has now and then run a third faster than the same setting
+    // runs a migration (2 x 4, 2 x 7, 3 x 4 seen, once each and not again on
+    // their next visit), and one visit would adopt it
+    const int nc = (int)cand.size();
+    for (int v = 0; v < 2 * nc; v++) {
+        Cand &c = cand[v < nc ? v : 2 * nc - 1 - v];
         q->group = c.L == 1 ? g1 : (c.g >= N ? 0 : c.g);
         if (batch_set_lanes(q, c.L)) return -1;
         const int L = std::min(c.L, (int)q->ex.size());
-        const int rc = q->real == RTM_REAL_F64 ? time_wave<double>(q, L, steps, &c.t)
-                                               : time_wave<float>(q, L, steps, &c.t);
+        double t = 0.0;
+        const int rc = q->real == RTM_REAL_F64 ? time_wave<double>(q, L, steps, &t)
+                                               : time_wave<float>(q, L, steps, &t);
         if (rc) return -1;
+        c.t = std::max(c.t, t);
+    }
+    int k = 0;
+    for (const Cand &c : cand) {
+        q->group = c.L == 1 ? g1 : (c.g >= N ? 0 : c.g);
+        q->lanes = c.L;
         if (k < cap) {
             if (lanes) lanes[k] = c.L;
             if (group_sizes) group_sizes[k] = group_size(q);
