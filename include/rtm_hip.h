@@ -277,9 +277,11 @@ int rtm_batch_get_group(rtm_batch *batch, int *sims_per_group, int *groups);
 /* Shape-tunes each candidate group size (1/2 to 2 times the size that fills
  * the 224 MiB budget, each taken to the largest group of its balanced split,
  * and the whole batch) on group 0, timing forward steps that write
- * the snapshot stores (overwritten by the next migration), and keeps the
- * smallest group within 2% of the least time per realization-step; row k =
- * group_sizes[k], us_per_sim_step[k].  It runs one group at a time (lanes 1). */
+ * the snapshot stores (overwritten by the next migration); each group within
+ * 10% of the fastest is timed a second time at its tuned shape and the
+ * slower reading counts.  Keeps the smallest group within 2% of the least
+ * time per realization-step; row k = group_sizes[k], us_per_sim_step[k].
+ * It runs one group at a time (lanes 1). */
 int rtm_batch_autotune_group(rtm_batch *batch, int steps, int cap, int *group_sizes, double *us_per_sim_step,
                              int *n_out);
 /* Launch groups in flight at once (no reference counterpart): with lanes > 1
@@ -296,8 +298,9 @@ int rtm_batch_get_lanes(rtm_batch *batch, int *lanes);
  * count), each as a wave of its first groups in flight, on the host clock
  * in two sweeps over the candidates (the slowest reading counts), at
  * the current shape, stepping forward steps that write the snapshot stores;
- * keeps the first within 2% of the least time per realization-step.  Row k
- * = lanes[k], group_sizes[k] (the largest group), us_per_sim_step[k]. */
+ * keeps one lane if it is within 2% of the least time per realization-step,
+ * else the fastest.  Row k = lanes[k], group_sizes[k] (the largest group),
+ * us_per_sim_step[k]. */
 int rtm_batch_autotune_lanes(rtm_batch *batch, int steps, int cap, int *lanes, int *group_sizes,
                              double *us_per_sim_step, int *n_out);
 int rtm_batch_set_velocity(rtm_batch *batch, int sim, const float *vpad);   /* padded [X][Z] */

@@ -640,6 +640,17 @@ static int batch_autotune_t(rtm_batch *q, int steps, int cap, int *shapes, doubl
     return 0;
 }
 
+// The time per step of launch group 0 at the current shape, writing its
+// snapshot stores as a forward pass does.
+template <typename R>
+static int time_group0(rtm_batch *q, int steps, double *ms)
+{
+    const int flags = q->sims[0].ringmode ? 0 : rtm::F_SNAP;
+    std::vector<SimDesc<R>> d;
+    for (int s = 0; s < group_size(q); s++) d.push_back(q->sims[s].desc<R>());
+    return time_steps<R>(q->ex[0], d, steps, q->graph_block, q->stream, ms, flags);
+}
+
 extern "C" int rtm_batch_autotune(rtm_batch *q, int steps, int cap, int *shapes, double *ms_per_step, int *n_out)
 {
     if (!q || steps < 1) { rtm_set_error("rtm_batch_autotune: bad argument"); return -1; }
@@ -742,14 +753,30 @@ extern "C" int rtm_batch_autotune_group(rtm_batch *q, int steps, int cap, int *g
         const int rc = q->real == RTM_REAL_F64 ? batch_autotune_t<double>(q, steps, 0, nullptr, nullptr, nullptr, &t, true)
                                                : batch_autotune_t<float>(q, steps, 0, nullptr, nullptr, nullptr, &t, true);
         if (rc) return -1;
-        const double per = t * 1e3 / group_size(q);
+        timed.push_back({t * 1e3 / group_size(q), g, Shape{q->rz, q->interior, q->seg_rows}});
+    }
+    for (const Timed &t : timed) best_t = std::min(best_t, t.per);
+    // a second reading of each group within 10% of the fastest, at its tuned
+    // shape, the slower kept: a single timing now and then reads about a
+    // third fast (a group of 16 once at 2.13 us per realization-step against
+    // 3.25 on its next reading)
+    for (Timed &t : timed) {
+        if (t.per > best_t * 1.10) continue;
+        q->group = t.g >= N ? 0 : t.g;
+        if (batch_set_tiling(q, t.shape.rz, q->graph_block, t.shape.interior, t.shape.seg)) return -1;
+        double ms = 0.0;
+        const int rc = q->real == RTM_REAL_F64 ? time_group0<double>(q, steps, &ms) : time_group0<float>(q, steps, &ms);
+        if (rc) return -1;
+        t.per = std::max(t.per, ms * 1e3 / group_size(q));
+    }
+    best_t = 1e30;
+    for (const Timed &t : timed) {
         if (k < cap) {
-            if (group_sizes) group_sizes[k] = g;
-            if (us_per_sim_step) us_per_sim_step[k] = per;
+            if (group_sizes) group_sizes[k] = t.g;
+            if (us_per_sim_step) us_per_sim_step[k] = t.per;
         }
         k++;
-        timed.push_back({per, g, Shape{q->rz, q->interior, q->seg_rows}});
-        best_t = std::min(best_t, per);
+        best_t = std::min(best_t, t.per);
     }
     if (n_out) *n_out = k < cap ? k : cap;
     // the smallest group within 2% of the fastest: the timing leaves out the
@@ -826,9 +853,10 @@ static int time_wave(rtm_batch *q, int L, int steps, double *us)
 
 // Launch groups in flight at once: one lane at the current group size, then
 // two lanes at group sizes of 1/8 to 5/8 of the size that fills the cache
-// budget and three at 1/8 and 1/4 (each balanced over a multiple of the
+// budget and three at 1/8 to 1/4 (each balanced over a multiple of the
 // lanes), each timed as a wave of its first groups in flight at the current
-// shape; the first within 2% of the least time per realization-step wins.
+// shape; one lane wins while within 2% of the least time per
+// realization-step, else the fastest.
 extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *lanes, int *group_sizes,
                                         double *us_per_sim_step, int *n_out)
 {
@@ -897,12 +925,17 @@ extern "C" int rtm_batch_autotune_lanes(rtm_batch *q, int steps, int cap, int *l
     if (n_out) *n_out = k < cap ? k : cap;
     double best_t = 1e30;
     for (const Cand &c : cand) best_t = std::min(best_t, c.t);
+    // one lane while it is within 2% of the fastest (no streams beside the
+    // batch's own), else the fastest: among lanes, smaller groups timed
+    // within 2% of a larger one still ran whole migrations 1-2% slower
+    // (two lanes of 4 / 5 / 6 on 64 realizations of 1801 x 451)
     const Cand *win = &cand[0];
-    for (const Cand &c : cand)
-        if (c.t <= best_t * 1.02) {
-            win = &c;
-            break;
-        }
+    if (cand[0].t > best_t * 1.02)
+        for (const Cand &c : cand)
+            if (c.t == best_t) {
+                win = &c;
+                break;
+            }
     q->group = win->L == 1 ? g1 : (win->g >= N ? 0 : win->g);
     if (batch_set_lanes(q, win->L)) return -1;
     Shape sh{q->rz, q->interior, q->seg_rows};

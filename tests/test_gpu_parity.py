@@ -369,7 +369,8 @@ def test_batch_group_autotune_keeps_results(gpu_available):
 
 def test_batch_lanes_autotune_keeps_results(gpu_available):
     """The lanes autotuner times one lane at the current group size, then two
-    lanes; it keeps the first row within 2% of the fastest, the images stay
+    and three lanes; it keeps one lane while that is within 2% of the fastest
+    row, else the fastest row, the images stay
     bit-identical to the oracle's, and a new batch of the same grids starts
     with the tuned lanes and group size."""
     say("[gpu] batch lanes autotune")
@@ -390,7 +391,8 @@ def test_batch_lanes_autotune_keeps_results(gpu_available):
         assert rows[0]["lanes"] == 1 and len(rows) >= 2 and all(r["lanes"] in (2, 3) for r in rows[1:])
         assert all(r["us_per_sim_step"] > 0 for r in rows)
         fastest = min(r["us_per_sim_step"] for r in rows)
-        win = next(r for r in rows if r["us_per_sim_step"] <= fastest * 1.02)
+        win = rows[0] if rows[0]["us_per_sim_step"] <= fastest * 1.02 else \
+            next(r for r in rows if r["us_per_sim_step"] == fastest)
         assert b.lanes() == win["lanes"] and b.group()[0] == win["group"]
         state = (b.lanes(), b.group(), b.tiling())
         b.zero_images()
